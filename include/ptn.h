@@ -1,0 +1,162 @@
+/*
+ * ptn.h -- C ABI of the gfx950 (MI355X) library behind pytenet_amd.
+ *
+ * The reference (cmendl/pytenet 1.3.0) is pure Python/NumPy; its swap point is
+ * the set of local operations that the sweep drivers call
+ * (pytenet/tdvp.py:223-238, pytenet/dmrg.py:181-189 and the functions listed
+ * per entry point below). Each entry point here replaces one of them.
+ *
+ * Conventions
+ *   - All tensors are device pointers to row-major complex128 (interleaved
+ *     re, im; the layout of torch.complex128 and numpy.complex128) unless
+ *     stated otherwise. Shapes follow the reference:
+ *       MPS tensor A (Dl, d, Dr); MPO tensor W (Ml, d_out, d_in, Mr);
+ *       left block L (Dl_ket, Ml, Dl_bra); right block R (Dr_ket, Mr, Dr_bra).
+ *   - `precision` selects the arithmetic of the dense contractions:
+ *       PTN_FP64   complex128 operands, FP64 MFMA (the parity mode);
+ *       PTN_FP32   complex64 operands, FP32 MFMA, FP32 accumulation;
+ *       PTN_BF16X3 each FP32 operand split into two bf16 terms, three bf16
+ *                  MFMA products (hi*hi + hi*lo + lo*hi), FP32 accumulation.
+ *     Inputs and outputs stay complex128 in every mode; Krylov vectors,
+ *     inner products, norms and the small tridiagonal problems are FP64.
+ *   - Every call is asynchronous on `stream` (a hipStream_t), returns a status
+ *     code and never throws. Scratch memory is passed in by the caller
+ *     (`ws`, `ws_bytes`); the matching *_workspace() function gives the size.
+ *   - Scalars written by the library (norms, energies, status words) go to
+ *     device memory given by the caller, so no call synchronises the stream.
+ */
+#ifndef PTN_H
+#define PTN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PTN_OK               0
+#define PTN_ERR_ARG          1   /* invalid shape or argument */
+#define PTN_ERR_WORKSPACE    2   /* workspace too small */
+#define PTN_ERR_HIP          3   /* HIP runtime error (see ptn_last_hip_error) */
+#define PTN_ERR_UNSUPPORTED  4   /* precision / shape combination not built */
+
+/* precision modes */
+#define PTN_FP64    0
+#define PTN_FP32    1
+#define PTN_BF16X3  2
+
+/* local operator kinds for the Krylov solvers */
+#define PTN_OP_HEFF 0   /* L.W.A.R (one or two sites; for two sites pass d = d0*d1 and W = merged pair) */
+#define PTN_OP_BOND 1   /* L.C.R ("zero-site" bond operator) */
+
+/* Krylov status words (int32[4] in device memory):
+ *   [0] number of Lanczos vectors used (numiter, or j+1 after a breakdown at step j)
+ *   [1] j+1 if beta[j] < 100*n*eps stopped the iteration at step j, else 0
+ *   [2] 1 if a norm or inner product was not finite, else 0
+ *   [3] reserved (0) */
+
+int ptn_version(void);
+const char* ptn_status_string(int status);
+const char* ptn_last_hip_error(void);
+
+/* ---- K1: effective Hamiltonian, replaces pytenet/chain_ops.py:237-279 apply_local_hamiltonian(a, w, l, r)
+ * out[lb, sb, rb] = sum L[l,m,lb] W[m,sb,s,m'] A[l,s,r] R[r,m',rb] */
+size_t ptn_heff_workspace(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr);
+int ptn_heff_apply(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr,
+                   const void* l, const void* w, const void* r, const void* a, void* out,
+                   void* ws, size_t ws_bytes, void* stream);
+
+/* ---- K2: bond operator, replaces pytenet/chain_ops.py:282-317 apply_local_bond_contraction(c, l, r)
+ * out[lb, rb] = sum L[l,m,lb] C[l,r] R[r,m,rb] */
+size_t ptn_bond_workspace(int precision, int64_t Dl, int64_t Dr, int64_t M);
+int ptn_bond_apply(int precision, int64_t Dl, int64_t Dr, int64_t M,
+                   const void* l, const void* r, const void* c, void* out,
+                   void* ws, size_t ws_bytes, void* stream);
+
+/* ---- K3: left environment step, replaces pytenet/chain_ops.py:60-99 contraction_operator_step_left(a, b, w, l)
+ * out[r, m', rb] = sum A[l,s,r] W[m,sb,s,m'] L[l,m,lb] conj(B[lb,sb,rb]); A and B are (Dl, d, Dr) */
+size_t ptn_env_workspace(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr);
+int ptn_env_left(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr,
+                 const void* a, const void* b, const void* w, const void* l, void* out,
+                 void* ws, size_t ws_bytes, void* stream);
+
+/* ---- K4: right environment step, replaces pytenet/chain_ops.py:16-57 contraction_operator_step_right(a, b, w, r)
+ * out[l, m, lb] = sum A[l,s,r] W[m,sb,s,m'] R[r,m',rb] conj(B[lb,sb,rb]) */
+int ptn_env_right(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr,
+                  const void* a, const void* b, const void* w, const void* r, void* out,
+                  void* ws, size_t ws_bytes, void* stream);
+
+/* ---- K5/K7: strided complex128 matrix product, replaces the tensordot/einsum
+ * pair products pytenet/mps.py:528-535 (merge), pytenet/tdvp.py:84,112 and
+ * pytenet/mps.py:472,490 (absorbing R or C into a neighbour).
+ * C[m*c_m + n*c_n] (+)= sum_k opA(m,k) opB(k,n), opA(m,k) = A[m*a_m + k*a_k] (conj if conjA),
+ * opB(k,n) = B[k*b_k + n*b_n] (conj if conjB). Strides in complex elements. */
+size_t ptn_zgemm_workspace(int64_t M, int64_t N, int64_t K);
+int ptn_zgemm(int64_t M, int64_t N, int64_t K,
+              const void* A, int64_t a_m, int64_t a_k, int conjA,
+              const void* B, int64_t b_k, int64_t b_n, int conjB,
+              void* C, int64_t c_m, int64_t c_n, int accumulate,
+              void* ws, size_t ws_bytes, void* stream);
+
+/* ---- K8-K10: device Lanczos with the local operator applied in place of the
+ * reference's matvec closure (pytenet/tdvp.py:223-238, pytenet/dmrg.py:181-189). */
+typedef struct {
+  int kind;                 /* PTN_OP_HEFF or PTN_OP_BOND */
+  int precision;            /* PTN_FP64 / PTN_FP32 / PTN_BF16X3 */
+  int64_t Dl, d, Dr, Ml, Mr;/* bond operator: d = 1, Ml = Mr = M */
+  const void* l;            /* L (Dl, Ml, Dl) */
+  const void* w;            /* W (Ml, d, d, Mr); NULL for the bond operator */
+  const void* r;            /* R (Dr, Mr, Dr) */
+} ptn_local_op;
+
+size_t ptn_krylov_workspace(const ptn_local_op* op, int numiter);
+
+/* expm(dt*H) v by `numiter` Lanczos steps, replaces pytenet/krylov.py:122-136
+ * (Hermitian branch) with pytenet/krylov.py:12-57 and :142-150. dt is complex.
+ * alpha_beta (device double[2*numiter], optional): alpha then beta. */
+int ptn_krylov_expm(const ptn_local_op* op, const void* v, double dt_re, double dt_im,
+                    int numiter, void* out, int32_t* info, double* alpha_beta,
+                    void* ws, size_t ws_bytes, void* stream);
+
+/* lowest Ritz pair, replaces pytenet/krylov.py:110-119 with numeig = 1:
+ * out = Ritz vector, *energy (device double) = lowest Ritz value. */
+int ptn_krylov_eigh(const ptn_local_op* op, const void* v, int numiter, void* out,
+                    double* energy, int32_t* info, double* alpha_beta,
+                    void* ws, size_t ws_bytes, void* stream);
+
+/* ---- K11: Householder QR with LAPACK's zgeqrf + zungqr conventions (beta =
+ * -sign(Re alpha) * |(alpha, x)|, tau = 0 for an exactly zero sub-column),
+ * replaces np.linalg.qr(mode="reduced") at pytenet/block_sparse_util.py:157.
+ * A (m x n, row stride lda) -> Q (m x k, ldq), R (k x n, ldr), k = min(m, n). */
+size_t ptn_qr_workspace(int64_t m, int64_t n);
+int ptn_qr(int64_t m, int64_t n, const void* a, int64_t lda, void* q, int64_t ldq,
+           void* r, int64_t ldr, void* ws, size_t ws_bytes, void* stream);
+
+/* ---- K12: thin SVD A = U diag(s) Vh (one-sided Jacobi, FP64), singular values
+ * in descending order; replaces np.linalg.svd(full_matrices=False) at
+ * pytenet/block_sparse_util.py:294. U (m x k, ldu), s (k doubles), Vh (k x n, ldvh).
+ * info (device int32[4]): [0] sweeps used, [1] 1 if not converged, [2] non-finite flag. */
+size_t ptn_svd_workspace(int64_t m, int64_t n);
+int ptn_svd(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t ldu,
+            double* s, void* vh, int64_t ldvh, int32_t* info,
+            void* ws, size_t ws_bytes, void* stream);
+
+/* ---- K14: multiply a (n0, n1, n2) complex tensor by real factors along axis 0
+ * or axis 2, in place; replaces the broadcasts at pytenet/mps.py:558-566. */
+int ptn_scale_axis(int64_t n0, int64_t n1, int64_t n2, void* a, const double* s, int axis,
+                   void* stream);
+
+/* ---- a17: quantum-number sparsity of a 3-index tensor (n0, n1, n2):
+ * *out (device double) = max |a[i,j,k]| over entries with q0[i]+q1[j]+q2[k] != 0
+ * (pytenet/block_sparse_util.py:47-53); with zero_them != 0 those entries are
+ * also set to zero (block_sparse_util.py:56-64). q* are device int64 arrays. */
+int ptn_qsparse3(int64_t n0, int64_t n1, int64_t n2, const int64_t* q0, const int64_t* q1,
+                 const int64_t* q2, void* a, int zero_them, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PTN_H */

@@ -1,0 +1,453 @@
+"""
+Device layer: wraps the C ABI of libptn_hip.so for torch device tensors.
+
+PyTorch is used for device memory (its caching allocator owns every buffer),
+for the current HIP stream, and for plain data movement (gathers/scatters of
+quantum-number sectors). All arithmetic of the sweep runs in the library's
+own gfx950 kernels. Every tensor handed to the library is complex128,
+contiguous and resident on the current GPU.
+"""
+
+import ctypes
+import warnings
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import c_p
+
+PRECISIONS = {"fp64": _lib.PTN_FP64, "fp32": _lib.PTN_FP32, "bf16x3": _lib.PTN_BF16X3}
+
+
+class PtnError(RuntimeError):
+    """A call into libptn_hip.so failed."""
+
+
+def _check(status, what):
+    if status != _lib.PTN_OK:
+        lib = _lib.load()
+        msg = lib.ptn_status_string(status).decode()
+        if status == _lib.PTN_ERR_HIP:
+            msg += ": " + lib.ptn_last_hip_error().decode()
+        raise PtnError(f"{what} failed ({status}: {msg})")
+
+
+def precision_code(precision):
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}")
+    code = PRECISIONS[precision]
+    if code == _lib.PTN_BF16X3:
+        raise NotImplementedError("the split-bf16 mode is not built in this version")
+    return code
+
+
+def device():
+    """The current GPU; raises when none is visible (no CPU fallback)."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("pytenet_amd needs a ROCm GPU; none is visible")
+    _lib.load()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream():
+    return c_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    return c_p(t.data_ptr())
+
+
+class _Workspace:
+    """Grow-only scratch buffer per device, reused across calls (stream-ordered)."""
+
+    def __init__(self):
+        self.buf = {}
+
+    def get(self, nbytes):
+        dev = torch.cuda.current_device()
+        cur = self.buf.get(dev)
+        if cur is None or cur.numel() < nbytes:
+            nb = max(int(nbytes), 1 << 20)
+            if cur is not None:
+                nb = max(nb, int(cur.numel() * 1.25))
+            cur = torch.empty(nb, dtype=torch.uint8, device=torch.device("cuda", dev))
+            self.buf[dev] = cur
+        return cur
+
+
+_ws = _Workspace()
+
+
+def workspace(nbytes):
+    return _ws.get(nbytes)
+
+
+def as_dev(x):
+    """complex128, contiguous, on the current GPU (uploads NumPy arrays)."""
+    dev = device()
+    if isinstance(x, np.ndarray) or np.isscalar(x):
+        return torch.as_tensor(np.ascontiguousarray(x, dtype=np.complex128), device=dev)
+    if not isinstance(x, torch.Tensor):
+        x = torch.as_tensor(np.asarray(x))
+    if x.device != dev:
+        x = x.to(dev)
+    if x.dtype != torch.complex128:
+        x = x.to(torch.complex128)
+    return x.contiguous()
+
+
+def to_host(x):
+    if isinstance(x, torch.Tensor):
+        return x.detach().cpu().numpy()
+    return np.asarray(x)
+
+
+# ---------------------------------------------------------------------------
+# local contractions (K1-K4)
+
+def heff_apply(a, w, l, r, precision="fp64"):
+    """out[lb,sb,rb] = sum L[l,m,lb] W[m,sb,s,m'] A[l,s,r] R[r,m',rb]."""
+    a, w, l, r = as_dev(a), as_dev(w), as_dev(l), as_dev(r)
+    Dl, d, Dr = a.shape
+    Ml, Mr = w.shape[0], w.shape[3]
+    _shape_check(w.shape == (Ml, d, d, Mr) and l.shape == (Dl, Ml, Dl) and r.shape == (Dr, Mr, Dr),
+                 "apply_local_hamiltonian", a, w, l, r)
+    prec = precision_code(precision)
+    lib = _lib.load()
+    out = torch.empty((Dl, d, Dr), dtype=torch.complex128, device=a.device)
+    nb = lib.ptn_heff_workspace(prec, Dl, d, Dr, Ml, Mr)
+    ws = workspace(nb)
+    _check(lib.ptn_heff_apply(prec, Dl, d, Dr, Ml, Mr, ptr(l), ptr(w), ptr(r), ptr(a), ptr(out),
+                              ptr(ws), ws.numel(), stream()), "ptn_heff_apply")
+    return out
+
+
+def bond_apply(c, l, r, precision="fp64"):
+    """out[lb,rb] = sum L[l,m,lb] C[l,r] R[r,m,rb]."""
+    c, l, r = as_dev(c), as_dev(l), as_dev(r)
+    Dl, Dr = c.shape
+    M = l.shape[1]
+    _shape_check(l.shape == (Dl, M, Dl) and r.shape == (Dr, M, Dr),
+                 "apply_local_bond_contraction", c, l, r)
+    prec = precision_code(precision)
+    lib = _lib.load()
+    out = torch.empty((Dl, Dr), dtype=torch.complex128, device=c.device)
+    ws = workspace(lib.ptn_bond_workspace(prec, Dl, Dr, M))
+    _check(lib.ptn_bond_apply(prec, Dl, Dr, M, ptr(l), ptr(r), ptr(c), ptr(out), ptr(ws),
+                              ws.numel(), stream()), "ptn_bond_apply")
+    return out
+
+
+def env_left(a, b, w, l, precision="fp64"):
+    """out[r,m',rb] = sum A[l,s,r] W[m,sb,s,m'] L[l,m,lb] conj(B[lb,sb,rb])."""
+    a, b, w, l = as_dev(a), as_dev(b), as_dev(w), as_dev(l)
+    Dl, d, Dr = a.shape
+    Ml, Mr = w.shape[0], w.shape[3]
+    _shape_check(b.shape == a.shape and w.shape == (Ml, d, d, Mr) and l.shape == (Dl, Ml, Dl),
+                 "contraction_operator_step_left", a, b, w, l)
+    prec = precision_code(precision)
+    lib = _lib.load()
+    out = torch.empty((Dr, Mr, Dr), dtype=torch.complex128, device=a.device)
+    ws = workspace(lib.ptn_env_workspace(prec, Dl, d, Dr, Ml, Mr))
+    _check(lib.ptn_env_left(prec, Dl, d, Dr, Ml, Mr, ptr(a), ptr(b), ptr(w), ptr(l), ptr(out),
+                            ptr(ws), ws.numel(), stream()), "ptn_env_left")
+    return out
+
+
+def env_right(a, b, w, r, precision="fp64"):
+    """out[l,m,lb] = sum A[l,s,r] W[m,sb,s,m'] R[r,m',rb] conj(B[lb,sb,rb])."""
+    a, b, w, r = as_dev(a), as_dev(b), as_dev(w), as_dev(r)
+    Dl, d, Dr = a.shape
+    Ml, Mr = w.shape[0], w.shape[3]
+    _shape_check(b.shape == a.shape and w.shape == (Ml, d, d, Mr) and r.shape == (Dr, Mr, Dr),
+                 "contraction_operator_step_right", a, b, w, r)
+    prec = precision_code(precision)
+    lib = _lib.load()
+    out = torch.empty((Dl, Ml, Dl), dtype=torch.complex128, device=a.device)
+    ws = workspace(lib.ptn_env_workspace(prec, Dl, d, Dr, Ml, Mr))
+    _check(lib.ptn_env_right(prec, Dl, d, Dr, Ml, Mr, ptr(a), ptr(b), ptr(w), ptr(r), ptr(out),
+                             ptr(ws), ws.numel(), stream()), "ptn_env_right")
+    return out
+
+
+def _shape_check(ok, what, *ts):
+    if not ok:
+        raise AssertionError(f"{what}: incompatible shapes {[tuple(t.shape) for t in ts]}")
+
+
+# ---------------------------------------------------------------------------
+# strided complex products (K5, K7)
+
+def matmul(A, B, conjA=False, conjB=False, out=None, accumulate=False):
+    """C = op(A) @ op(B) for 2-D complex128 device tensors of any strides."""
+    assert A.dim() == 2 and B.dim() == 2 and A.shape[1] == B.shape[0]
+    assert A.dtype == torch.complex128 and B.dtype == torch.complex128
+    M, K = A.shape
+    N = B.shape[1]
+    if out is None:
+        out = torch.empty((M, N), dtype=torch.complex128, device=A.device)
+        accumulate = False
+    lib = _lib.load()
+    _check(lib.ptn_zgemm(M, N, K, ptr(A), A.stride(0), A.stride(1), int(conjA),
+                         ptr(B), B.stride(0), B.stride(1), int(conjB),
+                         ptr(out), out.stride(0), out.stride(1), int(accumulate),
+                         None, 0, stream()), "ptn_zgemm")
+    return out
+
+
+def merge_pair(a0, a1):
+    """Theta[l, s0*d1+s1, r] = sum_k a0[l,s0,k] a1[k,s1,r]."""
+    a0, a1 = as_dev(a0), as_dev(a1)
+    Dl, d0, k = a0.shape
+    k1, d1, Dr = a1.shape
+    assert k == k1
+    t = matmul(a0.reshape(Dl * d0, k), a1.reshape(k, d1 * Dr))
+    return t.reshape(Dl, d0 * d1, Dr)
+
+
+# ---------------------------------------------------------------------------
+# Krylov solvers (K8-K10)
+
+def _local_op(kind, precision, l, w, r, Dl, d, Dr):
+    op = _lib.LocalOp()
+    op.kind = kind
+    op.precision = precision_code(precision)
+    op.Dl, op.d, op.Dr = Dl, d, Dr
+    op.Ml = l.shape[1]
+    op.Mr = r.shape[1]
+    op.l = l.data_ptr()
+    op.w = w.data_ptr() if w is not None else None
+    op.r = r.data_ptr()
+    return op
+
+
+def krylov_local(kind, v, l, w, r, numiter, mode, dt=0.0, precision="fp64", info=None):
+    """Lanczos on the local operator (PTN_OP_HEFF or PTN_OP_BOND):
+    mode "expm" -> expm(dt*H) v; mode "eigh" -> (lowest Ritz vector, energy tensor).
+    `info` (int32[4] device tensor) receives the status words; returns
+    (out, energy or None, info)."""
+    v, l, r = as_dev(v), as_dev(l), as_dev(r)
+    if w is not None:
+        w = as_dev(w)
+    if kind == _lib.PTN_OP_HEFF:
+        Dl, d, Dr = v.shape
+    else:
+        (Dl, Dr), d = v.shape, 1
+    op = _local_op(kind, precision, l, w, r, Dl, d, Dr)
+    lib = _lib.load()
+    nb = lib.ptn_krylov_workspace(ctypes.byref(op), int(numiter))
+    ws = workspace(nb)
+    out = torch.empty_like(v)
+    if info is None:
+        info = torch.zeros(4, dtype=torch.int32, device=v.device)
+    if mode == "expm":
+        dt = complex(dt)
+        _check(lib.ptn_krylov_expm(ctypes.byref(op), ptr(v), dt.real, dt.imag, int(numiter),
+                                   ptr(out), ptr(info), None, ptr(ws), ws.numel(), stream()),
+               "ptn_krylov_expm")
+        return out, None, info
+    energy = torch.zeros(1, dtype=torch.float64, device=v.device)
+    _check(lib.ptn_krylov_eigh(ctypes.byref(op), ptr(v), int(numiter), ptr(out), ptr(energy),
+                               ptr(info), None, ptr(ws), ws.numel(), stream()), "ptn_krylov_eigh")
+    return out, energy, info
+
+
+def report_krylov_status(info_rows):
+    """Emit the reference's breakdown warnings (krylov.py:44-50) and raise on
+    non-finite values or a zero start vector (krylov.py:30), from a host copy
+    of the status words (rows of 4 int32)."""
+    for row in np.asarray(info_rows).reshape(-1, 4):
+        if row[3]:
+            raise AssertionError("Lanczos start vector has norm zero")
+        if row[2] & 1:
+            raise FloatingPointError("non-finite value encountered during Lanczos iteration")
+        if row[1]:
+            warnings.warn(f"beta[{row[1] - 1}] ~= 0 encountered during Lanczos iteration.",
+                          RuntimeWarning)
+
+
+# ---------------------------------------------------------------------------
+# QR / SVD (K11, K12) and elementwise helpers
+
+def qr(a):
+    """Reduced Householder QR of a 2-D complex128 device tensor (LAPACK conventions)."""
+    a = as_dev(a)
+    m, n = a.shape
+    k = min(m, n)
+    q = torch.empty((m, k), dtype=torch.complex128, device=a.device)
+    r = torch.empty((k, n), dtype=torch.complex128, device=a.device)
+    if m == 0 or n == 0:
+        return q, r
+    lib = _lib.load()
+    ws = workspace(lib.ptn_qr_workspace(m, n))
+    _check(lib.ptn_qr(m, n, ptr(a), a.stride(0), ptr(q), k, ptr(r), n, ptr(ws), ws.numel(),
+                      stream()), "ptn_qr")
+    return q, r
+
+
+def svd(a):
+    """Thin SVD of a 2-D complex128 device tensor: (u, s, vh), s descending (device float64).
+    Synchronises the stream (the Jacobi sweeps are checked for convergence on the host)."""
+    a = as_dev(a)
+    m, n = a.shape
+    k = min(m, n)
+    u = torch.empty((m, k), dtype=torch.complex128, device=a.device)
+    vh = torch.empty((k, n), dtype=torch.complex128, device=a.device)
+    s = torch.empty(k, dtype=torch.float64, device=a.device)
+    if k == 0:
+        return u, s, vh
+    lib = _lib.load()
+    ws = workspace(lib.ptn_svd_workspace(m, n))
+    info = torch.zeros(4, dtype=torch.int32, device=a.device)
+    _check(lib.ptn_svd(m, n, ptr(a), a.stride(0), ptr(u), k, ptr(s), ptr(vh), n, ptr(info),
+                       ptr(ws), ws.numel(), stream()), "ptn_svd")
+    inf = info.cpu().numpy()
+    if inf[2]:
+        raise FloatingPointError("non-finite singular value")
+    if inf[1]:
+        raise np.linalg.LinAlgError("SVD did not converge")
+    return u, s, vh
+
+
+def scale_axis(t, s, axis):
+    """In place: t (n0, n1, n2) *= s along axis 0 or 2 (s real float64 device tensor)."""
+    assert t.is_contiguous() and t.dtype == torch.complex128
+    s = s.to(device=t.device, dtype=torch.float64).contiguous()
+    lib = _lib.load()
+    n0, n1, n2 = t.shape
+    _check(lib.ptn_scale_axis(n0, n1, n2, ptr(t), ptr(s), axis, stream()), "ptn_scale_axis")
+    return t
+
+
+def _qnums3(t, qnums):
+    qs = [np.asarray(q, dtype=np.int64).reshape(-1) for q in qnums]
+    shape = tuple(len(q) for q in qs)
+    assert shape == tuple(t.shape), "quantum numbers must match the tensor shape"
+    return [torch.as_tensor(q, device=t.device) for q in qs]
+
+
+def _as3(t, qnums):
+    """View a tensor with len(qnums) axes as 3 axes (merging trailing/leading axes)."""
+    qs = [np.asarray(q, dtype=np.int64).reshape(-1) for q in qnums]
+    if len(qs) == 3:
+        return t, qs
+    if len(qs) == 2:
+        return t.reshape(t.shape[0], 1, t.shape[1]), [qs[0], np.zeros(1, np.int64), qs[1]]
+    if len(qs) == 4:
+        mid = np.add.outer(qs[1], qs[2]).reshape(-1)
+        return t.reshape(t.shape[0], t.shape[1] * t.shape[2], t.shape[3]), [qs[0], mid, qs[3]]
+    raise ValueError("quantum-number checks support 2, 3 or 4 axes")
+
+
+def qsparse_violation(t, qnums):
+    """max |entry| over entries whose quantum numbers do not sum to zero."""
+    t = as_dev(t)
+    t3, qs = _as3(t, qnums)
+    q0, q1, q2 = _qnums3(t3, qs)
+    out = torch.zeros(1, dtype=torch.float64, device=t.device)
+    lib = _lib.load()
+    _check(lib.ptn_qsparse3(*t3.shape, ptr(q0), ptr(q1), ptr(q2), ptr(t3), 0, ptr(out),
+                            stream()), "ptn_qsparse3")
+    return float(out.item())
+
+
+def enforce_qsparsity(t, qnums):
+    """Zero, in place, the entries of a complex128 device tensor whose quantum numbers do not sum to zero."""
+    assert isinstance(t, torch.Tensor) and t.dtype == torch.complex128 and t.is_contiguous()
+    t3, qs = _as3(t, qnums)
+    q0, q1, q2 = _qnums3(t3, qs)
+    out = torch.zeros(1, dtype=torch.float64, device=t.device)
+    lib = _lib.load()
+    _check(lib.ptn_qsparse3(*t3.shape, ptr(q0), ptr(q1), ptr(q2), ptr(t3), 1, ptr(out),
+                            stream()), "ptn_qsparse3")
+
+
+# ---------------------------------------------------------------------------
+# block-sparse factorisations (block_sparse_util.py:106-180, 244-319)
+
+def _sectors(q0, q1):
+    """Host bookkeeping: stable sort permutations and the common sectors."""
+    p0 = np.argsort(q0, kind="mergesort")
+    p1 = np.argsort(q1, kind="mergesort")
+    s0, s1 = q0[p0], q1[p1]
+    common = np.intersect1d(q0, q1)
+    out = []
+    for qn in common:
+        r = np.flatnonzero(s0 == qn)
+        c = np.flatnonzero(s1 == qn)
+        out.append((qn, p0[r[0]:r[-1] + 1], p1[c[0]:c[-1] + 1]))
+    return out
+
+
+def block_sparse_qr(a, q0, q1):
+    """Per-sector QR; the sector blocks are gathered, factorised on the GPU and
+    scattered back in sector order (the column order of the reference)."""
+    a = as_dev(a)
+    assert a.dim() == 2 and len(q0) == a.shape[0] and len(q1) == a.shape[1]
+    q0 = np.asarray(q0)
+    q1 = np.asarray(q1)
+    if not np.any(q0) and not np.any(q1):
+        # one sector holding the whole matrix (zero quantum numbers)
+        q, r = qr(a)
+        return q, r, np.zeros(q.shape[1], dtype=q0.dtype)
+    secs = _sectors(q0, q1)
+    if not secs:
+        if qsparse_violation(a, [np.zeros(a.shape[0], np.int64), np.zeros(a.shape[1], np.int64)]) != 0:
+            raise AssertionError("no common quantum numbers, but the matrix is not zero")
+        q = torch.zeros((a.shape[0], 1), dtype=torch.complex128, device=a.device)
+        q[0, 0] = 1
+        r = torch.zeros((1, a.shape[1]), dtype=torch.complex128, device=a.device)
+        return q, r, q0[:1]
+    kmax = min(a.shape)
+    q = torch.zeros((a.shape[0], kmax), dtype=torch.complex128, device=a.device)
+    r = torch.zeros((kmax, a.shape[1]), dtype=torch.complex128, device=a.device)
+    qi = np.zeros(kmax, dtype=q0.dtype)
+    k = 0
+    for qn, rows, cols in secs:
+        ri = torch.as_tensor(rows, device=a.device)
+        ci = torch.as_tensor(cols, device=a.device)
+        blk = a.index_select(0, ri).index_select(1, ci).contiguous()
+        qs, rs = qr(blk)
+        kk = qs.shape[1]
+        q[ri, k:k + kk] = qs
+        r[k:k + kk, ci] = rs
+        qi[k:k + kk] = qn
+        k += kk
+    return q[:, :k].contiguous(), r[:k, :].contiguous(), qi[:k]
+
+
+def block_sparse_svd(a, q0, q1):
+    """Per-sector SVD: (u, s, v, q) with s a host float64 array (sector order,
+    descending within a sector) and u, v device tensors."""
+    a = as_dev(a)
+    assert a.dim() == 2 and len(q0) == a.shape[0] and len(q1) == a.shape[1]
+    q0 = np.asarray(q0)
+    q1 = np.asarray(q1)
+    if not np.any(q0) and not np.any(q1):
+        u, s, vh = svd(a)
+        return u, s.cpu().numpy(), vh, np.zeros(len(s), dtype=q0.dtype)
+    secs = _sectors(q0, q1)
+    if not secs:
+        u = torch.zeros((a.shape[0], 1), dtype=torch.complex128, device=a.device)
+        if a.shape[0] > 0:
+            u[0, 0] = 1
+        v = torch.zeros((1, a.shape[1]), dtype=torch.complex128, device=a.device)
+        return u, np.zeros(1), v, q0[:1]
+    kmax = min(a.shape)
+    u = torch.zeros((a.shape[0], kmax), dtype=torch.complex128, device=a.device)
+    v = torch.zeros((kmax, a.shape[1]), dtype=torch.complex128, device=a.device)
+    s_parts, qi = [], np.zeros(kmax, dtype=q0.dtype)
+    k = 0
+    for qn, rows, cols in secs:
+        ri = torch.as_tensor(rows, device=a.device)
+        ci = torch.as_tensor(cols, device=a.device)
+        blk = a.index_select(0, ri).index_select(1, ci).contiguous()
+        us, ss, vs = svd(blk)
+        kk = len(ss)
+        u[ri, k:k + kk] = us
+        v[k:k + kk, ci] = vs
+        s_parts.append(ss)
+        qi[k:k + kk] = qn
+        k += kk
+    s = torch.cat(s_parts).cpu().numpy()
+    return u[:, :k].contiguous(), s, v[:k, :].contiguous(), qi[:k]

@@ -1,0 +1,207 @@
+// extern "C" entry points (declared in include/ptn.h) and the small
+// elementwise kernels that do not belong to a larger file.
+#include <stdio.h>
+#include "common.h"
+#include "ops.h"
+
+namespace ptn {
+
+size_t krylov_ws_bytes(const ptn_local_op* op, int numiter);
+int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, double dt_im,
+               int numiter, void* out, double* energy, int32_t* info, double* alpha_beta,
+               void* ws, size_t ws_bytes, hipStream_t s);
+size_t qr_ws_bytes(int64_t m, int64_t n);
+int qr_run(int64_t m, int64_t n, const void* a, int64_t lda, void* q, int64_t ldq, void* r,
+           int64_t ldr, void* ws, size_t ws_bytes, hipStream_t s);
+size_t svd_ws_bytes(int64_t m, int64_t n);
+int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t ldu, double* s,
+            void* vh, int64_t ldvh, int32_t* info, void* ws, size_t ws_bytes, hipStream_t st);
+
+static thread_local hipError_t g_last_hip = hipSuccess;
+void ptn_set_last_hip_error(hipError_t e) { g_last_hip = e; }
+
+// a (n0, n1, n2) *= s along axis 0 (s[i]) or axis 2 (s[k])
+__global__ void k_scale_axis(int64_t n0, int64_t n1, int64_t n2, double2* __restrict__ a,
+                             const double* __restrict__ s, int axis) {
+  const int64_t total = n0 * n1 * n2;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const double f = axis == 0 ? s[t / (n1 * n2)] : s[t % n2];
+    double2 v = a[t];
+    a[t] = make_double2(v.x * f, v.y * f);
+  }
+}
+
+// max |a| over entries whose three quantum numbers do not sum to zero
+// (optionally zeroing them); per-block maxima combined by an atomic max on the
+// bit pattern, which orders like the value for non-negative doubles
+__global__ void k_qsparse3(int64_t n0, int64_t n1, int64_t n2, const int64_t* __restrict__ q0,
+                           const int64_t* __restrict__ q1, const int64_t* __restrict__ q2,
+                           double2* __restrict__ a, int zero_them,
+                           unsigned long long* __restrict__ out) {
+  __shared__ double red[256];
+  double m = 0;
+  const int64_t total = n0 * n1 * n2;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / (n1 * n2), j = (t / n2) % n1, k = t % n2;
+    if (q0[i] + q1[j] + q2[k] != 0) {
+      double2 v = a[t];
+      m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+      if (zero_them) a[t] = make_double2(0.0, 0.0);
+    }
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+}  // namespace ptn
+
+using namespace ptn;
+
+extern "C" {
+
+int ptn_version(void) { return 100; }
+
+const char* ptn_status_string(int status) {
+  switch (status) {
+    case PTN_OK: return "ok";
+    case PTN_ERR_ARG: return "invalid shape or argument";
+    case PTN_ERR_WORKSPACE: return "workspace too small";
+    case PTN_ERR_HIP: return "HIP runtime error";
+    case PTN_ERR_UNSUPPORTED: return "unsupported precision or shape";
+  }
+  return "unknown status";
+}
+
+const char* ptn_last_hip_error(void) { return hipGetErrorString(g_last_hip); }
+
+size_t ptn_heff_workspace(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml,
+                          int64_t Mr) {
+  Dims D = {Dl, d, Dr, Ml, Mr};
+  return heff_ws_bytes(precision, D);
+}
+
+int ptn_heff_apply(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr,
+                   const void* l, const void* w, const void* r, const void* a, void* out, void* ws,
+                   size_t ws_bytes, void* stream) {
+  if (precision != PTN_FP64 && precision != PTN_FP32) return PTN_ERR_UNSUPPORTED;
+  Dims D = {Dl, d, Dr, Ml, Mr};
+  return heff_apply(precision, D, l, PTN_T_C128, w, r, PTN_T_C128, a, PTN_T_C128, out,
+                    PTN_T_C128, ws, ws_bytes, (hipStream_t)stream);
+}
+
+size_t ptn_bond_workspace(int precision, int64_t Dl, int64_t Dr, int64_t M) {
+  return bond_ws_bytes(precision, Dl, Dr, M);
+}
+
+int ptn_bond_apply(int precision, int64_t Dl, int64_t Dr, int64_t M, const void* l, const void* r,
+                   const void* c, void* out, void* ws, size_t ws_bytes, void* stream) {
+  if (precision != PTN_FP64 && precision != PTN_FP32) return PTN_ERR_UNSUPPORTED;
+  return bond_apply(precision, Dl, Dr, M, l, PTN_T_C128, r, PTN_T_C128, c, PTN_T_C128, out,
+                    PTN_T_C128, ws, ws_bytes, (hipStream_t)stream);
+}
+
+size_t ptn_env_workspace(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml,
+                         int64_t Mr) {
+  Dims D = {Dl, d, Dr, Ml, Mr};
+  return env_ws_bytes(precision, D);
+}
+
+int ptn_env_left(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr,
+                 const void* a, const void* b, const void* w, const void* l, void* out, void* ws,
+                 size_t ws_bytes, void* stream) {
+  if (precision != PTN_FP64 && precision != PTN_FP32) return PTN_ERR_UNSUPPORTED;
+  Dims D = {Dl, d, Dr, Ml, Mr};
+  return env_left(precision, D, a, b, w, l, PTN_T_C128, out, ws, ws_bytes, (hipStream_t)stream);
+}
+
+int ptn_env_right(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr,
+                  const void* a, const void* b, const void* w, const void* r, void* out, void* ws,
+                  size_t ws_bytes, void* stream) {
+  if (precision != PTN_FP64 && precision != PTN_FP32) return PTN_ERR_UNSUPPORTED;
+  Dims D = {Dl, d, Dr, Ml, Mr};
+  return env_right(precision, D, a, b, w, r, PTN_T_C128, out, ws, ws_bytes, (hipStream_t)stream);
+}
+
+size_t ptn_zgemm_workspace(int64_t M, int64_t N, int64_t K) {
+  (void)M; (void)N; (void)K;
+  return 0;
+}
+
+int ptn_zgemm(int64_t M, int64_t N, int64_t K, const void* A, int64_t a_m, int64_t a_k, int conjA,
+              const void* B, int64_t b_k, int64_t b_n, int conjB, void* C, int64_t c_m,
+              int64_t c_n, int accumulate, void* ws, size_t ws_bytes, void* stream) {
+  ZGemm g = {M, N, K, A, a_m, a_k, conjA, B, b_k, b_n, conjB, C, c_m, c_n, accumulate, 1, 0, 0, 0};
+  return zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, ws, ws_bytes,
+               (hipStream_t)stream);
+}
+
+size_t ptn_krylov_workspace(const ptn_local_op* op, int numiter) {
+  return krylov_ws_bytes(op, numiter);
+}
+
+int ptn_krylov_expm(const ptn_local_op* op, const void* v, double dt_re, double dt_im, int numiter,
+                    void* out, int32_t* info, double* alpha_beta, void* ws, size_t ws_bytes,
+                    void* stream) {
+  return krylov_run(op, 0, v, dt_re, dt_im, numiter, out, nullptr, info, alpha_beta, ws, ws_bytes,
+                    (hipStream_t)stream);
+}
+
+int ptn_krylov_eigh(const ptn_local_op* op, const void* v, int numiter, void* out, double* energy,
+                    int32_t* info, double* alpha_beta, void* ws, size_t ws_bytes, void* stream) {
+  return krylov_run(op, 1, v, 0.0, 0.0, numiter, out, energy, info, alpha_beta, ws, ws_bytes,
+                    (hipStream_t)stream);
+}
+
+size_t ptn_qr_workspace(int64_t m, int64_t n) { return qr_ws_bytes(m, n); }
+
+int ptn_qr(int64_t m, int64_t n, const void* a, int64_t lda, void* q, int64_t ldq, void* r,
+           int64_t ldr, void* ws, size_t ws_bytes, void* stream) {
+  return qr_run(m, n, a, lda, q, ldq, r, ldr, ws, ws_bytes, (hipStream_t)stream);
+}
+
+size_t ptn_svd_workspace(int64_t m, int64_t n) { return svd_ws_bytes(m, n); }
+
+int ptn_svd(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t ldu, double* s,
+            void* vh, int64_t ldvh, int32_t* info, void* ws, size_t ws_bytes, void* stream) {
+  return svd_run(m, n, a, lda, u, ldu, s, vh, ldvh, info, ws, ws_bytes, (hipStream_t)stream);
+}
+
+int ptn_scale_axis(int64_t n0, int64_t n1, int64_t n2, void* a, const double* s, int axis,
+                   void* stream) {
+  if (n0 < 0 || n1 < 0 || n2 < 0 || (axis != 0 && axis != 2)) return PTN_ERR_ARG;
+  const int64_t total = n0 * n1 * n2;
+  if (total == 0) return PTN_OK;
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  k_scale_axis<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(n0, n1, n2, (double2*)a, s,
+                                                                   axis);
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+int ptn_qsparse3(int64_t n0, int64_t n1, int64_t n2, const int64_t* q0, const int64_t* q1,
+                 const int64_t* q2, void* a, int zero_them, double* out, void* stream) {
+  if (n0 < 0 || n1 < 0 || n2 < 0) return PTN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  k_zero_u64<<<1, 1, 0, s>>>((unsigned long long*)out);
+  const int64_t total = n0 * n1 * n2;
+  if (total > 0) {
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    k_qsparse3<<<(unsigned)blocks, 256, 0, s>>>(n0, n1, n2, q0, q1, q2, (double2*)a, zero_them,
+                                                (unsigned long long*)out);
+  }
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+}  // extern "C"

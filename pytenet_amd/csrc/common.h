@@ -1,0 +1,73 @@
+// Shared definitions for the gfx950 kernels of the MPS sweep library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/ptn.h"
+
+namespace ptn {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+struct cd { double x, y; };   // complex128, interleaved (matches torch/NumPy)
+struct cf { float x, y; };    // complex64
+
+__device__ __forceinline__ cd cmul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ cd cconj(cd a) { return {a.x, -a.y}; }
+
+template <typename T> struct cplx_of;
+template <> struct cplx_of<double> { typedef double2 type; };
+template <> struct cplx_of<float> { typedef float2 type; };
+
+// load a complex element of either precision as (re, im) in the compute type
+template <typename Tc> __device__ __forceinline__ void load_c(const double2* p, Tc& re, Tc& im) {
+  double2 v = *p; re = (Tc)v.x; im = (Tc)v.y;
+}
+template <typename Tc> __device__ __forceinline__ void load_c(const float2* p, Tc& re, Tc& im) {
+  float2 v = *p; re = (Tc)v.x; im = (Tc)v.y;
+}
+__device__ __forceinline__ void store_c(double2* p, double re, double im) { *p = make_double2(re, im); }
+__device__ __forceinline__ void store_c(float2* p, double re, double im) {
+  *p = make_float2((float)re, (float)im);
+}
+
+#define PTN_HIP_CHECK(expr)                                  \
+  do {                                                       \
+    hipError_t e_ = (expr);                                  \
+    if (e_ != hipSuccess) { ptn_set_last_hip_error(e_); return PTN_ERR_HIP; } \
+  } while (0)
+
+#define PTN_LAUNCH_CHECK()                                   \
+  do {                                                       \
+    hipError_t e_ = hipGetLastError();                       \
+    if (e_ != hipSuccess) { ptn_set_last_hip_error(e_); return PTN_ERR_HIP; } \
+  } while (0)
+
+void ptn_set_last_hip_error(hipError_t e);
+
+// ---------------------------------------------------------------------------
+// Complex GEMM descriptor. Element (m, k) of op(A) is A[m*a_m + k*a_k]
+// (conjugated if conjA), element (k, n) of op(B) is B[k*b_k + n*b_n]; C[m*c_m + n*c_n].
+// Strides are in complex elements. bz_* are batch strides (blockIdx.z).
+struct ZGemm {
+  int64_t M, N, K;
+  const void* A; int64_t a_m, a_k; int conjA;
+  const void* B; int64_t b_k, b_n; int conjB;
+  void* C; int64_t c_m, c_n;
+  int accumulate;      // C += op(A) op(B) when 1
+  int64_t batch;       // number of independent problems
+  int64_t bz_a, bz_b, bz_c;
+};
+
+// element type codes for the GEMM / mix operands
+enum { PTN_T_C128 = 0, PTN_T_C64 = 1 };
+
+int zgemm(const ZGemm& g, int compute /*PTN_FP64 | PTN_FP32*/, int ta, int tb, int tc,
+          void* ws, size_t ws_bytes, hipStream_t s);
+
+// Y[l, o, r] = sum_i Wm[o, i] X[l, i, r]   (complex; Wm complex128 on device)
+int mode_mix(int64_t nl, int64_t ni, int64_t no, int64_t nr, const void* wm, const void* x, int tx,
+             void* y, int ty, hipStream_t s);
+
+}  // namespace ptn

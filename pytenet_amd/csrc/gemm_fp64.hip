@@ -1,0 +1,22 @@
+// Complex matrix products, FP64 MFMA instantiations and the zgemm dispatcher.
+#include "gemm_impl.h"
+
+namespace ptn {
+
+int zgemm_fp32(const ZGemm& z, int ta, int tb, int tc, hipStream_t s);
+
+int zgemm(const ZGemm& z, int compute, int ta, int tb, int tc, void* ws, size_t ws_bytes,
+          hipStream_t s) {
+  (void)ws; (void)ws_bytes;
+  if (z.M < 0 || z.N < 0 || z.K < 0 || z.batch < 1) return PTN_ERR_ARG;
+  if (z.M == 0 || z.N == 0) return PTN_OK;
+  if (compute == PTN_FP64) {
+    if (ta == PTN_T_C128 && tb == PTN_T_C128 && tc == PTN_T_C128)
+      return launch_typed<double, double2, double2, double2>(z, s);
+    return PTN_ERR_UNSUPPORTED;
+  }
+  if (compute == PTN_FP32) return zgemm_fp32(z, ta, tb, tc, s);
+  return PTN_ERR_UNSUPPORTED;
+}
+
+}  // namespace ptn

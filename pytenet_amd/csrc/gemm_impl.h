@@ -1,0 +1,256 @@
+// Complex matrix products on CDNA4 MFMA through the real embedding
+//   [Re C, Im C] = [Re A, Im A] . [[Re B, Im B], [-Im B, Re B]]
+// i.e. a complex (M x N x K) product is a real (M x 2N x 2K) product whose A
+// operand is the interleaved complex row itself and whose B operand is
+// expanded from (Re B, Im B) when it is read from LDS. This keeps the
+// interleaved complex layout of torch/NumPy in HBM, needs no extra flops
+// (8 real flops per complex multiply-add, as four real products would) and
+// uses one accumulator set per output tile.
+//
+// MFMA: v_mfma_f64_16x16x4f64 (FP64) and v_mfma_f32_16x16x4f32 (FP32) take
+// A[i][k] in lane i + 16k and B[k][j] in lane j + 16k; the 16x16 result has
+// column lane%16 and, in accumulator element v, row lane/16 + 4v (FP64) or
+// row 4*(lane/16) + v (FP32).
+#pragma once
+#include "common.h"
+
+namespace ptn {
+
+__device__ __forceinline__ d4_t mfma16(double a, double b, d4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4_t mfma16(float a, float b, f4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+template <typename Tc> struct acc4;
+template <> struct acc4<double> {
+  typedef d4_t type;
+  static __device__ __forceinline__ int row(int group, int v) { return group + 4 * v; }
+};
+template <> struct acc4<float> {
+  typedef f4_t type;
+  static __device__ __forceinline__ int row(int group, int v) { return 4 * group + v; }
+};
+
+// Workgroup = 4 waves as 2 x 2; each wave owns (16*TM) rows x (8*TN) complex columns.
+template <int TM, int TN> struct TileCfg {
+  static constexpr int WM = 16 * TM;
+  static constexpr int WN = 8 * TN;
+  static constexpr int BM = 2 * WM;
+  static constexpr int BN = 2 * WN;
+  static constexpr int BK = 8;            // complex k per LDS stage = 4 MFMA k-steps
+  static constexpr int NA = BM * BK / 256;
+  static constexpr int NB = BN * BK / 256;
+};
+
+struct GemmKArgs {
+  int64_t M, N, K;
+  const void* A; int64_t a_m, a_k; int conjA;
+  const void* B; int64_t b_k, b_n; int conjB;
+  void* C; int64_t c_m, c_n;
+  int accumulate;
+  int64_t bz_a, bz_b, bz_c;
+  int tiles_n;          // number of column tiles
+  int tiles_m;
+};
+
+// map the linear workgroup id so that the workgroups that run on one XCD
+// (hardware assigns id % 8 to XCD id % 8) walk a contiguous range of tiles,
+// column-tile fastest within a group of 8 row tiles, for L2 reuse of A and B.
+__device__ __forceinline__ void tile_coords(int wg, int nwg, int tiles_m, int tiles_n, int& tm,
+                                            int& tn) {
+  int logical = wg;
+  if ((nwg & 7) == 0) logical = (wg & 7) * (nwg >> 3) + (wg >> 3);
+  const int group = 8;
+  int per_group = group * tiles_n;
+  int g = logical / per_group;
+  int first_m = g * group;
+  int rows = min(tiles_m - first_m, group);
+  int in = logical - g * per_group;
+  tm = first_m + in % rows;
+  tn = in / rows;
+}
+
+template <typename Tc, typename TA, typename TB, typename TC, int TM, int TN, bool A_KCONTIG,
+          bool B_NCONTIG>
+__global__ __launch_bounds__(256) void zgemm_kernel(GemmKArgs g) {
+  typedef TileCfg<TM, TN> T;
+  constexpr int BM = T::BM, BN = T::BN, BK = T::BK, NA = T::NA, NB = T::NB;
+  typedef typename acc4<Tc>::type acc_t;
+
+  __shared__ Tc As[2 * BK][BM + 1];
+  __shared__ Tc Bs[BK][2 * BN + 2];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm0 = (wave >> 1) * T::WM;
+  const int wn0 = (wave & 1) * T::WN;
+
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, g.tiles_m, g.tiles_n, tm, tn);
+  const int64_t m0 = (int64_t)tm * BM;
+  const int64_t n0 = (int64_t)tn * BN;
+  const int64_t bz = blockIdx.z;
+  const TA* __restrict__ A = (const TA*)g.A + bz * g.bz_a;
+  const TB* __restrict__ B = (const TB*)g.B + bz * g.bz_b;
+  TC* __restrict__ C = (TC*)g.C + bz * g.bz_c;
+  const Tc sa = g.conjA ? (Tc)-1 : (Tc)1;
+  const Tc sb = g.conjB ? (Tc)-1 : (Tc)1;
+
+  Tc ra[NA][2], rb[NB][2];
+  auto load_tiles = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      int e = tid + 256 * i;
+      int row, kc;
+      if (A_KCONTIG) { row = e / BK; kc = e % BK; } else { kc = e / BM; row = e % BM; }
+      int64_t m = m0 + row, k = k0 + kc;
+      Tc re = 0, im = 0;
+      if (m < g.M && k < g.K) load_c<Tc>(A + m * g.a_m + k * g.a_k, re, im);
+      ra[i][0] = re; ra[i][1] = sa * im;
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      int e = tid + 256 * i;
+      int col, kc;
+      if (B_NCONTIG) { kc = e / BN; col = e % BN; } else { col = e / BK; kc = e % BK; }
+      int64_t n = n0 + col, k = k0 + kc;
+      Tc re = 0, im = 0;
+      if (n < g.N && k < g.K) load_c<Tc>(B + k * g.b_k + n * g.b_n, re, im);
+      rb[i][0] = re; rb[i][1] = sb * im;
+    }
+  };
+  auto store_tiles = [&]() {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      int e = tid + 256 * i;
+      int row, kc;
+      if (A_KCONTIG) { row = e / BK; kc = e % BK; } else { kc = e / BM; row = e % BM; }
+      As[2 * kc][row] = ra[i][0];
+      As[2 * kc + 1][row] = ra[i][1];
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      int e = tid + 256 * i;
+      int col, kc;
+      if (B_NCONTIG) { kc = e / BN; col = e % BN; } else { col = e / BK; kc = e % BK; }
+      Bs[kc][2 * col] = rb[i][0];
+      Bs[kc][2 * col + 1] = rb[i][1];
+    }
+  };
+
+  acc_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  // per-lane selection of the expanded B operand: real k parity (a) and real
+  // column parity (b): (0,0) Re, (0,1) Im, (1,0) -Im, (1,1) Re
+  const int kk = lane >> 4;
+  const int pa = kk & 1, pb = lane & 1;
+  const bool take_im = (pa != pb);
+  const Tc im_sign = (pa == 1) ? (Tc)-1 : (Tc)1;
+
+  const int nk = (int)((g.K + BK - 1) / BK);
+  load_tiles(0);
+  for (int t = 0; t < nk; ++t) {
+    __syncthreads();
+    store_tiles();
+    __syncthreads();
+    if (t + 1 < nk) load_tiles((int64_t)(t + 1) * BK);
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      Tc av[TM], bv[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) av[i] = As[4 * ks + kk][wm0 + 16 * i + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const Tc* p = &Bs[2 * ks + (kk >> 1)][2 * (wn0 + 8 * j + ((lane & 15) >> 1))];
+        Tc re = p[0], im = p[1];
+        bv[j] = take_im ? im_sign * im : re;
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = mfma16(av[i], bv[j], acc[i][j]);
+    }
+  }
+
+  // epilogue: lane holds the real (pb = 0) or imaginary (pb = 1) part of
+  // complex column n for the rows of its accumulator elements v; the partner
+  // lane (lane ^ 1) holds the other part. Even lanes store the rows of
+  // elements 0, 1 and odd lanes those of elements 2, 3.
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      Tc mine[4], other[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        mine[v] = acc[i][j][v];
+        other[v] = __shfl_xor(mine[v], 1);
+      }
+      const int64_t n = n0 + wn0 + 8 * j + ((lane & 15) >> 1);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        // constant register indices only (a lane-dependent index would force scratch)
+        const int v = 2 * pb + h;
+        const int64_t m = m0 + wm0 + 16 * i + acc4<Tc>::row(kk, v);
+        double re = pb ? (double)other[2 + h] : (double)mine[h];
+        double im = pb ? (double)mine[2 + h] : (double)other[h];
+        if (m < g.M && n < g.N) {
+          TC* p = C + m * g.c_m + n * g.c_n;
+          if (g.accumulate) {
+            double pr, pi;
+            load_c<double>(p, pr, pi);
+            re += pr; im += pi;
+          }
+          store_c(p, re, im);
+        }
+      }
+    }
+  }
+}
+
+template <typename Tc, typename TA, typename TB, typename TC, int TM, int TN>
+inline int launch_cfg(const ZGemm& z, hipStream_t s) {
+  typedef TileCfg<TM, TN> T;
+  GemmKArgs g;
+  g.M = z.M; g.N = z.N; g.K = z.K;
+  g.A = z.A; g.a_m = z.a_m; g.a_k = z.a_k; g.conjA = z.conjA;
+  g.B = z.B; g.b_k = z.b_k; g.b_n = z.b_n; g.conjB = z.conjB;
+  g.C = z.C; g.c_m = z.c_m; g.c_n = z.c_n;
+  g.accumulate = z.accumulate;
+  g.bz_a = z.bz_a; g.bz_b = z.bz_b; g.bz_c = z.bz_c;
+  g.tiles_m = (int)((z.M + T::BM - 1) / T::BM);
+  g.tiles_n = (int)((z.N + T::BN - 1) / T::BN);
+  int64_t nwg = (int64_t)g.tiles_m * g.tiles_n;
+  if (nwg > 0x7fffffff || z.batch > 65535) return PTN_ERR_ARG;
+  dim3 grid((unsigned)nwg, 1, (unsigned)z.batch);
+  // a K-contiguous A when its k stride is 1 (or K == 1), else M-contiguous
+  const bool ak = (z.a_k == 1) || (z.a_m != 1);
+  const bool bn = (z.b_n == 1) || (z.b_k != 1);
+  if (ak && bn)
+    zgemm_kernel<Tc, TA, TB, TC, TM, TN, true, true><<<grid, 256, 0, s>>>(g);
+  else if (ak && !bn)
+    zgemm_kernel<Tc, TA, TB, TC, TM, TN, true, false><<<grid, 256, 0, s>>>(g);
+  else if (!ak && bn)
+    zgemm_kernel<Tc, TA, TB, TC, TM, TN, false, true><<<grid, 256, 0, s>>>(g);
+  else
+    zgemm_kernel<Tc, TA, TB, TC, TM, TN, false, false><<<grid, 256, 0, s>>>(g);
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+// tile choice: the large tile (128 x 64 complex) when it still gives at least
+// two workgroups per CU, else the small one (64 x 32)
+template <typename Tc, typename TA, typename TB, typename TC>
+inline int launch_typed(const ZGemm& z, hipStream_t s) {
+  int64_t big = ((z.M + 127) / 128) * ((z.N + 63) / 64) * z.batch;
+  if (big >= 512) return launch_cfg<Tc, TA, TB, TC, 4, 4>(z, s);
+  return launch_cfg<Tc, TA, TB, TC, 2, 2>(z, s);
+}
+
+}  // namespace ptn

@@ -1,0 +1,443 @@
+// Device-resident Lanczos for the local TDVP/DMRG problems.
+//
+// Follows pytenet/krylov.py:12-57 (plain three-term recurrence, no
+// reorthogonalisation, breakdown when beta_j < 100*n*eps), krylov.py:142-150
+// (eigendecomposition of the tridiagonal matrix) and krylov.py:110-136
+// (Ritz vector / Krylov exponential). All scalars (alpha, beta, norms, the
+// tridiagonal eigensystem and the combination coefficients) live in device
+// memory and are FP64, so a whole local solve is enqueued without a host
+// round trip. A breakdown or a non-finite value stops the recurrence through a
+// flag that later kernels read; the host reads the status words afterwards.
+#include <float.h>
+#include "common.h"
+#include "ops.h"
+
+namespace ptn {
+
+constexpr int LZ_MAX = 64;        // largest supported numiter
+constexpr int RED_BLOCKS = 512;   // fixed reduction width: deterministic sums
+
+struct LzState {
+  double alpha[LZ_MAX];
+  double beta[LZ_MAX];
+  double coef_re[LZ_MAX];
+  double coef_im[LZ_MAX];
+  double nrm0;
+  double energy;
+  int nvalid;      // Lanczos vectors in use
+  int breakdown;   // j + 1 of a beta breakdown, 0 if none
+  int nonfinite;   // a norm / inner product was not finite
+  int zero_start;  // the start vector had norm zero
+  int stopped;     // recurrence stopped (breakdown, non-finite or zero start)
+  int noconv;      // tridiagonal QL iteration did not converge
+};
+
+// sum of the RED_BLOCKS partials in a fixed order; every block of a kernel
+// computes the identical value (256 threads)
+__device__ double block_sum_partials(const double* __restrict__ part) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  red[t] = part[t] + part[t + 256];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  double s = red[0];
+  __syncthreads();
+  return s;
+}
+
+__device__ void block_store_partial(double v, double* out) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) out[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_state_init(LzState* st, int numiter) {
+  int t = threadIdx.x;
+  if (t < LZ_MAX) {
+    st->alpha[t] = 0; st->beta[t] = 0; st->coef_re[t] = 0; st->coef_im[t] = 0;
+  }
+  if (t == 0) {
+    st->nrm0 = 0; st->energy = 0; st->nvalid = numiter; st->breakdown = 0;
+    st->nonfinite = 0; st->zero_start = 0; st->stopped = 0; st->noconv = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sumsq_partial(int64_t n, const double2* __restrict__ x,
+                                                       double* __restrict__ part) {
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * 256) {
+    double2 v = x[i];
+    s += v.x * v.x + v.y * v.y;
+  }
+  block_store_partial(s, part);
+}
+
+// Re <w, v> partials
+__global__ __launch_bounds__(256) void k_dot_partial(int64_t n, const double2* __restrict__ w,
+                                                     const double2* __restrict__ v,
+                                                     double* __restrict__ part) {
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * 256) {
+    double2 a = w[i], b = v[i];
+    s += a.x * b.x + a.y * b.y;
+  }
+  block_store_partial(s, part);
+}
+
+// v0 = x / |x|  (krylov.py:29-31)
+__global__ __launch_bounds__(256) void k_start(int64_t n, const double2* __restrict__ x,
+                                               double2* __restrict__ v0,
+                                               const double* __restrict__ part, LzState* st) {
+  double nrm = sqrt(block_sum_partials(part));
+  bool bad = !(nrm > 0.0) || !isfinite(nrm);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->nrm0 = nrm;
+    if (bad) {
+      if (nrm == 0.0) st->zero_start = 1; else st->nonfinite = 1;
+      st->stopped = 1;
+      st->nvalid = 0;
+    }
+  }
+  if (bad) return;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double2 a = x[i];
+    v0[i] = make_double2(a.x / nrm, a.y / nrm);
+  }
+}
+
+// alpha_j = Re <w, v_j>;  w -= alpha_j v_j + beta_{j-1} v_{j-1};  partials of |w|^2
+__global__ __launch_bounds__(256) void k_alpha_update(int64_t n, double2* __restrict__ w,
+                                                      const double2* __restrict__ vj,
+                                                      const double2* __restrict__ vjm1, int j,
+                                                      const double* __restrict__ part_in,
+                                                      double* __restrict__ part_out, LzState* st) {
+  if (st->stopped) return;
+  const double a = block_sum_partials(part_in);
+  if (!isfinite(a)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->nonfinite = 1; st->stopped = 1; st->nvalid = j + 1; st->alpha[j] = a;
+    }
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha[j] = a;
+  const double b = (j > 0) ? st->beta[j - 1] : 0.0;
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * 256) {
+    double2 x = w[i], v = vj[i];
+    double tr = a * v.x, ti = a * v.y;
+    if (j > 0) {
+      double2 u = vjm1[i];
+      tr += b * u.x; ti += b * u.y;
+    }
+    x.x -= tr; x.y -= ti;
+    w[i] = x;
+    s += x.x * x.x + x.y * x.y;
+  }
+  block_store_partial(s, part_out);
+}
+
+// beta_j = |w|; breakdown test; v_{j+1} = w / beta_j
+__global__ __launch_bounds__(256) void k_beta_next(int64_t n, const double2* __restrict__ w,
+                                                   double2* __restrict__ vnext, int j,
+                                                   const double* __restrict__ part_in,
+                                                   double thresh, LzState* st) {
+  if (st->stopped) return;
+  const double b = sqrt(block_sum_partials(part_in));
+  const bool nf = !isfinite(b);
+  const bool brk = !nf && (b < thresh);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (nf) { st->nonfinite = 1; st->stopped = 1; st->nvalid = j + 1; }
+    else if (brk) { st->breakdown = j + 1; st->stopped = 1; st->nvalid = j + 1; }
+    else st->beta[j] = b;
+  }
+  if (nf || brk) return;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double2 x = w[i];
+    vnext[i] = make_double2(x.x / b, x.y / b);
+  }
+}
+
+// last alpha (krylov.py:54-56)
+__global__ __launch_bounds__(256) void k_alpha_last(int j, const double* __restrict__ part_in,
+                                                    LzState* st) {
+  if (st->stopped) return;
+  const double a = block_sum_partials(part_in);
+  if (threadIdx.x == 0) {
+    st->alpha[j] = a;
+    if (!isfinite(a)) st->nonfinite = 1;
+  }
+}
+
+// Eigendecomposition of the nv x nv tridiagonal matrix by the implicit QL
+// method with Wilkinson shifts (one wave: every lane runs the same scalar
+// recurrence on the diagonal/off-diagonal, lane k applies the plane
+// rotations to row k of the eigenvector matrix), eigenvalues sorted
+// ascending as numpy.linalg.eigh returns them; then the coefficients:
+//   mode 0 (expm):  c = U (|v| exp(dt lambda) * U[0, :])    (krylov.py:132-136)
+//   mode 1 (eigh):  c = U[:, 0], energy = lambda_0          (krylov.py:116-119)
+__global__ __launch_bounds__(64) void k_tridiag(LzState* st, int mode, double dt_re,
+                                                double dt_im) {
+  __shared__ double d[LZ_MAX], e[LZ_MAX];
+  __shared__ double Z[LZ_MAX][LZ_MAX + 1];
+  __shared__ double tre[LZ_MAX], tim[LZ_MAX];
+  const int lane = threadIdx.x;
+  const int n = st->nvalid;
+  if (n <= 0 || st->nonfinite) return;
+  d[lane] = (lane < n) ? st->alpha[lane] : 0.0;
+  e[lane] = (lane < n - 1) ? st->beta[lane] : 0.0;
+  for (int k = 0; k < LZ_MAX; ++k) Z[lane][k] = (lane == k) ? 1.0 : 0.0;
+  __syncthreads();
+  int noconv = 0;
+  for (int l = 0; l < n; ++l) {
+    int iter = 0;
+    for (;;) {
+      int m = l;
+      while (m < n - 1) {
+        const double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) <= DBL_EPSILON * dd) break;
+        ++m;
+      }
+      if (m == l) break;
+      if (++iter > 100) { noconv = 1; break; }
+      // shift from the leading 2 x 2 block of the unreduced part
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + copysign(r, g));
+      double s = 1.0, c = 1.0, p = 0.0;
+      bool split = false;
+      for (int i = m - 1; i >= l; --i) {
+        const double f = s * e[i];
+        const double b = c * e[i];
+        r = hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= p;
+          e[m] = 0.0;
+          split = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        const double z1 = Z[lane][i + 1], z0 = Z[lane][i];
+        Z[lane][i + 1] = s * z0 + c * z1;
+        Z[lane][i] = c * z0 - s * z1;
+      }
+      if (split) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  // ascending order (selection sort; the permutation is identical in every lane)
+  for (int i = 0; i < n - 1; ++i) {
+    int k = i;
+    double dm = d[i];
+    for (int j = i + 1; j < n; ++j)
+      if (d[j] < dm) { dm = d[j]; k = j; }
+    if (k != i) {
+      const double t = d[i];
+      d[k] = t;
+      d[i] = dm;
+      const double zt = Z[lane][i];
+      Z[lane][i] = Z[lane][k];
+      Z[lane][k] = zt;
+    }
+  }
+  __syncthreads();
+  if (mode == 0) {
+    if (lane < n) {
+      // |v| * exp(dt * lambda_j) * U[0, j]
+      const double lam = d[lane];
+      const double mag = exp(dt_re * lam);
+      const double ph = dt_im * lam;
+      const double er = st->nrm0 * (mag * cos(ph));
+      const double ei = st->nrm0 * (mag * sin(ph));
+      tre[lane] = er * Z[0][lane];
+      tim[lane] = ei * Z[0][lane];
+    }
+    __syncthreads();
+    if (lane < n) {
+      double cr = 0, ci = 0;
+      for (int j = 0; j < n; ++j) {
+        cr += Z[lane][j] * tre[j];
+        ci += Z[lane][j] * tim[j];
+      }
+      st->coef_re[lane] = cr;
+      st->coef_im[lane] = ci;
+    }
+  } else {
+    if (lane < n) {
+      st->coef_re[lane] = Z[lane][0];
+      st->coef_im[lane] = 0.0;
+    }
+    if (lane == 0) st->energy = d[0];
+  }
+  if (lane == 0 && noconv) st->noconv = 1;
+}
+
+// out = sum_{j < nvalid} c_j V_j  (krylov.py:118, :136)
+__global__ __launch_bounds__(256) void k_combine(int64_t n, const double2* __restrict__ V,
+                                                 const LzState* __restrict__ st,
+                                                 double2* __restrict__ out) {
+  __shared__ double cr[LZ_MAX], ci[LZ_MAX];
+  const int nv = st->nonfinite ? 0 : st->nvalid;
+  if (threadIdx.x < LZ_MAX) {
+    cr[threadIdx.x] = st->coef_re[threadIdx.x];
+    ci[threadIdx.x] = st->coef_im[threadIdx.x];
+  }
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double sr = 0, si = 0;
+    for (int j = 0; j < nv; ++j) {
+      const double2 v = V[(int64_t)j * n + i];
+      sr += cr[j] * v.x - ci[j] * v.y;
+      si += cr[j] * v.y + ci[j] * v.x;
+    }
+    out[i] = make_double2(sr, si);
+  }
+}
+
+__global__ void k_export(const LzState* __restrict__ st, int32_t* info, double* alpha_beta,
+                         double* energy, int numiter) {
+  const int t = threadIdx.x;
+  if (t == 0) {
+    if (info) {
+      info[0] = st->nvalid;
+      info[1] = st->breakdown;
+      info[2] = st->nonfinite | (st->noconv << 1);
+      info[3] = st->zero_start;
+    }
+    if (energy) *energy = st->energy;
+  }
+  if (alpha_beta && t < numiter) {
+    alpha_beta[t] = st->alpha[t];
+    alpha_beta[numiter + t] = st->beta[t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+static inline size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct KrylovLayout {
+  size_t v, w, part_a, part_b, state, l64, r64, mv, total;
+};
+
+static int64_t op_vec_len(const ptn_local_op* op) {
+  return op->kind == PTN_OP_HEFF ? op->Dl * op->d * op->Dr : op->Dl * op->Dr;
+}
+
+static KrylovLayout krylov_layout(const ptn_local_op* op, int numiter) {
+  KrylovLayout k;
+  const int64_t n = op_vec_len(op);
+  size_t off = 0;
+  k.v = off; off += al((size_t)numiter * n * 16);
+  k.w = off; off += al((size_t)n * 16);
+  k.part_a = off; off += al(RED_BLOCKS * 8);
+  k.part_b = off; off += al(RED_BLOCKS * 8);
+  k.state = off; off += al(sizeof(LzState));
+  const bool lowp = op->precision != PTN_FP64;
+  const int64_t nl = op->Dl * op->Ml * op->Dl, nr = op->Dr * op->Mr * op->Dr;
+  k.l64 = off; off += lowp ? al((size_t)nl * 8) : 0;
+  k.r64 = off; off += lowp ? al((size_t)nr * 8) : 0;
+  k.mv = off;
+  if (op->kind == PTN_OP_HEFF) {
+    Dims D = {op->Dl, op->d, op->Dr, op->Ml, op->Mr};
+    off += al(heff_ws_bytes(op->precision, D));
+  } else {
+    off += al(bond_ws_bytes(op->precision, op->Dl, op->Dr, op->Ml));
+  }
+  k.total = off;
+  return k;
+}
+
+size_t krylov_ws_bytes(const ptn_local_op* op, int numiter) {
+  return krylov_layout(op, numiter).total;
+}
+
+int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, double dt_im,
+               int numiter, void* out, double* energy, int32_t* info, double* alpha_beta,
+               void* ws, size_t ws_bytes, hipStream_t s) {
+  if (!op || numiter < 1 || numiter > LZ_MAX) return PTN_ERR_ARG;
+  if (op->kind != PTN_OP_HEFF && op->kind != PTN_OP_BOND) return PTN_ERR_ARG;
+  if (op->precision != PTN_FP64 && op->precision != PTN_FP32) return PTN_ERR_UNSUPPORTED;
+  if (op->Dl <= 0 || op->Dr <= 0 || op->Ml <= 0 || op->Mr <= 0) return PTN_ERR_ARG;
+  if (op->kind == PTN_OP_HEFF && (op->d <= 0 || !op->w)) return PTN_ERR_ARG;
+  if (op->kind == PTN_OP_BOND && op->Ml != op->Mr) return PTN_ERR_ARG;
+  const KrylovLayout K = krylov_layout(op, numiter);
+  if (ws_bytes < K.total) return PTN_ERR_WORKSPACE;
+  char* base = (char*)ws;
+  double2* V = (double2*)(base + K.v);
+  double2* w = (double2*)(base + K.w);
+  double* pa = (double*)(base + K.part_a);
+  double* pb = (double*)(base + K.part_b);
+  LzState* st = (LzState*)(base + K.state);
+  void* mvws = base + K.mv;
+  const size_t mvbytes = K.total - K.mv;
+  const int64_t n = op_vec_len(op);
+  const double thresh = 100.0 * (double)n * DBL_EPSILON;
+
+  // environment blocks in the matvec's operand precision, converted once per solve
+  const void* L = op->l;
+  const void* R = op->r;
+  int tl = PTN_T_C128, tr = PTN_T_C128;
+  int st_;
+  if (op->precision != PTN_FP64) {
+    const int64_t nl = op->Dl * op->Ml * op->Dl, nr = op->Dr * op->Mr * op->Dr;
+    if ((st_ = convert_c128_c64(nl, op->l, base + K.l64, s))) return st_;
+    if ((st_ = convert_c128_c64(nr, op->r, base + K.r64, s))) return st_;
+    L = base + K.l64; R = base + K.r64;
+    tl = tr = PTN_T_C64;
+  }
+  Dims D = {op->Dl, op->d, op->Dr, op->Ml, op->Mr};
+  auto matvec = [&](const double2* x, double2* y) -> int {
+    if (op->kind == PTN_OP_HEFF)
+      return heff_apply(op->precision, D, L, tl, op->w, R, tr, x, PTN_T_C128, y, PTN_T_C128,
+                        mvws, mvbytes, s);
+    return bond_apply(op->precision, op->Dl, op->Dr, op->Ml, L, tl, R, tr, x, PTN_T_C128, y,
+                      PTN_T_C128, mvws, mvbytes, s);
+  };
+  int64_t gblocks = (n + 255) / 256;
+  if (gblocks > 2048) gblocks = 2048;
+  if (gblocks < 1) gblocks = 1;
+
+  k_state_init<<<1, 256, 0, s>>>(st, numiter);
+  k_sumsq_partial<<<RED_BLOCKS, 256, 0, s>>>(n, (const double2*)v, pa);
+  k_start<<<(unsigned)gblocks, 256, 0, s>>>(n, (const double2*)v, V, pa, st);
+  PTN_LAUNCH_CHECK();
+  for (int j = 0; j < numiter - 1; ++j) {
+    double2* vj = V + (int64_t)j * n;
+    if ((st_ = matvec(vj, w))) return st_;
+    k_dot_partial<<<RED_BLOCKS, 256, 0, s>>>(n, w, vj, pa);
+    k_alpha_update<<<RED_BLOCKS, 256, 0, s>>>(n, w, vj, j > 0 ? vj - n : vj, j, pa, pb, st);
+    k_beta_next<<<(unsigned)gblocks, 256, 0, s>>>(n, w, vj + n, j, pb, thresh, st);
+    PTN_LAUNCH_CHECK();
+  }
+  {
+    const int j = numiter - 1;
+    double2* vj = V + (int64_t)j * n;
+    if ((st_ = matvec(vj, w))) return st_;
+    k_dot_partial<<<RED_BLOCKS, 256, 0, s>>>(n, w, vj, pa);
+    k_alpha_last<<<1, 256, 0, s>>>(j, pa, st);
+  }
+  k_tridiag<<<1, 64, 0, s>>>(st, mode, dt_re, dt_im);
+  k_combine<<<(unsigned)gblocks, 256, 0, s>>>(n, V, st, (double2*)out);
+  k_export<<<1, 64, 0, s>>>(st, info, alpha_beta, energy, numiter);
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+}  // namespace ptn

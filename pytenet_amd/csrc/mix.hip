@@ -1,0 +1,72 @@
+// Mode mix of the small MPO index pairs (see ops.hip).
+#include "common.h"
+
+namespace ptn {
+
+// ---------------------------------------------------------------------------
+// Mode mix: Y[l, o, r] = sum_i Wm[o, i] X[l, i, r] for the small MPO index
+// pairs (ni, no <= 32). One thread per (l, r); the complex mixing matrix lives
+// in LDS and its zero entries (most of an XXZ/Ising MPO) are skipped with a
+// workgroup-uniform branch. HBM-bound: reads ni and writes no complex values per (l, r).
+constexpr int MIX_MAX = 32;
+
+template <typename TX, typename TY>
+__global__ __launch_bounds__(256) void mode_mix_kernel(int64_t nl, int ni, int no, int64_t nr,
+                                                       const double2* __restrict__ wm,
+                                                       const TX* __restrict__ x,
+                                                       TY* __restrict__ y) {
+  __shared__ double2 W[MIX_MAX * MIX_MAX];
+  for (int t = threadIdx.x; t < ni * no; t += blockDim.x) W[t] = wm[t];
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t l = blockIdx.y;
+  if (r >= nr) return;
+  double xr[MIX_MAX], xi[MIX_MAX];
+#pragma unroll
+  for (int i = 0; i < MIX_MAX; ++i) {
+    xr[i] = 0; xi[i] = 0;
+    if (i < ni) load_c<double>(x + (l * ni + i) * nr + r, xr[i], xi[i]);
+  }
+  for (int o = 0; o < no; ++o) {
+    double yr = 0, yi = 0;
+#pragma unroll
+    for (int i = 0; i < MIX_MAX; ++i) {
+      if (i < ni) {
+        double2 w = W[o * ni + i];
+        if (w.x != 0.0 || w.y != 0.0) {
+          yr += w.x * xr[i] - w.y * xi[i];
+          yi += w.x * xi[i] + w.y * xr[i];
+        }
+      }
+    }
+    store_c(y + (l * no + o) * nr + r, yr, yi);
+  }
+}
+
+int mode_mix(int64_t nl, int64_t ni, int64_t no, int64_t nr, const void* wm, const void* x, int tx,
+             void* y, int ty, hipStream_t s) {
+  if (ni > MIX_MAX || no > MIX_MAX || ni < 1 || no < 1) return PTN_ERR_UNSUPPORTED;
+  if (nl == 0 || nr == 0) return PTN_OK;
+  if (nl > 65535) return PTN_ERR_ARG;
+  dim3 grid((unsigned)((nr + 255) / 256), (unsigned)nl);
+  if (tx == PTN_T_C128 && ty == PTN_T_C128)
+    mode_mix_kernel<double2, double2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
+                                                           (const double2*)wm, (const double2*)x,
+                                                           (double2*)y);
+  else if (tx == PTN_T_C64 && ty == PTN_T_C64)
+    mode_mix_kernel<float2, float2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
+                                                         (const double2*)wm, (const float2*)x,
+                                                         (float2*)y);
+  else if (tx == PTN_T_C128 && ty == PTN_T_C64)
+    mode_mix_kernel<double2, float2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
+                                                          (const double2*)wm, (const double2*)x,
+                                                          (float2*)y);
+  else
+    mode_mix_kernel<float2, double2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
+                                                          (const double2*)wm, (const float2*)x,
+                                                          (double2*)y);
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+}  // namespace ptn

@@ -1,0 +1,25 @@
+// Internal interfaces between the kernel files of the sweep library.
+#pragma once
+#include "common.h"
+
+namespace ptn {
+
+struct Dims { int64_t Dl, d, Dr, Ml, Mr; };
+
+size_t heff_ws_bytes(int prec, const Dims& D);
+size_t env_ws_bytes(int prec, const Dims& D);
+size_t bond_ws_bytes(int prec, int64_t Dl, int64_t Dr, int64_t M);
+
+int heff_apply(int prec, const Dims& D, const void* l, int tl, const void* w, const void* r, int tr,
+               const void* a, int ta, void* out, int tout, void* ws, size_t ws_bytes,
+               hipStream_t s);
+int bond_apply(int prec, int64_t Dl, int64_t Dr, int64_t M, const void* l, int tl, const void* r,
+               int tr, const void* c, int tcin, void* out, int tout, void* ws, size_t ws_bytes,
+               hipStream_t s);
+int env_right(int prec, const Dims& D, const void* a, const void* b, const void* w, const void* r,
+              int tr, void* out, void* ws, size_t ws_bytes, hipStream_t s);
+int env_left(int prec, const Dims& D, const void* a, const void* b, const void* w, const void* l,
+             int tl, void* out, void* ws, size_t ws_bytes, hipStream_t s);
+int convert_c128_c64(int64_t n, const void* x, void* y, hipStream_t s);
+
+}  // namespace ptn

@@ -1,0 +1,178 @@
+// Local contractions of the sweep as two large complex products around an
+// MPO "mode mix":
+//   H_eff / right environment:  T1 = A.R   ->  T2 = W (x) T1  ->  L^T.T2  /  T2.B^H
+//   left environment:           T1 = L.B*  ->  T2 = W^T (x) T1 ->  A^T.T2
+//   bond operator:              T  = C.R   ->  L^T.T
+// The mode mix contracts the (physical, MPO-bond) index pair of T1 with the
+// MPO tensor for every (left bond, right bond) pair; it is HBM-bound and
+// keeps the two products dense and MFMA-shaped.
+#include "common.h"
+#include "ops.h"
+
+namespace ptn {
+
+static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+static inline size_t csize(int prec) { return prec == PTN_FP64 ? 16 : 8; }
+static inline int ctype(int prec) { return prec == PTN_FP64 ? PTN_T_C128 : PTN_T_C64; }
+static inline int gemm_compute(int prec) { return prec == PTN_FP64 ? PTN_FP64 : PTN_FP32; }
+
+// ---------------------------------------------------------------------------
+// W (Ml, d, d, Mr) viewed as the matrix Wmat[(m, sb), (s, m')]; the left
+// environment needs its transpose Wt[(s, m'), (m, sb)].
+__global__ void transpose_w_kernel(int64_t rows, int64_t cols, const double2* __restrict__ w,
+                                   double2* __restrict__ wt) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= rows * cols) return;
+  int64_t i = t / cols, j = t % cols;
+  wt[j * rows + i] = w[t];
+}
+
+__global__ void convert_c128_c64_kernel(int64_t n, const double2* __restrict__ x,
+                                        float2* __restrict__ y) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    y[t] = make_float2((float)x[t].x, (float)x[t].y);
+}
+
+int convert_c128_c64(int64_t n, const void* x, void* y, hipStream_t s) {
+  if (n == 0) return PTN_OK;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  convert_c128_c64_kernel<<<(unsigned)blocks, 256, 0, s>>>(n, (const double2*)x, (float2*)y);
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+// ---------------------------------------------------------------------------
+size_t heff_ws_bytes(int prec, const Dims& D) {
+  size_t t1 = (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
+  size_t t2 = (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
+  return align256(t1) + align256(t2) + 256;
+}
+
+size_t env_ws_bytes(int prec, const Dims& D) {
+  // right step: T1 (Dl*d, Mr*Dr), T2 (Dl*Ml, d*Dr); left step: T1 (Dl*Ml, d*Dr),
+  // T2 (Dl*d, Mr*Dr); plus the transposed MPO tensor
+  size_t a = (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
+  size_t b = (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
+  size_t wt = (size_t)(D.Ml * D.d * D.d * D.Mr) * 16;
+  return 2 * align256(a > b ? a : b) + align256(wt) + 256;
+}
+
+size_t bond_ws_bytes(int prec, int64_t Dl, int64_t Dr, int64_t M) {
+  return align256((size_t)Dl * (size_t)(M * Dr) * csize(prec)) + 256;
+}
+
+static bool dims_ok(const Dims& D) {
+  return D.Dl > 0 && D.d > 0 && D.Dr > 0 && D.Ml > 0 && D.Mr > 0 && D.d * D.Mr <= 32 &&
+         D.Ml * D.d <= 32;
+}
+
+// H_eff: out = L^T . (W (x) (A . R)). `a` and `out` are complex128 or
+// complex64 as given by ta / tout; L and R have types tl / tr.
+int heff_apply(int prec, const Dims& D, const void* l, int tl, const void* w, const void* r, int tr,
+               const void* a, int ta, void* out, int tout, void* ws, size_t ws_bytes,
+               hipStream_t s) {
+  if (!dims_ok(D)) return PTN_ERR_ARG;
+  if (ws_bytes < heff_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;
+  const int ct = ctype(prec), cp = gemm_compute(prec);
+  char* p = (char*)ws;
+  void* t1 = p;
+  p += align256((size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec));
+  void* t2 = p;
+  int st;
+  ZGemm g1 = {D.Dl * D.d, D.Mr * D.Dr, D.Dr,
+              a, D.Dr, 1, 0,
+              r, D.Mr * D.Dr, 1, 0,
+              t1, D.Mr * D.Dr, 1, 0, 1, 0, 0, 0};
+  if ((st = zgemm(g1, cp, ta, tr, ct, nullptr, 0, s))) return st;
+  if ((st = mode_mix(D.Dl, D.d * D.Mr, D.Ml * D.d, D.Dr, w, t1, ct, t2, ct, s))) return st;
+  ZGemm g3 = {D.Dl, D.d * D.Dr, D.Dl * D.Ml,
+              l, 1, D.Dl, 0,
+              t2, D.d * D.Dr, 1, 0,
+              out, D.d * D.Dr, 1, 0, 1, 0, 0, 0};
+  return zgemm(g3, cp, tl, ct, tout, nullptr, 0, s);
+}
+
+int env_right(int prec, const Dims& D, const void* a, const void* b, const void* w, const void* r,
+              int tr, void* out, void* ws, size_t ws_bytes, hipStream_t s) {
+  if (!dims_ok(D)) return PTN_ERR_ARG;
+  if (ws_bytes < env_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;
+  const int ct = ctype(prec), cp = gemm_compute(prec);
+  size_t ta = (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
+  size_t tb = (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
+  char* p = (char*)ws;
+  void* t1 = p;
+  p += align256(ta > tb ? ta : tb);
+  void* t2 = p;
+  int st;
+  ZGemm g1 = {D.Dl * D.d, D.Mr * D.Dr, D.Dr,
+              a, D.Dr, 1, 0,
+              r, D.Mr * D.Dr, 1, 0,
+              t1, D.Mr * D.Dr, 1, 0, 1, 0, 0, 0};
+  if ((st = zgemm(g1, cp, PTN_T_C128, tr, ct, nullptr, 0, s))) return st;
+  if ((st = mode_mix(D.Dl, D.d * D.Mr, D.Ml * D.d, D.Dr, w, t1, ct, t2, ct, s))) return st;
+  // out[(l, m), lb] = sum_{(sb, rb)} T2[(l, m), (sb, rb)] conj(B[lb, (sb, rb)])
+  ZGemm g3 = {D.Dl * D.Ml, D.Dl, D.d * D.Dr,
+              t2, D.d * D.Dr, 1, 0,
+              b, 1, D.d * D.Dr, 1,
+              out, D.Dl, 1, 0, 1, 0, 0, 0};
+  return zgemm(g3, cp, ct, PTN_T_C128, PTN_T_C128, nullptr, 0, s);
+}
+
+int env_left(int prec, const Dims& D, const void* a, const void* b, const void* w, const void* l,
+             int tl, void* out, void* ws, size_t ws_bytes, hipStream_t s) {
+  if (!dims_ok(D)) return PTN_ERR_ARG;
+  if (ws_bytes < env_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;
+  const int ct = ctype(prec), cp = gemm_compute(prec);
+  size_t ta = (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
+  size_t tb = (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
+  size_t big = align256(ta > tb ? ta : tb);
+  char* p = (char*)ws;
+  void* t1 = p;
+  void* t2 = p + big;
+  void* wt = p + 2 * big;
+  int64_t rows = D.Ml * D.d, cols = D.d * D.Mr;
+  transpose_w_kernel<<<(unsigned)((rows * cols + 255) / 256), 256, 0, s>>>(
+      rows, cols, (const double2*)w, (double2*)wt);
+  PTN_LAUNCH_CHECK();
+  int st;
+  // T1[(l, m), (sb, rb)] = sum_lb L[(l, m), lb] conj(B[lb, (sb, rb)])
+  ZGemm g1 = {D.Dl * D.Ml, D.d * D.Dr, D.Dl,
+              l, D.Dl, 1, 0,
+              b, D.d * D.Dr, 1, 1,
+              t1, D.d * D.Dr, 1, 0, 1, 0, 0, 0};
+  if ((st = zgemm(g1, cp, tl, PTN_T_C128, ct, nullptr, 0, s))) return st;
+  // T2[l, (s, m'), rb] = sum_{(m, sb)} W[m, sb, s, m'] T1[l, (m, sb), rb]
+  if ((st = mode_mix(D.Dl, D.Ml * D.d, D.d * D.Mr, D.Dr, wt, t1, ct, t2, ct, s))) return st;
+  // out[r, (m', rb)] = sum_{(l, s)} A[(l, s), r] T2[(l, s), (m', rb)]
+  ZGemm g3 = {D.Dr, D.Mr * D.Dr, D.Dl * D.d,
+              a, 1, D.Dr, 0,
+              t2, D.Mr * D.Dr, 1, 0,
+              out, D.Mr * D.Dr, 1, 0, 1, 0, 0, 0};
+  return zgemm(g3, cp, PTN_T_C128, ct, PTN_T_C128, nullptr, 0, s);
+}
+
+int bond_apply(int prec, int64_t Dl, int64_t Dr, int64_t M, const void* l, int tl, const void* r,
+               int tr, const void* c, int tcin, void* out, int tout, void* ws, size_t ws_bytes,
+               hipStream_t s) {
+  if (Dl <= 0 || Dr <= 0 || M <= 0) return PTN_ERR_ARG;
+  if (ws_bytes < bond_ws_bytes(prec, Dl, Dr, M)) return PTN_ERR_WORKSPACE;
+  const int ct = ctype(prec), cp = gemm_compute(prec);
+  void* t = ws;
+  int st;
+  // T[l, (m, rb)] = sum_r C[l, r] R[r, (m, rb)]
+  ZGemm g1 = {Dl, M * Dr, Dr,
+              c, Dr, 1, 0,
+              r, M * Dr, 1, 0,
+              t, M * Dr, 1, 0, 1, 0, 0, 0};
+  if ((st = zgemm(g1, cp, tcin, tr, ct, nullptr, 0, s))) return st;
+  // out[lb, rb] = sum_{(l, m)} L[(l, m), lb] T[(l, m), rb]
+  ZGemm g3 = {Dl, Dr, Dl * M,
+              l, 1, Dl, 0,
+              t, Dr, 1, 0,
+              out, Dr, 1, 0, 1, 0, 0, 0};
+  return zgemm(g3, cp, tl, ct, tout, nullptr, 0, s);
+}
+
+}  // namespace ptn

@@ -1,0 +1,160 @@
+"""
+End-to-end parity of the GPU sweep drivers (FP64 mode) with the reference
+trajectories (tests/golden) and with closed forms. MPS tensors are compared
+through gauge-invariant quantities: the full state vector (the SVD/QR gauge
+of individual tensors is a convention), norms, energies and bond dimensions.
+"""
+
+import copy
+import warnings
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import load_fixture, mps_arrays, mpo_arrays
+import pytenet_amd as ptn
+
+pytestmark = pytest.mark.gpu
+
+
+def _mps(qsite, qb, a):
+    psi = ptn.MPS(qsite, qb, fill="postpone")
+    psi.a = [x.copy() for x in a]
+    return psi
+
+
+def _mpo(z):
+    qs, qb, a = mpo_arrays(z)
+    return ptn.MPO.from_tensors(qs, qb, a)
+
+
+def _vector(qsite, qb, a):
+    psi = orc.MPS(qsite, qb, fill="postpone")
+    psi.a = a
+    return psi.to_vector()
+
+
+def _phase_aligned(x, ref):
+    z = np.vdot(ref, x)
+    return x * (abs(z) / z)
+
+
+def test_tdvp_singlesite_reference_trajectory():
+    z = load_fixture("traj_tdvp1_xxz_L10")
+    psi = _mps(*mps_arrays(z, "init_"))
+    nrm = ptn.tdvp_singlesite(_mpo(z), psi, complex(z["dt"]), int(z["numsteps"]),
+                              numiter_lanczos=int(z["numiter"]))
+    assert abs(nrm - float(z["nrm"])) < 1e-12
+    ref = _vector(*mps_arrays(z, "final_"))
+    assert isinstance(psi.a[0], np.ndarray)
+    np.testing.assert_allclose(psi.to_vector(), ref, atol=1e-10)
+    assert psi.bond_dims == [a.shape[0] for a in mps_arrays(z, "final_")[2]] + [1]
+
+
+def test_tdvp_twosite_reference_trajectory():
+    z = load_fixture("traj_tdvp2_xxz_L12")
+    psi = _mps(*mps_arrays(z, "init_"))
+    H = _mpo(z)
+    ptn.tdvp_twosite(H, psi, complex(z["dt"]), int(z["numsteps"]),
+                     numiter_lanczos=int(z["numiter"]), tol_split=float(z["tol"]))
+    qsite, qb, a = mps_arrays(z, "final_")
+    assert psi.bond_dims == [x.shape[0] for x in a] + [1]
+    np.testing.assert_allclose(psi.to_vector(), _vector(qsite, qb, a), atol=1e-10)
+    assert abs(ptn.mpo_average(psi, H) - complex(z["energy"])) < 1e-10
+
+
+def test_dmrg_singlesite_reference_trajectory():
+    z = load_fixture("traj_dmrg1_xxz_L16")
+    psi = _mps(*mps_arrays(z, "init_"))
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        en = ptn.dmrg_singlesite(_mpo(z), psi, int(z["numsweeps"]),
+                                 numiter_lanczos=int(z["numiter"]))
+    np.testing.assert_allclose(en, z["en_min"], rtol=1e-10)
+    ref = _vector(*mps_arrays(z, "final_"))
+    np.testing.assert_allclose(_phase_aligned(psi.to_vector(), ref), ref, atol=1e-8)
+
+
+def test_dmrg_twosite_reference_trajectory_quantum_numbers():
+    z = load_fixture("traj_dmrg2_xxz_L10_qn")
+    psi = _mps(*mps_arrays(z, "init_"))
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        en = ptn.dmrg_twosite(_mpo(z), psi, int(z["numsweeps"]), numiter_lanczos=int(z["numiter"]))
+    np.testing.assert_allclose(en, z["en_min"], rtol=1e-10)
+    qsite, qb, a = mps_arrays(z, "final_")
+    for i in range(len(qb)):
+        assert np.array_equal(np.sort(psi.qbonds[i]), np.sort(qb[i]))
+    ref = _vector(qsite, qb, a)
+    np.testing.assert_allclose(_phase_aligned(psi.to_vector(), ref), ref, atol=1e-8)
+
+
+def test_tdvp_quantum_numbers_reference_trajectory():
+    z = load_fixture("traj_tdvp_xxz_L8_qn")
+    H = _mpo(z)
+    psi = _mps(*mps_arrays(z, "init_"))
+    p1, p2 = copy.deepcopy(psi), copy.deepcopy(psi)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        ptn.tdvp_singlesite(H, p1, complex(z["dt"]), int(z["numsteps"]), numiter_lanczos=5)
+        ptn.tdvp_twosite(H, p2, complex(z["dt"]), int(z["numsteps"]), numiter_lanczos=10)
+    np.testing.assert_allclose(p1.to_vector(), _vector(*mps_arrays(z, "one_")), atol=1e-10)
+    np.testing.assert_allclose(p2.to_vector(), _vector(*mps_arrays(z, "two_")), atol=1e-10)
+
+
+def test_tdvp_twosite_matches_exact_evolution():
+    from scipy.linalg import expm
+    rng = np.random.default_rng(17)
+    L, J, D, h = 8, 4/3, 5/13, -2/7
+    H = ptn.heisenberg_xxz_1d_mpo(L, J, D, h).zero_qnumbers()
+    bonds = [1, 2, 4, 8, 16, 8, 4, 2, 1]
+    psi = ptn.MPS(H.qsite, [np.zeros(b, dtype=int) for b in bonds], fill="random", rng=rng)
+    psi.orthonormalize("left")
+    dt, n = 0.02 - 0.05j, 6
+    ref = expm(-dt * n * H.to_matrix()) @ psi.to_vector()
+    ptn.tdvp_twosite(H, psi, dt, n, numiter_lanczos=10)
+    np.testing.assert_allclose(psi.to_vector(), ref, atol=1e-10)
+
+
+def test_dmrg_twosite_matches_exact_diagonalisation():
+    rng = np.random.default_rng(23)
+    L = 10
+    H = ptn.heisenberg_xxz_1d_mpo(L, -4/3, 2/8, 5/7)
+    psi = ptn.MPS.construct_random(L, H.qsite, 4, max_vdim=18, dtype="real", rng=rng)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        en = ptn.dmrg_twosite(H, psi, 2)
+    e_exact = np.linalg.eigvalsh(H.to_matrix())[0]
+    assert abs(en[-1] - e_exact) < 1e-12
+
+
+def test_tdvp_time_reversal_symmetry():
+    rng = np.random.default_rng(29)
+    L = 8
+    H = ptn.heisenberg_xxz_1d_mpo(L, 4/3, 5/13, -2/7).zero_qnumbers()
+    qb = [np.array([0])] + [np.zeros(5, dtype=int) for _ in range(L - 1)] + [np.array([0])]
+    psi = ptn.MPS(H.qsite, qb, fill="random", rng=rng)
+    psi.orthonormalize("left")
+    ref = psi.to_vector()
+    p1 = copy.deepcopy(psi)
+    ptn.tdvp_singlesite(H, p1, 0.5j, 1, numiter_lanczos=10)
+    ptn.tdvp_singlesite(H, p1, -0.5j, 1, numiter_lanczos=10)
+    np.testing.assert_allclose(p1.to_vector(), ref, atol=1e-10)
+
+
+def test_fp32_contractions_keep_fidelity():
+    """Mixed mode: FP32 MFMA contractions, FP64 Krylov/QR/SVD; small 2-site run."""
+    rng = np.random.default_rng(41)
+    L = 10
+    H = ptn.heisenberg_xxz_1d_mpo(L, 1.0, 0.8, -0.1).zero_qnumbers()
+    bonds = [min(16, 2**i, 2**(L - i)) for i in range(L + 1)]
+    psi = ptn.MPS(H.qsite, [np.zeros(b, dtype=int) for b in bonds], fill="random", rng=rng)
+    psi.orthonormalize("left")
+    p64, p32 = copy.deepcopy(psi), copy.deepcopy(psi)
+    ptn.tdvp_twosite(H, p64, -0.05j, 2, numiter_lanczos=8, tol_split=1e-10)
+    ptn.tdvp_twosite(H, p32, -0.05j, 2, numiter_lanczos=8, tol_split=1e-10, precision="fp32")
+    f = abs(ptn.mps_vdot(p64, p32))**2 / (ptn.mps_norm(p64)**2 * ptn.mps_norm(p32)**2)
+    assert abs(1 - f) < 1e-8
+    e64, e32 = ptn.mpo_average(p64, H).real, ptn.mpo_average(p32, H).real
+    assert abs(e32 - e64) < 1e-6 * abs(e64)

@@ -1,0 +1,253 @@
+"""
+Parity of the individual GPU kernels (called through the C ABI) with the
+oracle on the same seeded inputs. FP64 mode: relative error <= 1e-12 of the
+result's max norm; FP32 mode: <= 2e-5.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as orc
+from conftest import load_fixture
+from pytenet_amd import backend, _lib
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 2e-5
+
+
+def relerr(got, ref):
+    got = backend.to_host(got)
+    ref = np.asarray(ref)
+    return np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300)
+
+
+def dev(x):
+    return backend.as_dev(x)
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (7, 5, 3), (33, 65, 17), (130, 70, 9), (64, 128, 64),
+                                   (257, 129, 200)])
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+def test_zgemm_strides_and_ragged_shapes(M, N, K, ta, tb):
+    rng = np.random.default_rng(M * 1000 + N + K)
+    A = orc.crandn((K, M) if ta else (M, K), rng)
+    B = orc.crandn((N, K) if tb else (K, N), rng)
+    Ad, Bd = dev(A), dev(B)
+    opA = Ad.transpose(0, 1) if ta else Ad
+    opB = Bd.transpose(0, 1) if tb else Bd
+    refA = A.T if ta else A
+    refB = B.T if tb else B
+    assert relerr(backend.matmul(opA, opB), refA @ refB) < FP64_TOL
+    assert relerr(backend.matmul(opA, opB, conjA=True, conjB=True),
+                  refA.conj() @ refB.conj()) < FP64_TOL
+
+
+def test_zgemm_accumulate():
+    rng = np.random.default_rng(2)
+    A, B, C = orc.crandn((40, 30), rng), orc.crandn((30, 50), rng), orc.crandn((40, 50), rng)
+    Cd = dev(C)
+    backend.matmul(dev(A), dev(B), out=Cd, accumulate=True)
+    assert relerr(Cd, C + A @ B) < FP64_TOL
+
+
+def test_local_kernels_match_reference_fixture():
+    z = load_fixture("kernels_dense")
+    a, b, w, l, r, c, rb = (z[k] for k in ("a", "b", "w", "l", "r", "c", "rb"))
+    assert relerr(backend.heff_apply(a, w, l, r), z["heff"]) < FP64_TOL
+    assert relerr(backend.bond_apply(c, l, rb), z["bond"]) < FP64_TOL
+    assert relerr(backend.env_left(a, b, w, l), z["env_left"]) < FP64_TOL
+    assert relerr(backend.env_right(a, b, w, r), z["env_right"]) < FP64_TOL
+
+
+SHAPES = [(1, 2, 1, 1, 4), (3, 2, 5, 4, 5), (17, 2, 23, 5, 5), (40, 4, 36, 5, 5),
+          (64, 2, 64, 3, 3), (100, 4, 90, 5, 4)]
+
+
+@pytest.mark.parametrize("Dl,d,Dr,Ml,Mr", SHAPES)
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL)])
+def test_local_kernels_random(Dl, d, Dr, Ml, Mr, precision, tol):
+    rng = np.random.default_rng(Dl * 7 + Dr)
+    a = orc.crandn((Dl, d, Dr), rng)
+    b = orc.crandn((Dl, d, Dr), rng)
+    w = orc.crandn((Ml, d, d, Mr), rng)
+    l = orc.crandn((Dl, Ml, Dl), rng)
+    r = orc.crandn((Dr, Mr, Dr), rng)
+    c = orc.crandn((Dl, Dr), rng)
+    rm = orc.crandn((Dr, Ml, Dr), rng)
+    assert relerr(backend.heff_apply(a, w, l, r, precision),
+                  orc.apply_local_hamiltonian(a, w, l, r)) < tol
+    assert relerr(backend.env_left(a, b, w, l, precision),
+                  orc.contraction_operator_step_left(a, b, w, l)) < tol
+    assert relerr(backend.env_right(a, b, w, r, precision),
+                  orc.contraction_operator_step_right(a, b, w, r)) < tol
+    assert relerr(backend.bond_apply(c, l, rm, precision),
+                  orc.apply_local_bond_contraction(c, l, rm)) < tol
+
+
+def _hermitian_env(rng, D, M):
+    """Environment blocks of a physical chain are Hermitian in (ket, bra) for each m
+    when the MPO is Hermitian; random Hermitian blocks make H_eff Hermitian."""
+    x = orc.crandn((D, M, D), rng)
+    return 0.5 * (x + x.transpose(2, 1, 0).conj())
+
+
+def _heff_problem(seed, Dl=12, d=2, Dr=10, M=3):
+    rng = np.random.default_rng(seed)
+    w = orc.crandn((M, d, d, M), rng)
+    # Hermitian physical blocks: W[m, s', s, m'] = conj(W[m, s, s', m'])
+    w = 0.5 * (w + w.transpose(0, 2, 1, 3).conj())
+    l = _hermitian_env(rng, Dl, M)
+    r = _hermitian_env(rng, Dr, M)
+    a = orc.crandn((Dl, d, Dr), rng)
+    return a, w, l, r
+
+
+def test_heff_is_hermitian_for_hermitian_blocks():
+    a, w, l, r = _heff_problem(1)
+    rng = np.random.default_rng(9)
+    b = orc.crandn(a.shape, rng)
+    ha = backend.to_host(backend.heff_apply(a, w, l, r))
+    hb = backend.to_host(backend.heff_apply(b, w, l, r))
+    assert abs(np.vdot(b, ha) - np.vdot(hb, a)) < 1e-12 * abs(np.vdot(b, ha))
+
+
+@pytest.mark.parametrize("numiter", [1, 4, 8, 25])
+def test_krylov_expm_matches_oracle(numiter):
+    a, w, l, r = _heff_problem(numiter)
+    dt = -0.05j + 0.01
+    out, _, info = backend.krylov_local(_lib.PTN_OP_HEFF, a, l, w, r, numiter, "expm", dt)
+    ref = orc.expm_krylov(lambda x: orc.apply_local_hamiltonian(x.reshape(a.shape), w, l, r).reshape(-1),
+                          a.reshape(-1), dt, numiter, hermitian=True).reshape(a.shape)
+    assert relerr(out, ref) < 1e-11
+    inf = info.cpu().numpy()
+    assert inf[0] == numiter and inf[1] == 0 and inf[2] == 0
+
+
+def test_krylov_eigh_matches_oracle():
+    a, w, l, r = _heff_problem(5)
+    out, en, info = backend.krylov_local(_lib.PTN_OP_HEFF, a, l, w, r, 12, "eigh")
+    ev, u = orc.eigh_krylov(lambda x: orc.apply_local_hamiltonian(x.reshape(a.shape), w, l, r).reshape(-1),
+                            a.reshape(-1), 12, 1)
+    assert abs(float(en.item()) - ev[0]) < 1e-11 * abs(ev[0])
+    got = backend.to_host(out).reshape(-1)
+    ref = u[:, 0]
+    # eigenvector sign is a convention of the tridiagonal solver
+    phase = np.vdot(ref, got)
+    assert abs(abs(phase) - 1) < 1e-10
+    assert np.abs(got - phase * ref).max() < 1e-9
+
+
+def test_krylov_bond_expm_matches_oracle():
+    rng = np.random.default_rng(31)
+    D1, D2, M = 9, 11, 4
+    l = _hermitian_env(rng, D1, M)
+    r = _hermitian_env(rng, D2, M)
+    c = orc.crandn((D1, D2), rng)
+    out, _, _ = backend.krylov_local(_lib.PTN_OP_BOND, c, l, None, r, 6, "expm", 0.025j)
+    ref = orc.expm_krylov(lambda x: orc.apply_local_bond_contraction(x.reshape(c.shape), l, r).reshape(-1),
+                          c.reshape(-1), 0.025j, 6, hermitian=True).reshape(c.shape)
+    assert relerr(out, ref) < 1e-11
+
+
+def test_krylov_breakdown_on_invariant_subspace():
+    """A start vector in a 2-dimensional invariant subspace of H_eff stops after 2 vectors."""
+    Dl, d, Dr = 1, 4, 1
+    w = np.zeros((1, d, d, 1), dtype=complex)
+    w[0, :, :, 0] = np.diag([1.0, 2.0, 3.0, 4.0])
+    one = np.ones((1, 1, 1))
+    a = np.array([1.0, 1.0, 0.0, 0.0], dtype=complex).reshape(Dl, d, Dr)
+    out, _, info = backend.krylov_local(_lib.PTN_OP_HEFF, a, one, w, one, 4, "expm", -0.3j)
+    inf = info.cpu().numpy()
+    assert inf[0] == 2 and inf[1] == 2
+    with pytest.warns(RuntimeWarning):
+        ref = orc.expm_krylov(lambda x: (np.diag([1., 2, 3, 4]) @ x), a.reshape(-1), -0.3j, 4)
+    assert relerr(out.reshape(-1), ref) < 1e-12
+    with pytest.warns(RuntimeWarning, match="beta\\[1\\]"):
+        backend.report_krylov_status(inf)
+
+
+def test_krylov_zero_start_vector_raises():
+    a, w, l, r = _heff_problem(3)
+    _, _, info = backend.krylov_local(_lib.PTN_OP_HEFF, np.zeros_like(a), l, w, r, 4, "expm", 0.1)
+    with pytest.raises(AssertionError):
+        backend.report_krylov_status(info.cpu().numpy())
+
+
+def test_krylov_nonfinite_input_raises():
+    a, w, l, r = _heff_problem(4)
+    a = a.copy()
+    a[0, 0, 0] = np.nan
+    _, _, info = backend.krylov_local(_lib.PTN_OP_HEFF, a, l, w, r, 4, "expm", 0.1)
+    with pytest.raises(FloatingPointError):
+        backend.report_krylov_status(info.cpu().numpy())
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (5, 3), (3, 5), (16, 8), (33, 33), (96, 40), (40, 96)])
+def test_qr_matches_lapack(m, n):
+    rng = np.random.default_rng(m * 100 + n)
+    a = orc.crandn((m, n), rng)
+    q, r = backend.qr(dev(a))
+    qr_, rr_ = np.linalg.qr(a)
+    assert relerr(q, qr_) < 1e-12
+    assert relerr(r, rr_) < 1e-12
+
+
+def test_qr_rank_deficient_matches_lapack():
+    rng = np.random.default_rng(77)
+    a = orc.crandn((12, 8), rng)
+    a[:, 3] = 0
+    a[:, 6] = 0
+    a[9:, :] = 0
+    q, r = backend.qr(dev(a))
+    qr_, rr_ = np.linalg.qr(a)
+    assert relerr(q, qr_) < 1e-12
+    assert relerr(r, rr_) < 1e-12
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (6, 4), (4, 6), (24, 24), (64, 30), (30, 64), (130, 128)])
+def test_svd_matches_lapack(m, n):
+    rng = np.random.default_rng(m * 31 + n)
+    a = orc.crandn((m, n), rng)
+    # graded singular values down to 1e-9 relative
+    u0, _, v0 = np.linalg.svd(a, full_matrices=False)
+    k = min(m, n)
+    s0 = np.logspace(0, -9, k)
+    a = (u0 * s0) @ v0
+    u, s, vh = backend.svd(dev(a))
+    s = s.cpu().numpy()
+    np.testing.assert_allclose(s, s0, rtol=1e-9, atol=1e-14)
+    uh = backend.to_host(u)
+    vhh = backend.to_host(vh)
+    np.testing.assert_allclose(uh.conj().T @ uh, np.identity(k), atol=1e-12)
+    np.testing.assert_allclose(vhh @ vhh.conj().T, np.identity(k), atol=1e-12)
+    np.testing.assert_allclose((uh * s) @ vhh, a, atol=1e-13)
+
+
+def test_block_sparse_factorisations_match_reference_fixture():
+    z = load_fixture("kernels_blocksparse")
+    q, r, qi = backend.block_sparse_qr(z["qr_a"], z["qr_q0"], z["qr_q1"])
+    assert relerr(q, z["qr_q"]) < 1e-12 and relerr(r, z["qr_r"]) < 1e-12
+    assert np.array_equal(qi, z["qr_qi"])
+    u, s, v, qs = backend.block_sparse_svd(z["svd_a"], z["svd_q0"], z["svd_q1"])
+    np.testing.assert_allclose(s, z["svd_s"], rtol=1e-12, atol=1e-14)
+    assert np.array_equal(qs, z["svd_q"])
+    uh, vh = backend.to_host(u), backend.to_host(v)
+    np.testing.assert_allclose((uh * s) @ vh, z["svd_a"], atol=1e-13)
+
+
+def test_qsparse_kernels():
+    rng = np.random.default_rng(6)
+    q0, q1, q2 = rng.integers(-2, 3, 5), np.array([1, -1]), rng.integers(-2, 3, 7)
+    a = orc.crandn((5, 2, 7), rng)
+    mask = np.add.outer(np.add.outer(q0, q1), q2) != 0
+    t = dev(a)
+    assert backend.qsparse_violation(t, [q0, q1, q2]) == pytest.approx(
+        np.max(np.maximum(np.abs(a.real), np.abs(a.imag))[mask]))
+    backend.enforce_qsparsity(t, [q0, q1, q2])
+    ref = a.copy()
+    ref[mask] = 0
+    assert np.array_equal(backend.to_host(t), ref)
+    assert backend.qsparse_violation(t, [q0, q1, q2]) == 0
