@@ -285,9 +285,10 @@ def qr(a):
     return q, r
 
 
-def svd(a):
+def svd(a, return_info=False):
     """Thin SVD of a 2-D complex128 device tensor: (u, s, vh), s descending (device float64).
-    Synchronises the stream (the Jacobi sweeps are checked for convergence on the host)."""
+    Synchronises the stream (the Jacobi sweeps are checked for convergence on the host).
+    With return_info, also returns the status words (sweeps, not converged, non-finite)."""
     a = as_dev(a)
     m, n = a.shape
     k = min(m, n)
@@ -305,7 +306,10 @@ def svd(a):
     if inf[2]:
         raise FloatingPointError("non-finite singular value")
     if inf[1]:
-        raise np.linalg.LinAlgError("SVD did not converge")
+        warnings.warn(f"Jacobi SVD of a {m} x {n} matrix stopped after {inf[0]} sweeps "
+                      "before every column pair met the orthogonality tolerance", RuntimeWarning)
+    if return_info:
+        return u, s, vh, inf
     return u, s, vh
 
 

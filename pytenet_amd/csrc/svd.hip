@@ -1,24 +1,38 @@
-// Thin SVD of a complex128 matrix by one-sided (Hestenes) Jacobi in FP64.
+// Thin SVD of a complex128 matrix: QR preconditioning followed by blocked
+// one-sided (Hestenes) Jacobi, all in FP64.
 //
-// X = A (m >= n) or X = A^H (m < n) is held column-major; pairs of columns
-// are orthogonalised by plane rotations until every pair satisfies
-// |x_p^H x_q| <= tol * |x_p| |x_q|. Then sigma_j = |x_j|, U_j = x_j / sigma_j
-// and V accumulates the rotations: X V = U diag(sigma). One-sided Jacobi
-// computes small singular values to high relative accuracy, which the
-// discarded-weight truncation relies on.
+// With B = A (m >= n) or B = A^H (m < n), a p x q matrix with p >= q:
+//   1. B = Q R by the Householder QR of qr.hip;
+//   2. X = R^H (q x q, column-major, padded with zero columns to a multiple of
+//      2*JB) is orthogonalised by one-sided Jacobi, X V = U_X diag(sigma);
+//      starting from R^H instead of B cuts the number of sweeps several-fold
+//      for graded spectra (QR-preconditioned Jacobi SVD, Drmac & Veselic);
+//   3. B = (Q V) diag(sigma) U_X^H, so U_B = Q V (one complex product) and V_B = U_X.
+// Jacobi: columns are grouped in blocks of JB; a round of the block tournament
+// pairs every block with another and each block pair (2*JB columns) is
+// orthogonalised by
+//   k_jac_gram:  partial Gram matrices G = Y^H Y of the pair's columns over row segments;
+//   k_jac_solve: one cyclic sweep of two-sided Jacobi on G in LDS (parallel
+//                ordering, JB disjoint rotations per step), accumulating the unitary J;
+//   k_jac_apply: Y <- Y J and V <- V J, one row per thread.
+// Sweeps repeat until no pair needs a rotation, i.e. every column pair
+// satisfies |x_i^H x_j| <= tol |x_i| |x_j|. Then sigma_j = |x_j| and
+// U_X = X / sigma. One-sided Jacobi computes small singular values to high
+// relative accuracy, which the discarded-weight truncation relies on.
 //
-// Column pairs follow the round-robin (circle) tournament, so each round is
-// q/2 independent pair rotations, one workgroup each. The host checks the
-// rotation count once per sweep.
-//
-// Rotation for a pair (x, y) with a = |x|^2, b = |y|^2, c = x^H y = |c| e^{i phi}:
+// Plane rotation for columns (x, y) with a = |x|^2, b = |y|^2, c = x^H y = |c| e^{i phi}:
 //   zeta = (b - a) / (2|c|), t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)),
-//   cs = 1 / sqrt(1 + t^2), sn = cs t,
-//   x' = cs x - sn e^{-i phi} y,   y' = sn e^{i phi} x + cs y.
+//   cs = 1 / sqrt(1 + t^2), s = cs t e^{i phi},
+//   x' = cs x - conj(s) y,   y' = s x + cs y.
 #include <float.h>
 #include "common.h"
 
 namespace ptn {
+
+constexpr int JB = 16;          // columns per block
+constexpr int JP = 2 * JB;      // columns per block pair
+constexpr int SEG = 128;        // rows per workgroup segment (apply: one row per thread)
+constexpr int GCH = 64;         // rows per LDS chunk in the Gram kernel
 
 __device__ double wg_sum3(double v, double* red) {
   const int t = threadIdx.x;
@@ -33,86 +47,179 @@ __device__ double wg_sum3(double v, double* red) {
   return s;
 }
 
-// X (column-major p x q): from A (m x n, lda) directly or as A^H
-__global__ void k_svd_load(int64_t m, int64_t n, const double2* __restrict__ a, int64_t lda,
-                           int adjoint, double2* __restrict__ X) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= m * n) return;
-  int64_t i = t / n, j = t % n;
-  double2 v = a[i * lda + j];
-  if (!adjoint) X[j * m + i] = v;                              // X = A, p = m
-  else X[i * n + j] = make_double2(v.x, -v.y);                 // X = A^H, p = n, X[j][i]
-}
-
 __global__ void k_eye(int64_t q, double2* __restrict__ V) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= q * q) return;
-  V[t] = make_double2((t / q) == (t % q) ? 1.0 : 0.0, 0.0);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < q * q;
+       t += (int64_t)gridDim.x * blockDim.x)
+    V[t] = make_double2((t / q) == (t % q) ? 1.0 : 0.0, 0.0);
 }
 
-__device__ __forceinline__ void round_pair(int64_t qe, int64_t r, int64_t k, int64_t& p,
-                                           int64_t& q) {
-  const int64_t cyc = qe - 1;
+// pair k of round r of the round-robin (circle) tournament over ne players
+__device__ __forceinline__ void round_pair(int ne, int r, int k, int& p, int& q) {
+  const int cyc = ne - 1;
   if (k == 0) { p = r; q = cyc; }
   else { p = (r + k) % cyc; q = (r - k + cyc) % cyc; }
-  if (p > q) { int64_t t = p; p = q; q = t; }
+  if (p > q) { int t = p; p = q; q = t; }
 }
 
-__global__ __launch_bounds__(256) void k_jacobi_round(int64_t pr, int64_t qn, int64_t qe,
-                                                      int64_t r, double2* __restrict__ X,
-                                                      double2* __restrict__ V, double tol,
-                                                      int* __restrict__ rotations) {
-  __shared__ double red[256];
-  int64_t p, q;
-  round_pair(qe, r, blockIdx.x, p, q);
-  if (q >= qn) return;   // padding column
-  double2* x = X + p * pr;
-  double2* y = X + q * pr;
-  double a = 0, b = 0, cr = 0, ci = 0;
-  for (int64_t i = threadIdx.x; i < pr; i += 256) {
-    double2 u = x[i], v = y[i];
-    a += u.x * u.x + u.y * u.y;
-    b += v.x * v.x + v.y * v.y;
-    cr += u.x * v.x + u.y * v.y;     // conj(u) v
-    ci += u.x * v.y - u.y * v.x;
+// column index (in X / V) of local column c (0..JP-1) of the pair (bp, bq)
+__device__ __forceinline__ int64_t pair_col(int bp, int bq, int c) {
+  return (c < JB) ? (int64_t)bp * JB + c : (int64_t)bq * JB + (c - JB);
+}
+
+// partial Gram matrices: Gp[pair][seg][i][j] = sum_{rows in seg} conj(Y[row][i]) Y[row][j]
+__global__ __launch_bounds__(256) void k_jac_gram(int64_t p, int nblk, int r,
+                                                  const double2* __restrict__ X,
+                                                  double2* __restrict__ Gp, int nseg) {
+  __shared__ double2 Y[GCH][JP + 1];
+  int bp, bq;
+  round_pair(nblk, r, blockIdx.y, bp, bq);
+  const int seg = blockIdx.x;
+  const int64_t r0 = (int64_t)seg * SEG;
+  const int64_t r1 = (r0 + SEG < p) ? r0 + SEG : p;
+  const int t = threadIdx.x;
+  // thread t owns the 4 entries (i, j) = (t / 8, 4 * (t % 8) + e)
+  const int gi = t >> 3, gj = 4 * (t & 7);
+  double gr[4] = {0, 0, 0, 0}, gim[4] = {0, 0, 0, 0};
+  for (int64_t c0 = r0; c0 < r1; c0 += GCH) {
+    __syncthreads();
+    for (int e = t; e < GCH * JP; e += 256) {
+      const int col = e / GCH, row = e % GCH;
+      const int64_t grow = c0 + row;
+      Y[row][col] = (grow < r1) ? X[pair_col(bp, bq, col) * p + grow] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int row = 0; row < GCH; ++row) {
+      const double2 yi = Y[row][gi];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const double2 yj = Y[row][gj + e];
+        gr[e] += yi.x * yj.x + yi.y * yj.y;
+        gim[e] += yi.x * yj.y - yi.y * yj.x;
+      }
+    }
   }
-  a = wg_sum3(a, red);
-  b = wg_sum3(b, red);
-  cr = wg_sum3(cr, red);
-  ci = wg_sum3(ci, red);
-  const double ac = hypot(cr, ci);
-  if (!(ac > tol * sqrt(a) * sqrt(b))) return;   // also skips zero columns
-  const double zeta = (b - a) / (2.0 * ac);
-  const double t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-  const double cs = 1.0 / sqrt(1.0 + t * t);
-  const double sn = cs * t;
-  const double er = cr / ac, ei = ci / ac;   // e^{i phi}
-  // x' = cs x - sn conj(e) y ; y' = sn e x + cs y
-  const double s1r = sn * er, s1i = -sn * ei;   // sn e^{-i phi}
-  const double s2r = sn * er, s2i = sn * ei;    // sn e^{i phi}
-  for (int64_t i = threadIdx.x; i < pr; i += 256) {
-    double2 u = x[i], v = y[i];
-    double2 nu, nv;
-    nu.x = cs * u.x - (s1r * v.x - s1i * v.y);
-    nu.y = cs * u.y - (s1r * v.y + s1i * v.x);
-    nv.x = (s2r * u.x - s2i * u.y) + cs * v.x;
-    nv.y = (s2r * u.y + s2i * u.x) + cs * v.y;
-    x[i] = nu;
-    y[i] = nv;
+  double2* out = Gp + ((int64_t)blockIdx.y * nseg + seg) * JP * JP;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) out[gi * JP + gj + e] = make_double2(gr[e], gim[e]);
+}
+
+// one cyclic Jacobi sweep on the pair's Gram matrix (sum of the partials,
+// fixed order); writes the accumulated unitary J and whether any rotation
+// was needed (one workgroup per pair)
+__global__ __launch_bounds__(256) void k_jac_solve(int nblk, const double2* __restrict__ Gp,
+                                                   int nseg, double tol, double big,
+                                                   double2* __restrict__ Jg,
+                                                   int* __restrict__ rotflag,
+                                                   int* __restrict__ counters) {
+  __shared__ double2 G[JP][JP + 1];
+  __shared__ double2 J[JP][JP + 1];
+  __shared__ double rc[JB], rsr[JB], rsi[JB];
+  __shared__ int rot_any, rot_big;
+  const int t = threadIdx.x;
+  const double2* gsrc = Gp + (int64_t)blockIdx.x * nseg * JP * JP;
+  for (int e = t; e < JP * JP; e += 256) {
+    double sr = 0, si = 0;
+    for (int s = 0; s < nseg; ++s) {
+      const double2 v = gsrc[(int64_t)s * JP * JP + e];
+      sr += v.x;
+      si += v.y;
+    }
+    G[e / JP][e % JP] = make_double2(sr, si);
+    J[e / JP][e % JP] = make_double2((e / JP) == (e % JP) ? 1.0 : 0.0, 0.0);
   }
-  double2* vx = V + p * qn;
-  double2* vy = V + q * qn;
-  for (int64_t i = threadIdx.x; i < qn; i += 256) {
-    double2 u = vx[i], v = vy[i];
-    double2 nu, nv;
-    nu.x = cs * u.x - (s1r * v.x - s1i * v.y);
-    nu.y = cs * u.y - (s1r * v.y + s1i * v.x);
-    nv.x = (s2r * u.x - s2i * u.y) + cs * v.x;
-    nv.y = (s2r * u.y + s2i * u.x) + cs * v.y;
-    vx[i] = nu;
-    vy[i] = nv;
+  if (t == 0) { rot_any = 0; rot_big = 0; }
+  __syncthreads();
+  for (int step = 0; step < JP - 1; ++step) {
+    if (t < JB) {
+      int i, j;
+      round_pair(JP, step, t, i, j);
+      const double a = G[i][i].x, b = G[j][j].x;
+      const double2 c = G[i][j];
+      const double ac = hypot(c.x, c.y);
+      const double scale = sqrt(fmax(a, 0.0)) * sqrt(fmax(b, 0.0));
+      double cs = 1.0, sr = 0.0, si = 0.0;
+      if (ac > 0.0 && ac > tol * scale) {
+        const double zeta = (b - a) / (2.0 * ac);
+        const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        cs = 1.0 / sqrt(1.0 + tt * tt);
+        const double sn = cs * tt;
+        sr = sn * (c.x / ac);   // s = sn e^{i phi}
+        si = sn * (c.y / ac);
+        rot_any = 1;
+        if (ac > big * scale) rot_big = 1;
+      }
+      rc[t] = cs; rsr[t] = sr; rsi[t] = si;
+    }
+    __syncthreads();
+    // columns of G and J: x' = cs x - conj(s) y, y' = s x + cs y
+    for (int e = t; e < JB * JP * 2; e += 256) {
+      const int k = e % JB, row = (e / JB) % JP, mat = e / (JB * JP);
+      int i, j;
+      round_pair(JP, step, k, i, j);
+      const double cs = rc[k], sr = rsr[k], si = rsi[k];
+      double2 (*M)[JP + 1] = mat ? J : G;
+      const double2 x = M[row][i], y = M[row][j];
+      // conj(s) y = (sr y.x + si y.y, sr y.y - si y.x);  s x = (sr x.x - si x.y, sr x.y + si x.x)
+      M[row][i] = make_double2(cs * x.x - (sr * y.x + si * y.y), cs * x.y - (sr * y.y - si * y.x));
+      M[row][j] = make_double2((sr * x.x - si * x.y) + cs * y.x, (sr * x.y + si * x.x) + cs * y.y);
+    }
+    __syncthreads();
+    // rows of G (left multiplication by J^H): row_i' = cs row_i - s row_j,
+    // row_j' = conj(s) row_i + cs row_j
+    for (int e = t; e < JB * JP; e += 256) {
+      const int k = e % JB, col = e / JB;
+      int i, j;
+      round_pair(JP, step, k, i, j);
+      const double cs = rc[k], sr = rsr[k], si = rsi[k];
+      const double2 x = G[i][col], y = G[j][col];
+      G[i][col] = make_double2(cs * x.x - (sr * y.x - si * y.y), cs * x.y - (sr * y.y + si * y.x));
+      G[j][col] = make_double2((sr * x.x + si * x.y) + cs * y.x, (sr * x.y - si * x.x) + cs * y.y);
+    }
+    __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(rotations, 1);
+  double2* jout = Jg + (int64_t)blockIdx.x * JP * JP;
+  for (int e = t; e < JP * JP; e += 256) jout[e] = J[e / JP][e % JP];
+  if (t == 0) {
+    rotflag[blockIdx.x] = rot_any;
+    if (rot_big) atomicAdd(counters, 1);
+  }
+}
+
+// Y <- Y J and V <- V J for the rows of segment blockIdx.x (one row per thread)
+__global__ __launch_bounds__(128) void k_jac_apply(int64_t p, int64_t qp, int nblk, int r,
+                                                   double2* __restrict__ X,
+                                                   double2* __restrict__ V,
+                                                   const double2* __restrict__ Jg,
+                                                   const int* __restrict__ rotflag) {
+  __shared__ double2 J[JP][JP + 1];
+  if (!rotflag[blockIdx.y]) return;
+  int bp, bq;
+  round_pair(nblk, r, blockIdx.y, bp, bq);
+  const int t = threadIdx.x;
+  const double2* jsrc = Jg + (int64_t)blockIdx.y * JP * JP;
+  for (int e = t; e < JP * JP; e += SEG) J[e / JP][e % JP] = jsrc[e];
+  __syncthreads();
+  for (int pass = 0; pass < 2; ++pass) {
+    double2* M = pass ? V : X;
+    const int64_t ld = pass ? qp : p;
+    const int64_t row = (int64_t)blockIdx.x * SEG + t;
+    if (row < ld) {
+      double2 y[JP];
+#pragma unroll
+      for (int c = 0; c < JP; ++c) y[c] = M[pair_col(bp, bq, c) * ld + row];
+#pragma unroll 4
+      for (int c = 0; c < JP; ++c) {
+        double sr = 0, si = 0;
+#pragma unroll
+        for (int kk = 0; kk < JP; ++kk) {
+          const double2 jk = J[kk][c];
+          sr += y[kk].x * jk.x - y[kk].y * jk.y;
+          si += y[kk].x * jk.y + y[kk].y * jk.x;
+        }
+        M[pair_col(bp, bq, c) * ld + row] = make_double2(sr, si);
+      }
+    }
+  }
 }
 
 // column norms of X
@@ -130,12 +237,12 @@ __global__ __launch_bounds__(256) void k_colnorms(int64_t pr, const double2* __r
 }
 
 // descending order of the q column norms (stable: ties keep column order),
-// one workgroup; rank of column j = #{i : s_i > s_j} + #{i < j : s_i == s_j}
+// one workgroup; rank of column j = #{i : s_i > s_j} + #{i < j : s_i == s_j}.
+// A non-finite norm is ordered as -1 (norms are >= 0) so that the ranks stay a
+// permutation; the flag reports it.
 __global__ __launch_bounds__(256) void k_rank(int64_t q, const double* __restrict__ nrm,
                                               int64_t* __restrict__ perm, double* __restrict__ s_out,
                                               int* __restrict__ nonfinite) {
-  // a non-finite norm is ordered as -1 (norms are >= 0) so that the ranks
-  // stay a permutation; the flag reports it
   for (int64_t j = threadIdx.x; j < q; j += 256) {
     const double sj = nrm[j];
     if (!isfinite(sj)) atomicOr(nonfinite, 1);
@@ -150,39 +257,65 @@ __global__ __launch_bounds__(256) void k_rank(int64_t q, const double* __restric
   }
 }
 
-// outputs in row-major: for X = A: U[i][c] = X[perm c][i] / s_c, Vh[c][j] = conj(V[j][perm c]);
-// for X = A^H: U[i][c] = V[i][perm c], Vh[c][j] = conj(X[j][perm c]) / s_c
-__global__ void k_svd_out(int64_t m, int64_t n, int64_t k, int adjoint, const double2* __restrict__ X,
-                          const double2* __restrict__ V, const int64_t* __restrict__ perm,
-                          const double* __restrict__ s, double2* __restrict__ u, int64_t ldu,
-                          double2* __restrict__ vh, int64_t ldvh) {
+// row-major outputs, pc = perm[c]:
+//   m >= n: U[i][c] = UB[i][pc],           Vh[c][j] = conj(X[j][pc]) / s_c
+//   m <  n: U[i][c] = X[i][pc] / s_c,       Vh[c][j] = conj(UB[j][pc])
+// X is column-major with leading dimension q; UB is row-major p x q.
+__global__ void k_svd_out(int64_t m, int64_t n, int64_t k, int adjoint, int64_t q,
+                          const double2* __restrict__ X, const double2* __restrict__ UB,
+                          const int64_t* __restrict__ perm, const double* __restrict__ s,
+                          double2* __restrict__ u, int64_t ldu, double2* __restrict__ vh,
+                          int64_t ldvh) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < m * k) {
-    int64_t i = t / k, c = t % k;
-    int64_t pc = perm[c];
+    const int64_t i = t / k, c = t % k, pc = perm[c];
     double2 val;
     if (!adjoint) {
-      double2 x = X[pc * m + i];
-      double sc = s[c];
-      val = sc > 0 ? make_double2(x.x / sc, x.y / sc) : make_double2(0.0, 0.0);
+      val = UB[i * q + pc];
     } else {
-      val = V[pc * m + i];     // V is m x m here (q = m)
+      const double2 x = X[pc * q + i];
+      const double sc = s[c];
+      val = sc > 0 ? make_double2(x.x / sc, x.y / sc) : make_double2(0.0, 0.0);
     }
     u[i * ldu + c] = val;
   }
   if (t < k * n) {
-    int64_t c = t / n, j = t % n;
-    int64_t pc = perm[c];
+    const int64_t c = t / n, j = t % n, pc = perm[c];
     double2 val;
     if (!adjoint) {
-      double2 v = V[pc * n + j];   // V is n x n (q = n), column pc
-      val = make_double2(v.x, -v.y);
-    } else {
-      double2 x = X[pc * n + j];   // X is n x m column-major (p = n)
-      double sc = s[c];
+      const double2 x = X[pc * q + j];
+      const double sc = s[c];
       val = sc > 0 ? make_double2(x.x / sc, -x.y / sc) : make_double2(0.0, 0.0);
+    } else {
+      const double2 v = UB[j * q + pc];
+      val = make_double2(v.x, -v.y);
     }
     vh[c * ldvh + j] = val;
+  }
+}
+
+// B = A^H (row-major n x m) from A (m x n, lda)
+__global__ void k_adjoint(int64_t m, int64_t n, const double2* __restrict__ a, int64_t lda,
+                          double2* __restrict__ b) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m * n) return;
+  const int64_t i = t / n, j = t % n;
+  const double2 v = a[i * lda + j];
+  b[j * m + i] = make_double2(v.x, -v.y);
+}
+
+// X (column-major q x qp) = R^H for the row-major q x q R: X[c][r] = conj(R[c][r]) in
+// memory order; padding columns are zero
+__global__ void k_load_rh(int64_t q, int64_t qp, const double2* __restrict__ R,
+                          double2* __restrict__ X) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < q * qp;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    if (t < q * q) {
+      const double2 v = R[t];
+      X[t] = make_double2(v.x, -v.y);
+    } else {
+      X[t] = make_double2(0.0, 0.0);
+    }
   }
 }
 
@@ -208,59 +341,123 @@ static int* pinned_counter() {
   return p;
 }
 
-size_t svd_ws_bytes(int64_t m, int64_t n) {
-  const int64_t q = m < n ? m : n, p = m < n ? n : m;
-  return al((size_t)p * q * 16) + al((size_t)q * q * 16) + al((size_t)q * 8) * 2 +
-         al((size_t)q * 8) + al(16);
+size_t qr_ws_bytes(int64_t m, int64_t n);
+int qr_run(int64_t m, int64_t n, const void* a, int64_t lda, void* q, int64_t ldq, void* r,
+           int64_t ldr, void* ws, size_t ws_bytes, hipStream_t s);
+
+struct SvdLayout {
+  int64_t p, q, qp, nseg;
+  int nblk;
+  size_t bt, qm, rm, qrws, x, v, ub, gp, jg, rflag, nrm, perm, cnt, total;
+};
+
+static SvdLayout svd_layout(int64_t m, int64_t n) {
+  SvdLayout L;
+  L.q = m < n ? m : n;
+  L.p = m < n ? n : m;
+  L.qp = ((L.q + JP - 1) / JP) * JP;
+  L.nblk = (int)(L.qp / JB);
+  L.nseg = (L.q + SEG - 1) / SEG;     // Jacobi runs on X = R^H with q rows
+  const int npairs = L.nblk / 2;
+  size_t off = 0;
+  L.bt = off; off += (m < n) ? al((size_t)L.p * L.q * 16) : 0;
+  L.qm = off; off += al((size_t)L.p * L.q * 16);
+  L.rm = off; off += al((size_t)L.q * L.q * 16);
+  L.qrws = off; off += al(qr_ws_bytes(L.p, L.q));
+  L.x = off; off += al((size_t)L.q * L.qp * 16);
+  L.v = off; off += al((size_t)L.qp * L.qp * 16);
+  L.ub = off; off += al((size_t)L.p * L.q * 16);
+  L.gp = off; off += al((size_t)npairs * L.nseg * JP * JP * 16);
+  L.jg = off; off += al((size_t)npairs * JP * JP * 16);
+  L.rflag = off; off += al((size_t)npairs * 4);
+  L.nrm = off; off += al((size_t)L.qp * 8);
+  L.perm = off; off += al((size_t)L.qp * 8);
+  L.cnt = off; off += al(16);
+  L.total = off;
+  return L;
 }
+
+size_t svd_ws_bytes(int64_t m, int64_t n) { return svd_layout(m, n).total; }
 
 int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t ldu, double* s,
             void* vh, int64_t ldvh, int32_t* info, void* ws, size_t ws_bytes, hipStream_t st) {
   if (m <= 0 || n <= 0 || lda < n) return PTN_ERR_ARG;
   const int64_t k = m < n ? m : n;
   if (ldu < k || ldvh < n) return PTN_ERR_ARG;
-  if (ws_bytes < svd_ws_bytes(m, n)) return PTN_ERR_WORKSPACE;
+  const SvdLayout L = svd_layout(m, n);
+  if (ws_bytes < L.total) return PTN_ERR_WORKSPACE;
+  const int npairs = L.nblk / 2;
+  const int64_t nvseg = (L.qp + SEG - 1) / SEG;
+  const int64_t aseg = L.nseg > nvseg ? L.nseg : nvseg;
+  if (npairs > 65535 || aseg > 65535) return PTN_ERR_ARG;
   const int adjoint = m < n;
-  const int64_t pr = adjoint ? n : m;    // rows of X
-  const int64_t qn = k;                  // columns of X
-  char* p = (char*)ws;
-  double2* X = (double2*)p; p += al((size_t)pr * qn * 16);
-  double2* V = (double2*)p; p += al((size_t)qn * qn * 16);
-  double* nrm = (double*)p; p += al((size_t)qn * 8);
-  double* sbuf = (double*)p; p += al((size_t)qn * 8);
-  int64_t* perm = (int64_t*)p; p += al((size_t)qn * 8);
-  int* counters = (int*)p;   // [0] rotations in the current sweep, [1] non-finite flag
+  char* base = (char*)ws;
+  double2* Q = (double2*)(base + L.qm);
+  double2* R = (double2*)(base + L.rm);
+  double2* X = (double2*)(base + L.x);
+  double2* V = (double2*)(base + L.v);
+  double2* UB = (double2*)(base + L.ub);
+  double2* Gp = (double2*)(base + L.gp);
+  double2* Jg = (double2*)(base + L.jg);
+  int* rflag = (int*)(base + L.rflag);
+  double* nrm = (double*)(base + L.nrm);
+  int64_t* perm = (int64_t*)(base + L.perm);
+  int* counters = (int*)(base + L.cnt);   // [0] rotations in the current sweep, [1] non-finite
+  int stt;
 
-  k_svd_load<<<(unsigned)((m * n + 255) / 256), 256, 0, st>>>(m, n, (const double2*)a, lda,
-                                                               adjoint, X);
-  k_eye<<<(unsigned)((qn * qn + 255) / 256), 256, 0, st>>>(qn, V);
+  // 1. B = Q R
+  const void* B = a;
+  int64_t ldb = lda;
+  if (adjoint) {
+    k_adjoint<<<(unsigned)((m * n + 255) / 256), 256, 0, st>>>(m, n, (const double2*)a, lda,
+                                                                (double2*)(base + L.bt));
+    PTN_LAUNCH_CHECK();
+    B = base + L.bt;
+    ldb = m;
+  }
+  if ((stt = qr_run(L.p, L.q, B, ldb, Q, L.q, R, L.q, base + L.qrws, al(qr_ws_bytes(L.p, L.q)), st)))
+    return stt;
+  // 2. one-sided Jacobi on X = R^H
+  k_load_rh<<<1024, 256, 0, st>>>(L.q, L.qp, R, X);
+  k_eye<<<1024, 256, 0, st>>>(L.qp, V);
   PTN_LAUNCH_CHECK();
   int* host_rot = pinned_counter();
   if (!host_rot) return PTN_ERR_HIP;
-  const double tol = DBL_EPSILON * sqrt((double)pr);
-  const int64_t qe = (qn % 2) ? qn + 1 : qn;
+  const double tol = DBL_EPSILON * sqrt((double)L.q);
+  // a sweep without rotations above this relative size is followed by one
+  // final sweep (quadratic convergence takes it to tol) instead of repeating
+  // until rounding noise stops triggering tiny rotations
+  const double big = 1e-7;
   const int max_sweeps = 60;
-  int sweeps = 0, converged = (qn < 2);
+  int sweeps = 0, converged = (L.q < 2), final_sweep = 0;
   k_set_ints<<<1, 64, 0, st>>>(counters, 2, 0);
   while (!converged && sweeps < max_sweeps) {
     k_set_ints<<<1, 64, 0, st>>>(counters, 1, 0);
-    for (int64_t r = 0; r < qe - 1; ++r)
-      k_jacobi_round<<<(unsigned)(qe / 2), 256, 0, st>>>(pr, qn, qe, r, X, V, tol, counters);
+    for (int r = 0; r < L.nblk - 1; ++r) {
+      k_jac_gram<<<dim3((unsigned)L.nseg, (unsigned)npairs), 256, 0, st>>>(L.q, L.nblk, r, X, Gp,
+                                                                           (int)L.nseg);
+      k_jac_solve<<<(unsigned)npairs, 256, 0, st>>>(L.nblk, Gp, (int)L.nseg, tol, big, Jg, rflag,
+                                                     counters);
+      k_jac_apply<<<dim3((unsigned)aseg, (unsigned)npairs), SEG, 0, st>>>(
+          L.q, L.qp, L.nblk, r, X, V, Jg, rflag);
+    }
     PTN_LAUNCH_CHECK();
     PTN_HIP_CHECK(hipMemcpyAsync(host_rot, counters, sizeof(int), hipMemcpyDeviceToHost, st));
     PTN_HIP_CHECK(hipStreamSynchronize(st));
     ++sweeps;
-    converged = (*host_rot == 0);
+    if (final_sweep) converged = 1;
+    else if (*host_rot == 0) final_sweep = 1;
   }
-  k_colnorms<<<(unsigned)qn, 256, 0, st>>>(pr, X, nrm);
-  k_rank<<<1, 256, 0, st>>>(qn, nrm, perm, s, counters + 1);
+  // 3. U_B = Q V (V column-major with leading dimension qp)
+  ZGemm g = {L.p, L.q, L.q, Q, L.q, 1, 0, V, 1, L.qp, 0, UB, L.q, 1, 0, 1, 0, 0, 0};
+  if ((stt = zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, st))) return stt;
+  k_colnorms<<<(unsigned)L.q, 256, 0, st>>>(L.q, X, nrm);
+  k_rank<<<1, 256, 0, st>>>(L.q, nrm, perm, s, counters + 1);
   const int64_t tot = (m * k > k * n) ? m * k : k * n;
-  k_svd_out<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(m, n, k, adjoint, X, V, perm, s,
+  k_svd_out<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(m, n, k, adjoint, L.q, X, UB, perm, s,
                                                             (double2*)u, ldu, (double2*)vh, ldvh);
-  PTN_LAUNCH_CHECK();
   if (info) k_svd_info<<<1, 64, 0, st>>>(info, sweeps, converged ? 0 : 1, counters);
   PTN_LAUNCH_CHECK();
-  (void)sbuf;
   return PTN_OK;
 }
 

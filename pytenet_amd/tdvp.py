@@ -21,7 +21,7 @@ from .block_sparse_util import qnumber_flatten, block_sparse_qr
 from . import backend
 from .sweep import SweepContext, absorb_left, absorb_right, flip
 
-__all__ = ["tdvp_singlesite", "tdvp_twosite"]
+__all__ = ["tdvp_singlesite", "tdvp_twosite", "TwoSiteTDVP"]
 
 
 def tdvp_singlesite(hamiltonian: MPO, psi: MPS, dt, numsteps: int, numiter_lanczos: int = 25,
@@ -68,46 +68,70 @@ def tdvp_singlesite(hamiltonian: MPO, psi: MPS, dt, numsteps: int, numiter_lancz
     return nrm
 
 
+class TwoSiteTDVP:
+    """State of a two-site TDVP run on the GPU (tdvp.py:121-220), split into
+    setup() (right-orthonormalisation, environments, merged MPO pairs) and
+    sweep() (one symmetric time step), so that sweeps can be timed on their own."""
+
+    def __init__(self, hamiltonian: MPO, psi: MPS, dt, numiter_lanczos: int = 25, tol_split=0,
+                 precision: str = "fp64", max_bond=None):
+        self.nsites = hamiltonian.nsites
+        assert self.nsites == psi.nsites
+        assert self.nsites >= 2
+        self.H, self.psi, self.dt = hamiltonian, psi, dt
+        self.tol, self.max_bond = tol_split, max_bond
+        self.ctx = SweepContext(hamiltonian, psi, numiter_lanczos, precision)
+
+    def setup(self):
+        psi, H = self.psi, self.H
+        self.nrm = psi.orthonormalize(mode="right")
+        self.lblocks, self.rblocks = self.ctx.init_blocks(psi)
+        self.h2 = [backend.as_dev(mpo_merge_tensor_pair(H.a[i], H.a[i + 1]))
+                   for i in range(self.nsites - 1)]
+        return self.nrm
+
+    def sweep(self):
+        psi, ctx, W, h2 = self.psi, self.ctx, self.ctx.W, self.h2
+        lblocks, rblocks = self.lblocks, self.rblocks
+        dt, qs, n = self.dt, psi.qsite, self.nsites
+        for i in range(n - 2):
+            a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
+            a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -0.5 * dt)
+            psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
+                a2, qs, qs, (psi.qbonds[i], psi.qbonds[i + 2]), "right", tol=self.tol,
+                max_bond=self.max_bond)
+            lblocks[i + 1] = ctx.env_left(psi.a[i], W[i], lblocks[i])
+            psi.a[i + 1] = ctx.site_expm(psi.a[i + 1], lblocks[i + 1], W[i + 1], rblocks[i + 1],
+                                         0.5 * dt)
+        i = n - 2
+        a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
+        a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -dt)
+        psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
+            a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=self.tol,
+            max_bond=self.max_bond)
+        rblocks[i] = ctx.env_right(psi.a[i + 1], W[i + 1], rblocks[i + 1])
+        for i in reversed(range(n - 2)):
+            psi.a[i + 1] = ctx.site_expm(psi.a[i + 1], lblocks[i + 1], W[i + 1], rblocks[i + 1],
+                                         0.5 * dt)
+            a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
+            a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -0.5 * dt)
+            psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
+                a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=self.tol,
+                max_bond=self.max_bond)
+            rblocks[i] = ctx.env_right(psi.a[i + 1], W[i + 1], rblocks[i + 1])
+        ctx.flush()
+
+    def finish(self):
+        self.ctx.finish(self.psi)
+
+
 def tdvp_twosite(hamiltonian: MPO, psi: MPS, dt, numsteps: int, numiter_lanczos: int = 25,
                  tol_split=0, precision: str = "fp64", max_bond=None):
     """Symmetric two-site TDVP (tdvp.py:121-220). `psi` is overwritten with the
     evolved state; returns the norm of the initial psi."""
-    nsites = hamiltonian.nsites
-    assert nsites == psi.nsites
-    assert nsites >= 2
-    ctx = SweepContext(hamiltonian, psi, numiter_lanczos, precision)
-    W = ctx.W
-    nrm = psi.orthonormalize(mode="right")
-    lblocks, rblocks = ctx.init_blocks(psi)
-    h2 = [backend.as_dev(mpo_merge_tensor_pair(hamiltonian.a[i], hamiltonian.a[i + 1]))
-          for i in range(nsites - 1)]
-    qs = psi.qsite
+    run = TwoSiteTDVP(hamiltonian, psi, dt, numiter_lanczos, tol_split, precision, max_bond)
+    nrm = run.setup()
     for _ in range(numsteps):
-        for i in range(nsites - 2):
-            a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
-            a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -0.5 * dt)
-            psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
-                a2, qs, qs, (psi.qbonds[i], psi.qbonds[i + 2]), "right", tol=tol_split,
-                max_bond=max_bond)
-            lblocks[i + 1] = ctx.env_left(psi.a[i], W[i], lblocks[i])
-            psi.a[i + 1] = ctx.site_expm(psi.a[i + 1], lblocks[i + 1], W[i + 1], rblocks[i + 1],
-                                         0.5 * dt)
-        i = nsites - 2
-        a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
-        a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -dt)
-        psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
-            a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=tol_split,
-            max_bond=max_bond)
-        rblocks[i] = ctx.env_right(psi.a[i + 1], W[i + 1], rblocks[i + 1])
-        for i in reversed(range(nsites - 2)):
-            psi.a[i + 1] = ctx.site_expm(psi.a[i + 1], lblocks[i + 1], W[i + 1], rblocks[i + 1],
-                                         0.5 * dt)
-            a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
-            a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -0.5 * dt)
-            psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
-                a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=tol_split,
-                max_bond=max_bond)
-            rblocks[i] = ctx.env_right(psi.a[i + 1], W[i + 1], rblocks[i + 1])
-        ctx.flush()
-    ctx.finish(psi)
+        run.sweep()
+    run.finish()
     return nrm
