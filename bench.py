@@ -191,7 +191,12 @@ def main():
     torch.cuda.set_device(0)
     prewarm()
     if args.roofline_only:
-        res = {}
+        # a plain device copy of known size: calibrates the HBM byte counters of a profile run
+        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        res = {"copy_calibration_bytes": 2 * src.numel() * 8}
         for prec in ("fp64", args.mixed):
             ms, tf = heff_roofline(args.D, prec, reps=20)
             res[prec] = {"avg_ms": round(ms, 3), "achieved_tflops": round(tf, 2),
