@@ -58,7 +58,7 @@ def prewarm():
     """Load every kernel once on a tiny chain (not a timed or warmup step)."""
     import pytenet_amd as ptn
     H, psi = build_state(6, 8, 0)
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "bf16x3"):
         p = copy.deepcopy(psi)
         ptn.tdvp_twosite(H, p, -0.05j, 1, numiter_lanczos=4, tol_split=1e-10, precision=prec,
                          max_bond=8)
@@ -166,7 +166,44 @@ def cpu_baseline(D, L, numiter, budget_s=30.0):
                       f"({sample:.1f} s), times {2 * (L - 2) + 1} updates per sweep"}
 
 
+def profile_summary(dirs):
+    """Text tables from rocprofv3 csv output (no GPU needed): per kernel the
+    calls and durations of *_kernel_stats.csv, and the mean per dispatch of
+    every counter in *_counter_collection.csv (e.g. FETCH_SIZE / WRITE_SIZE
+    in KiB)."""
+    import csv
+    import glob
+    from collections import defaultdict
+
+    def short(name, width=110):
+        name = name.replace("void ", "").replace("HIP_vector_type<double, 2u>", "c128")
+        name = name.replace("HIP_vector_type<float, 2u>", "c64")
+        return name if len(name) <= width else name[:width] + "..."
+    for d in dirs:
+        for f in sorted(glob.glob(os.path.join(d, "**", "*_kernel_stats.csv"), recursive=True)):
+            rows = list(csv.DictReader(open(f)))
+            tot = sum(float(r["TotalDurationNs"]) for r in rows)
+            print(f"# {os.path.basename(f)}: total kernel time {tot / 1e6:.2f} ms")
+            print(f"{'calls':>8} {'total ms':>10} {'avg us':>10} {'%':>6}  kernel")
+            for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))[:30]:
+                print(f"{int(r['Calls']):8d} {float(r['TotalDurationNs']) / 1e6:10.3f} "
+                      f"{float(r['AverageNs']) / 1e3:10.2f} {float(r['Percentage']):6.2f}  "
+                      f"{short(r['Name'])}")
+        for f in sorted(glob.glob(os.path.join(d, "**", "*_counter_collection.csv"), recursive=True)):
+            agg = defaultdict(lambda: defaultdict(list))
+            for r in csv.DictReader(open(f)):
+                agg[r["Kernel_Name"]][r["Counter_Name"]].append(float(r["Counter_Value"]))
+            print(f"# {os.path.basename(f)}: mean counter value per dispatch")
+            print(f"{'calls':>6} {'counter':>12} {'mean':>14}  kernel")
+            for k, cs in agg.items():
+                for c, v in cs.items():
+                    print(f"{len(v):6d} {c:>12} {sum(v) / len(v):14.1f}  {short(k)}")
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--profile-summary":
+        profile_summary(sys.argv[2:])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
@@ -189,7 +226,6 @@ def main():
     import pytenet_amd as ptn
 
     torch.cuda.set_device(0)
-    prewarm()
     if args.roofline_only:
         # a plain device copy of known size: calibrates the HBM byte counters of a profile run
         src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
@@ -197,12 +233,13 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         res = {"copy_calibration_bytes": 2 * src.numel() * 8}
-        for prec in ("fp64", args.mixed):
+        for prec in ("fp64", "fp32", "bf16x3"):
             ms, tf = heff_roofline(args.D, prec, reps=20)
             res[prec] = {"avg_ms": round(ms, 3), "achieved_tflops": round(tf, 2),
                          "peak": PEAK_TFLOPS[prec], "frac": round(tf / PEAK_TFLOPS[prec], 4)}
         print(json.dumps({"roofline_only": res, "D": args.D}), flush=True)
         return
+    prewarm()
     H, psi0 = build_state(args.L, args.D, args.seed)
     print(f"[bench] state ready: L={args.L}, bonds max {max(psi0.bond_dims)}", file=sys.stderr,
           flush=True)

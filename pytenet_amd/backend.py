@@ -35,10 +35,7 @@ def _check(status, what):
 def precision_code(precision):
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}")
-    code = PRECISIONS[precision]
-    if code == _lib.PTN_BF16X3:
-        raise NotImplementedError("the split-bf16 mode is not built in this version")
-    return code
+    return PRECISIONS[precision]
 
 
 def device():

@@ -66,6 +66,10 @@ __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 
 using namespace ptn;
 
+static inline bool precision_ok(int p) {
+  return p == PTN_FP64 || p == PTN_FP32 || p == PTN_BF16X3;
+}
+
 extern "C" {
 
 int ptn_version(void) { return 100; }
@@ -92,7 +96,7 @@ size_t ptn_heff_workspace(int precision, int64_t Dl, int64_t d, int64_t Dr, int6
 int ptn_heff_apply(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr,
                    const void* l, const void* w, const void* r, const void* a, void* out, void* ws,
                    size_t ws_bytes, void* stream) {
-  if (precision != PTN_FP64 && precision != PTN_FP32) return PTN_ERR_UNSUPPORTED;
+  if (!precision_ok(precision)) return PTN_ERR_UNSUPPORTED;
   Dims D = {Dl, d, Dr, Ml, Mr};
   return heff_apply(precision, D, l, PTN_T_C128, w, r, PTN_T_C128, a, PTN_T_C128, out,
                     PTN_T_C128, ws, ws_bytes, (hipStream_t)stream);
@@ -104,7 +108,7 @@ size_t ptn_bond_workspace(int precision, int64_t Dl, int64_t Dr, int64_t M) {
 
 int ptn_bond_apply(int precision, int64_t Dl, int64_t Dr, int64_t M, const void* l, const void* r,
                    const void* c, void* out, void* ws, size_t ws_bytes, void* stream) {
-  if (precision != PTN_FP64 && precision != PTN_FP32) return PTN_ERR_UNSUPPORTED;
+  if (!precision_ok(precision)) return PTN_ERR_UNSUPPORTED;
   return bond_apply(precision, Dl, Dr, M, l, PTN_T_C128, r, PTN_T_C128, c, PTN_T_C128, out,
                     PTN_T_C128, ws, ws_bytes, (hipStream_t)stream);
 }
@@ -118,7 +122,7 @@ size_t ptn_env_workspace(int precision, int64_t Dl, int64_t d, int64_t Dr, int64
 int ptn_env_left(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr,
                  const void* a, const void* b, const void* w, const void* l, void* out, void* ws,
                  size_t ws_bytes, void* stream) {
-  if (precision != PTN_FP64 && precision != PTN_FP32) return PTN_ERR_UNSUPPORTED;
+  if (!precision_ok(precision)) return PTN_ERR_UNSUPPORTED;
   Dims D = {Dl, d, Dr, Ml, Mr};
   return env_left(precision, D, a, b, w, l, PTN_T_C128, out, ws, ws_bytes, (hipStream_t)stream);
 }
@@ -126,7 +130,7 @@ int ptn_env_left(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, i
 int ptn_env_right(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr,
                   const void* a, const void* b, const void* w, const void* r, void* out, void* ws,
                   size_t ws_bytes, void* stream) {
-  if (precision != PTN_FP64 && precision != PTN_FP32) return PTN_ERR_UNSUPPORTED;
+  if (!precision_ok(precision)) return PTN_ERR_UNSUPPORTED;
   Dims D = {Dl, d, Dr, Ml, Mr};
   return env_right(precision, D, a, b, w, r, PTN_T_C128, out, ws, ws_bytes, (hipStream_t)stream);
 }

@@ -1,9 +1,11 @@
-// Complex matrix products, FP64 MFMA instantiations and the zgemm dispatcher.
+// Complex matrix products, FP64 MFMA instantiations and the zgemm dispatcher
+// (FP64, FP32 and split-bf16 arithmetic).
 #include "gemm_impl.h"
 
 namespace ptn {
 
 int zgemm_fp32(const ZGemm& z, int ta, int tb, int tc, hipStream_t s);
+int zgemm_bf16x3(const ZGemm& z, int ta, int tb, int tc, hipStream_t s);
 
 int zgemm(const ZGemm& z, int compute, int ta, int tb, int tc, void* ws, size_t ws_bytes,
           hipStream_t s) {
@@ -16,6 +18,7 @@ int zgemm(const ZGemm& z, int compute, int ta, int tb, int tc, void* ws, size_t 
     return PTN_ERR_UNSUPPORTED;
   }
   if (compute == PTN_FP32) return zgemm_fp32(z, ta, tb, tc, s);
+  if (compute == PTN_BF16X3) return zgemm_bf16x3(z, ta, tb, tc, s);
   return PTN_ERR_UNSUPPORTED;
 }
 

@@ -373,7 +373,8 @@ int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, do
                void* ws, size_t ws_bytes, hipStream_t s) {
   if (!op || numiter < 1 || numiter > LZ_MAX) return PTN_ERR_ARG;
   if (op->kind != PTN_OP_HEFF && op->kind != PTN_OP_BOND) return PTN_ERR_ARG;
-  if (op->precision != PTN_FP64 && op->precision != PTN_FP32) return PTN_ERR_UNSUPPORTED;
+  if (op->precision != PTN_FP64 && op->precision != PTN_FP32 && op->precision != PTN_BF16X3)
+    return PTN_ERR_UNSUPPORTED;
   if (op->Dl <= 0 || op->Dr <= 0 || op->Ml <= 0 || op->Mr <= 0) return PTN_ERR_ARG;
   if (op->kind == PTN_OP_HEFF && (op->d <= 0 || !op->w)) return PTN_ERR_ARG;
   if (op->kind == PTN_OP_BOND && op->Ml != op->Mr) return PTN_ERR_ARG;

@@ -14,7 +14,7 @@ namespace ptn {
 static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 static inline size_t csize(int prec) { return prec == PTN_FP64 ? 16 : 8; }
 static inline int ctype(int prec) { return prec == PTN_FP64 ? PTN_T_C128 : PTN_T_C64; }
-static inline int gemm_compute(int prec) { return prec == PTN_FP64 ? PTN_FP64 : PTN_FP32; }
+static inline int gemm_compute(int prec) { return prec; }
 
 // ---------------------------------------------------------------------------
 // W (Ml, d, d, Mr) viewed as the matrix Wmat[(m, sb), (s, m')]; the left

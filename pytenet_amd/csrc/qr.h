@@ -1,0 +1,34 @@
+// Blocked Householder QR (internal interface shared by the QR and SVD entry points).
+#pragma once
+#include "common.h"
+
+namespace ptn {
+
+constexpr int QR_NB = 32;       // panel width (columns per compact-WY block)
+constexpr int QR_SPLITK = 8;    // K chunks of the V^H C products
+
+// Factorisation B = Q R of an m x n complex128 matrix, held in device workspace:
+//   W     column-major m x n; its strict upper triangle is that of R
+//   Vf    column-major m x k Householder vectors v_j (v_j[j] = 1, zero above)
+//   tau   k scalar factors, H_j = I - tau_j v_j v_j^H
+//   rdiag k diagonal entries of R
+//   T     per panel the QR_NB x QR_NB upper-triangular factor of the compact
+//         WY form H_j0 ... H_j0+nb-1 = I - V T V^H (row-major, zero padded)
+// plus scratch for the blocked updates (Vf^H C partials, the reduced product).
+struct QrFact {
+  int64_t m, n, k, npan, ncmax;
+  double2 *W, *Vf, *tau, *rdiag, *T, *G, *P, *Z;
+};
+
+size_t qr_fact_bytes(int64_t m, int64_t n, int64_t ncmax);
+// carve the buffers of a factorisation out of `ws` (qr_fact_bytes(m, n, ncmax) bytes)
+QrFact qr_fact_layout(int64_t m, int64_t n, int64_t ncmax, void* ws);
+// factor the row-major m x n matrix `a` (leading dimension lda)
+int qr_factor(QrFact& f, const double2* a, int64_t lda, hipStream_t s);
+// C <- Q C for the column-major m x nc matrix C (leading dimension ldc, nc <= ncmax),
+// Q = H_0 H_1 ... H_{k-1}. With `identity_tail` the caller guarantees that C
+// starts as [I_k; 0] (nc == k), so panel j0 only needs columns j0 .. k-1.
+int qr_apply_q(const QrFact& f, double2* C, int64_t ldc, int64_t nc, bool identity_tail,
+               hipStream_t s);
+
+}  // namespace ptn

@@ -6,11 +6,20 @@
 //     else beta = -sign(Re alpha) |(alpha, x)|, tau = (beta - alpha) / beta,
 //          v = [1, x / (alpha - beta)], R[j, j] = beta (real)
 //   A <- H^H A with H = I - tau v v^H;   Q = H_0 H_1 ... H_{k-1} [I_k; 0].
-// The matrix is factorised in a column-major copy so that every column
-// operation is a contiguous vector operation. Column j costs two launches
-// (reflector, trailing update); forming Q one launch per reflector.
+//
+// Blocked right-looking algorithm on a column-major copy (panels of QR_NB
+// columns):
+//   panel:    for each column j one launch (k_qr_col) computes reflector j in
+//             every workgroup (the same deterministic reduction) and applies
+//             it to one remaining column of the panel per workgroup;
+//   WY form:  T from V^H V (k_qr_vtv, k_qr_tmat), H_j0 ... = I - V T V^H;
+//   trailing: A <- A - V (T^H (V^H A)): V^H A as a split-K complex product on
+//             MFMA, the sum of the K chunks and the small T^H product in
+//             k_qr_reduce, and one accumulating MFMA product.
+// Q (or Q times a given matrix) is formed panel by panel in reverse order
+// with the same three steps and T in place of T^H.
 #include <float.h>
-#include "common.h"
+#include "qr.h"
 
 namespace ptn {
 
@@ -37,20 +46,32 @@ __global__ void k_to_colmajor(int64_t m, int64_t n, const double2* __restrict__ 
   w[j * m + i] = a[i * lda + j];
 }
 
-// reflector for column j (one workgroup); tau[j] = (re, im)
-__global__ __launch_bounds__(256) void k_qr_reflector(int64_t m, int64_t j, double2* __restrict__ W,
-                                                      double2* __restrict__ tau) {
-  double2* col = W + j * m;
-  // read alpha before the barrier inside wg_sum: thread 0 overwrites it below
-  const double2 alpha = col[j];
+// reflector j, applied to column c = j + blockIdx.x (< pe) of the panel.
+// Workgroup 0 stores v_j (with the unit and the zeros above), tau_j and R[j, j];
+// column j of W itself is left as it is (rows > j are not read again).
+__global__ __launch_bounds__(256) void k_qr_col(int64_t m, int64_t j, double2* __restrict__ W,
+                                                double2* __restrict__ Vf,
+                                                double2* __restrict__ tau,
+                                                double2* __restrict__ rdiag) {
+  const double2* xj = W + j * m;
+  const double2 alpha = xj[j];
   double s = 0;
   for (int64_t i = j + 1 + threadIdx.x; i < m; i += 256) {
-    double2 x = col[i];
+    const double2 x = xj[i];
     s += x.x * x.x + x.y * x.y;
   }
   const double xnorm = sqrt(wg_sum(s));
+  double2* vj = Vf + j * m;
   if (xnorm == 0.0 && alpha.y == 0.0) {
-    if (threadIdx.x == 0) tau[j] = make_double2(0.0, 0.0);
+    // H_j = I: the other columns are unchanged
+    if (blockIdx.x == 0) {
+      for (int64_t i = threadIdx.x; i < m; i += 256)
+        vj[i] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+      if (threadIdx.x == 0) {
+        tau[j] = make_double2(0.0, 0.0);
+        rdiag[j] = alpha;
+      }
+    }
     return;
   }
   // |(alpha_r, alpha_i, xnorm)| with scaling against overflow
@@ -59,52 +80,244 @@ __global__ __launch_bounds__(256) void k_qr_reflector(int64_t m, int64_t j, doub
   const double nrm = mx * sqrt((ar / mx) * (ar / mx) + (ai / mx) * (ai / mx) +
                                (xnorm / mx) * (xnorm / mx));
   const double beta = -copysign(nrm, alpha.x);
-  // scale factor 1 / (alpha - beta)
+  // v_i = x_i / (alpha - beta) for i > j
   const double dr = alpha.x - beta, di = alpha.y;
   const double den = dr * dr + di * di;
   const double sr = dr / den, si = -di / den;
-  for (int64_t i = j + 1 + threadIdx.x; i < m; i += 256) {
-    double2 x = col[i];
-    col[i] = make_double2(x.x * sr - x.y * si, x.x * si + x.y * sr);
+  const double2 t = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+  if (blockIdx.x == 0) {
+    for (int64_t i = threadIdx.x; i < m; i += 256) {
+      double2 v;
+      if (i < j) v = make_double2(0.0, 0.0);
+      else if (i == j) v = make_double2(1.0, 0.0);
+      else { const double2 x = xj[i]; v = make_double2(x.x * sr - x.y * si, x.x * si + x.y * sr); }
+      vj[i] = v;
+    }
+    if (threadIdx.x == 0) {
+      tau[j] = t;
+      rdiag[j] = make_double2(beta, 0.0);
+    }
+    return;
   }
-  if (threadIdx.x == 0) {
-    tau[j] = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-    col[j] = make_double2(beta, 0.0);
-  }
-}
-
-// apply H_j^H = I - conj(tau_j) v v^H to columns c = j+1 .. n-1 (one workgroup per column)
-__global__ __launch_bounds__(256) void k_qr_apply(int64_t m, int64_t n, int64_t j,
-                                                  double2* __restrict__ W,
-                                                  const double2* __restrict__ tau) {
-  const int64_t c = j + 1 + blockIdx.x;
-  if (c >= n) return;
-  const double2 t = tau[j];
-  if (t.x == 0.0 && t.y == 0.0) return;
-  const double2* v = W + j * m;
-  double2* x = W + c * m;
-  // w = v^H x with v_j = 1
+  // column c: w = v^H x, x <- x - conj(tau) w v (rows >= j)
+  double2* xc = W + (j + blockIdx.x) * m;
   double wr = 0, wi = 0;
   for (int64_t i = j + threadIdx.x; i < m; i += 256) {
-    double2 vi = (i == j) ? make_double2(1.0, 0.0) : v[i];
-    double2 xi = x[i];
-    wr += vi.x * xi.x + vi.y * xi.y;
-    wi += vi.x * xi.y - vi.y * xi.x;
+    double2 v;
+    if (i == j) v = make_double2(1.0, 0.0);
+    else { const double2 x = xj[i]; v = make_double2(x.x * sr - x.y * si, x.x * si + x.y * sr); }
+    const double2 xi = xc[i];
+    wr += v.x * xi.x + v.y * xi.y;
+    wi += v.x * xi.y - v.y * xi.x;
   }
   wr = wg_sum(wr);
   wi = wg_sum(wi);
-  // f = conj(tau) * w
-  const double fr = t.x * wr + t.y * wi, fi = t.x * wi - t.y * wr;
+  const double fr = t.x * wr + t.y * wi, fi = t.x * wi - t.y * wr;   // conj(tau) w
   for (int64_t i = j + threadIdx.x; i < m; i += 256) {
-    double2 vi = (i == j) ? make_double2(1.0, 0.0) : v[i];
-    double2 xi = x[i];
-    xi.x -= vi.x * fr - vi.y * fi;
-    xi.y -= vi.x * fi + vi.y * fr;
-    x[i] = xi;
+    double2 v;
+    if (i == j) v = make_double2(1.0, 0.0);
+    else { const double2 x = xj[i]; v = make_double2(x.x * sr - x.y * si, x.x * si + x.y * sr); }
+    double2 xi = xc[i];
+    xi.x -= v.x * fr - v.y * fi;
+    xi.y -= v.x * fi + v.y * fr;
+    xc[i] = xi;
   }
 }
 
-// Q (column-major m x k in Qc) := identity columns
+// G[a][b] = v_a^H v_b for the panel's reflectors (rows j0 .. m-1), one workgroup per entry
+__global__ __launch_bounds__(256) void k_qr_vtv(int64_t m, int64_t j0, int nbp,
+                                                const double2* __restrict__ Vf,
+                                                double2* __restrict__ G) {
+  const int a = blockIdx.x / nbp, b = blockIdx.x % nbp;
+  if (b <= a) return;   // only the strict upper triangle is used
+  const double2* va = Vf + (j0 + a) * m;
+  const double2* vb = Vf + (j0 + b) * m;
+  double sr = 0, si = 0;
+  for (int64_t i = j0 + threadIdx.x; i < m; i += 256) {
+    const double2 x = va[i], y = vb[i];
+    sr += x.x * y.x + x.y * y.y;
+    si += x.x * y.y - x.y * y.x;
+  }
+  sr = wg_sum(sr);
+  si = wg_sum(si);
+  if (threadIdx.x == 0) G[a * QR_NB + b] = make_double2(sr, si);
+}
+
+// compact WY factor (forward, columnwise): T[i][i] = tau_i,
+// T[0:i, i] = -tau_i T[0:i, 0:i] (V[:, 0:i]^H v_i); zero padded to QR_NB x QR_NB
+__global__ __launch_bounds__(64) void k_qr_tmat(int nbp, const double2* __restrict__ G,
+                                                const double2* __restrict__ tau,
+                                                double2* __restrict__ T) {
+  __shared__ double2 Ts[QR_NB][QR_NB + 1];
+  const int r = threadIdx.x;
+  if (r < QR_NB)
+    for (int c = 0; c < QR_NB; ++c) Ts[r][c] = make_double2(0.0, 0.0);
+  __syncthreads();
+  for (int i = 0; i < nbp; ++i) {
+    const double2 ti = tau[i];
+    double2 val = make_double2(0.0, 0.0);
+    if (r < i) {
+      double ar = 0, ai = 0;
+      for (int s = r; s < i; ++s) {
+        const double2 a = Ts[r][s], g = G[s * QR_NB + i];
+        ar += a.x * g.x - a.y * g.y;
+        ai += a.x * g.y + a.y * g.x;
+      }
+      val = make_double2(-(ti.x * ar - ti.y * ai), -(ti.x * ai + ti.y * ar));
+    }
+    __syncthreads();
+    if (r < i) Ts[r][i] = val;
+    if (r == i) Ts[i][i] = ti;
+    __syncthreads();
+  }
+  if (r < QR_NB)
+    for (int c = 0; c < QR_NB; ++c) T[r * QR_NB + c] = Ts[r][c];
+}
+
+// Y = sum of the nparts partial products P[s] (QR_NB x nc, rows < nbp used);
+// Z = -T^H Y (adjoint = 1) or Z = -T Y (adjoint = 0); one thread per column
+__global__ __launch_bounds__(64) void k_qr_reduce(int nbp, int64_t nc, int nparts,
+                                                  const double2* __restrict__ P,
+                                                  const double2* __restrict__ T, int adjoint,
+                                                  double2* __restrict__ Z) {
+  __shared__ double2 Ts[QR_NB][QR_NB + 1];
+  for (int e = threadIdx.x; e < QR_NB * QR_NB; e += 64) Ts[e / QR_NB][e % QR_NB] = T[e];
+  __syncthreads();
+  const int64_t c = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (c >= nc) return;
+  double2 y[QR_NB];
+#pragma unroll
+  for (int r = 0; r < QR_NB; ++r) {
+    double sr = 0, si = 0;
+    if (r < nbp)
+      for (int s = 0; s < nparts; ++s) {
+        const double2 v = P[((int64_t)s * QR_NB + r) * nc + c];
+        sr += v.x;
+        si += v.y;
+      }
+    y[r] = make_double2(sr, si);
+  }
+#pragma unroll
+  for (int r = 0; r < QR_NB; ++r) {
+    double zr = 0, zi = 0;
+    if (adjoint) {
+      // (T^H y)_r = sum_{r' <= r} conj(T[r'][r]) y_r'
+#pragma unroll
+      for (int q = 0; q <= r; ++q) {
+        const double2 a = Ts[q][r];
+        zr += a.x * y[q].x + a.y * y[q].y;
+        zi += a.x * y[q].y - a.y * y[q].x;
+      }
+    } else {
+      // (T y)_r = sum_{r' >= r} T[r][r'] y_r'
+#pragma unroll
+      for (int q = r; q < QR_NB; ++q) {
+        const double2 a = Ts[r][q];
+        zr += a.x * y[q].x - a.y * y[q].y;
+        zi += a.x * y[q].y + a.y * y[q].x;
+      }
+    }
+    if (r < nbp) Z[(int64_t)r * nc + c] = make_double2(-zr, -zi);
+  }
+}
+
+static inline size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t qr_fact_bytes(int64_t m, int64_t n, int64_t ncmax) {
+  const int64_t k = m < n ? m : n;
+  const int64_t npan = (k + QR_NB - 1) / QR_NB;
+  const int64_t nc = ncmax > n ? ncmax : n;
+  return al((size_t)m * n * 16) + al((size_t)m * k * 16) + 2 * al((size_t)(k + 1) * 16) +
+         al((size_t)(npan + 1) * QR_NB * QR_NB * 16) + al((size_t)QR_NB * QR_NB * 16) +
+         al((size_t)(QR_SPLITK + 1) * QR_NB * nc * 16) + al((size_t)QR_NB * nc * 16);
+}
+
+QrFact qr_fact_layout(int64_t m, int64_t n, int64_t ncmax, void* ws) {
+  QrFact f;
+  f.m = m; f.n = n; f.k = m < n ? m : n;
+  f.npan = (f.k + QR_NB - 1) / QR_NB;
+  f.ncmax = ncmax > n ? ncmax : n;
+  char* p = (char*)ws;
+  f.W = (double2*)p; p += al((size_t)m * n * 16);
+  f.Vf = (double2*)p; p += al((size_t)m * f.k * 16);
+  f.tau = (double2*)p; p += al((size_t)(f.k + 1) * 16);
+  f.rdiag = (double2*)p; p += al((size_t)(f.k + 1) * 16);
+  f.T = (double2*)p; p += al((size_t)(f.npan + 1) * QR_NB * QR_NB * 16);
+  f.G = (double2*)p; p += al((size_t)QR_NB * QR_NB * 16);
+  f.P = (double2*)p; p += al((size_t)(QR_SPLITK + 1) * QR_NB * f.ncmax * 16);
+  f.Z = (double2*)p;
+  return f;
+}
+
+// C[j0:m, c0:c0+nc] += V_panel Z with Z = -T^(H) V_panel^H C[j0:m, c0:c0+nc]
+// (C column-major with leading dimension ldc; panel columns j0 .. j0+nbp-1)
+static int apply_block(const QrFact& f, int64_t j0, int nbp, const double2* Tp, int adjoint,
+                       double2* C, int64_t ldc, int64_t c0, int64_t nc, hipStream_t s) {
+  const int64_t m = f.m, mr = m - j0;
+  if (nc <= 0 || mr <= 0) return PTN_OK;
+  const double2* V = f.Vf + j0 * m + j0;
+  double2* Cs = C + c0 * ldc + j0;
+  int st;
+  // partials P[s] = V^H C over K chunks of `chunk` rows
+  int64_t chunk = (mr + QR_SPLITK - 1) / QR_SPLITK;
+  chunk = ((chunk + 7) / 8) * 8;
+  const int64_t nfull = mr / chunk, rem = mr - nfull * chunk;
+  if (nfull > 0) {
+    ZGemm g = {nbp, nc, chunk, V, m, 1, 1, Cs, 1, ldc, 0, f.P, nc, 1, 0,
+               nfull, chunk, chunk, (int64_t)QR_NB * nc};
+    if ((st = zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, s))) return st;
+  }
+  if (rem > 0) {
+    ZGemm g = {nbp, nc, rem, V + nfull * chunk, m, 1, 1, Cs + nfull * chunk, 1, ldc, 0,
+               f.P + nfull * QR_NB * nc, nc, 1, 0, 1, 0, 0, 0};
+    if ((st = zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, s))) return st;
+  }
+  const int nparts = (int)(nfull + (rem > 0 ? 1 : 0));
+  k_qr_reduce<<<(unsigned)((nc + 63) / 64), 64, 0, s>>>(nbp, nc, nparts, f.P, Tp, adjoint, f.Z);
+  PTN_LAUNCH_CHECK();
+  // C += V Z
+  ZGemm g = {mr, nc, nbp, V, 1, m, 0, f.Z, nc, 1, 0, Cs, 1, ldc, 1, 1, 0, 0, 0};
+  return zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, s);
+}
+
+int qr_factor(QrFact& f, const double2* a, int64_t lda, hipStream_t s) {
+  const int64_t m = f.m, n = f.n, k = f.k;
+  if (m == 0 || n == 0) return PTN_OK;
+  k_to_colmajor<<<(unsigned)((m * n + 255) / 256), 256, 0, s>>>(m, n, a, lda, f.W);
+  PTN_LAUNCH_CHECK();
+  for (int64_t p = 0; p < f.npan; ++p) {
+    const int64_t j0 = p * QR_NB;
+    const int64_t pe = (j0 + QR_NB < k) ? j0 + QR_NB : k;
+    const int nbp = (int)(pe - j0);
+    for (int64_t j = j0; j < pe; ++j)
+      k_qr_col<<<(unsigned)(pe - j), 256, 0, s>>>(m, j, f.W, f.Vf, f.tau, f.rdiag);
+    PTN_LAUNCH_CHECK();
+    double2* Tp = f.T + p * QR_NB * QR_NB;
+    k_qr_vtv<<<(unsigned)(nbp * nbp), 256, 0, s>>>(m, j0, nbp, f.Vf, f.G);
+    k_qr_tmat<<<1, 64, 0, s>>>(nbp, f.G, f.tau + j0, Tp);
+    PTN_LAUNCH_CHECK();
+    int st;
+    if (pe < n && (st = apply_block(f, j0, nbp, Tp, 1, f.W, m, pe, n - pe, s))) return st;
+  }
+  return PTN_OK;
+}
+
+int qr_apply_q(const QrFact& f, double2* C, int64_t ldc, int64_t nc, bool identity_tail,
+               hipStream_t s) {
+  if (nc > f.ncmax) return PTN_ERR_ARG;
+  for (int64_t p = f.npan - 1; p >= 0; --p) {
+    const int64_t j0 = p * QR_NB;
+    const int64_t pe = (j0 + QR_NB < f.k) ? j0 + QR_NB : f.k;
+    const int64_t c0 = identity_tail ? j0 : 0;
+    int st;
+    if ((st = apply_block(f, j0, (int)(pe - j0), f.T + p * QR_NB * QR_NB, 0, C, ldc, c0,
+                          nc - c0, s)))
+      return st;
+  }
+  return PTN_OK;
+}
+
+// column-major m x k identity columns
 __global__ void k_q_init(int64_t m, int64_t k, double2* __restrict__ Qc) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= m * k) return;
@@ -112,40 +325,12 @@ __global__ void k_q_init(int64_t m, int64_t k, double2* __restrict__ Qc) {
   Qc[t] = make_double2(i == c ? 1.0 : 0.0, 0.0);
 }
 
-// Q[:, c] <- H_j Q[:, c] for c = j .. k-1, H_j = I - tau_j v v^H
-__global__ __launch_bounds__(256) void k_q_apply(int64_t m, int64_t k, int64_t j,
-                                                 const double2* __restrict__ W,
-                                                 const double2* __restrict__ tau,
-                                                 double2* __restrict__ Qc) {
-  const int64_t c = j + blockIdx.x;
-  if (c >= k) return;
-  const double2 t = tau[j];
-  if (t.x == 0.0 && t.y == 0.0) return;
-  const double2* v = W + j * m;
-  double2* x = Qc + c * m;
-  double wr = 0, wi = 0;
-  for (int64_t i = j + threadIdx.x; i < m; i += 256) {
-    double2 vi = (i == j) ? make_double2(1.0, 0.0) : v[i];
-    double2 xi = x[i];
-    wr += vi.x * xi.x + vi.y * xi.y;
-    wi += vi.x * xi.y - vi.y * xi.x;
-  }
-  wr = wg_sum(wr);
-  wi = wg_sum(wi);
-  const double fr = t.x * wr - t.y * wi, fi = t.x * wi + t.y * wr;  // tau * w
-  for (int64_t i = j + threadIdx.x; i < m; i += 256) {
-    double2 vi = (i == j) ? make_double2(1.0, 0.0) : v[i];
-    double2 xi = x[i];
-    xi.x -= vi.x * fr - vi.y * fi;
-    xi.y -= vi.x * fi + vi.y * fr;
-    x[i] = xi;
-  }
-}
-
-// Q (row-major, ldq) from Qc; R (row-major k x n, ldr) = upper triangle of W
+// Q (row-major, ldq) from Qc; R (row-major k x n, ldr) from the upper triangle of W
+// with the diagonal from rdiag
 __global__ void k_qr_out(int64_t m, int64_t n, int64_t k, const double2* __restrict__ W,
-                         const double2* __restrict__ Qc, double2* __restrict__ q, int64_t ldq,
-                         double2* __restrict__ r, int64_t ldr) {
+                         const double2* __restrict__ rdiag, const double2* __restrict__ Qc,
+                         double2* __restrict__ q, int64_t ldq, double2* __restrict__ r,
+                         int64_t ldr) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < m * k) {
     int64_t i = t / k, c = t % k;
@@ -153,15 +338,13 @@ __global__ void k_qr_out(int64_t m, int64_t n, int64_t k, const double2* __restr
   }
   if (t < k * n) {
     int64_t i = t / n, c = t % n;
-    r[i * ldr + c] = (i <= c) ? W[c * m + i] : make_double2(0.0, 0.0);
+    r[i * ldr + c] = (i < c) ? W[c * m + i] : (i == c ? rdiag[i] : make_double2(0.0, 0.0));
   }
 }
 
-static inline size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
-
 size_t qr_ws_bytes(int64_t m, int64_t n) {
   int64_t k = m < n ? m : n;
-  return al((size_t)m * n * 16) + al((size_t)m * k * 16) + al((size_t)(k + 1) * 16);
+  return al(qr_fact_bytes(m, n, n)) + al((size_t)m * k * 16);
 }
 
 int qr_run(int64_t m, int64_t n, const void* a, int64_t lda, void* q, int64_t ldq, void* r,
@@ -170,25 +353,16 @@ int qr_run(int64_t m, int64_t n, const void* a, int64_t lda, void* q, int64_t ld
   if (ws_bytes < qr_ws_bytes(m, n)) return PTN_ERR_WORKSPACE;
   const int64_t k = m < n ? m : n;
   if (m == 0 || n == 0) return PTN_OK;
-  char* p = (char*)ws;
-  double2* W = (double2*)p;
-  p += al((size_t)m * n * 16);
-  double2* Qc = (double2*)p;
-  p += al((size_t)m * k * 16);
-  double2* tau = (double2*)p;
-  k_to_colmajor<<<(unsigned)((m * n + 255) / 256), 256, 0, s>>>(m, n, (const double2*)a, lda, W);
-  PTN_LAUNCH_CHECK();
-  for (int64_t j = 0; j < k; ++j) {
-    k_qr_reflector<<<1, 256, 0, s>>>(m, j, W, tau);
-    if (j + 1 < n) k_qr_apply<<<(unsigned)(n - j - 1), 256, 0, s>>>(m, n, j, W, tau);
-  }
-  PTN_LAUNCH_CHECK();
+  QrFact f = qr_fact_layout(m, n, n, ws);
+  double2* Qc = (double2*)((char*)ws + al(qr_fact_bytes(m, n, n)));
+  int st;
+  if ((st = qr_factor(f, (const double2*)a, lda, s))) return st;
   k_q_init<<<(unsigned)((m * k + 255) / 256), 256, 0, s>>>(m, k, Qc);
-  for (int64_t j = k - 1; j >= 0; --j)
-    k_q_apply<<<(unsigned)(k - j), 256, 0, s>>>(m, k, j, W, tau, Qc);
+  PTN_LAUNCH_CHECK();
+  if ((st = qr_apply_q(f, Qc, m, k, true, s))) return st;
   const int64_t tot = (m * k > k * n) ? m * k : k * n;
-  k_qr_out<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(m, n, k, W, Qc, (double2*)q, ldq,
-                                                          (double2*)r, ldr);
+  k_qr_out<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(m, n, k, f.W, f.rdiag, Qc, (double2*)q,
+                                                          ldq, (double2*)r, ldr);
   PTN_LAUNCH_CHECK();
   return PTN_OK;
 }

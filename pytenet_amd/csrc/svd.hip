@@ -2,12 +2,13 @@
 // one-sided (Hestenes) Jacobi, all in FP64.
 //
 // With B = A (m >= n) or B = A^H (m < n), a p x q matrix with p >= q:
-//   1. B = Q R by the Householder QR of qr.hip;
+//   1. B = Q R by the blocked Householder QR of qr.hip (Q kept as reflectors);
 //   2. X = R^H (q x q, column-major, padded with zero columns to a multiple of
 //      2*JB) is orthogonalised by one-sided Jacobi, X V = U_X diag(sigma);
 //      starting from R^H instead of B cuts the number of sweeps several-fold
 //      for graded spectra (QR-preconditioned Jacobi SVD, Drmac & Veselic);
-//   3. B = (Q V) diag(sigma) U_X^H, so U_B = Q V (one complex product) and V_B = U_X.
+//   3. B = (Q V) diag(sigma) U_X^H, so U_B = Q V (the reflectors applied to
+//      [V; 0] in compact WY blocks) and V_B = U_X.
 // Jacobi: columns are grouped in blocks of JB; a round of the block tournament
 // pairs every block with another and each block pair (2*JB columns) is
 // orthogonalised by
@@ -26,6 +27,7 @@
 //   x' = cs x - conj(s) y,   y' = s x + cs y.
 #include <float.h>
 #include "common.h"
+#include "qr.h"
 
 namespace ptn {
 
@@ -105,7 +107,13 @@ __global__ __launch_bounds__(256) void k_jac_gram(int64_t p, int nblk, int r,
 
 // one cyclic Jacobi sweep on the pair's Gram matrix (sum of the partials,
 // fixed order); writes the accumulated unitary J and whether any rotation
-// was needed (one workgroup per pair)
+// was needed (one workgroup per pair). A step of the sweep rotates JB
+// disjoint index pairs (i_k, j_k) at once: with R_k = [[cs, s], [-conj(s), cs]]
+// acting on columns (i_k, j_k), G <- R^H G R is done as independent 2 x 2
+// block updates (rows of pair k, columns of pair l; one thread per (k, l))
+// and J <- J R as independent column-pair updates, so a step needs one
+// barrier after the rotation parameters and one after the updates.
+static_assert(JB * JB == 256, "the solve maps one thread to each (pair, pair) block of G");
 __global__ __launch_bounds__(256) void k_jac_solve(int nblk, const double2* __restrict__ Gp,
                                                    int nseg, double tol, double big,
                                                    double2* __restrict__ Jg,
@@ -129,6 +137,7 @@ __global__ __launch_bounds__(256) void k_jac_solve(int nblk, const double2* __re
   }
   if (t == 0) { rot_any = 0; rot_big = 0; }
   __syncthreads();
+  const int bk = t / JB, bl = t % JB;     // the (row pair, column pair) block of this thread
   for (int step = 0; step < JP - 1; ++step) {
     if (t < JB) {
       int i, j;
@@ -151,29 +160,35 @@ __global__ __launch_bounds__(256) void k_jac_solve(int nblk, const double2* __re
       rc[t] = cs; rsr[t] = sr; rsi[t] = si;
     }
     __syncthreads();
-    // columns of G and J: x' = cs x - conj(s) y, y' = s x + cs y
-    for (int e = t; e < JB * JP * 2; e += 256) {
-      const int k = e % JB, row = (e / JB) % JP, mat = e / (JB * JP);
-      int i, j;
-      round_pair(JP, step, k, i, j);
-      const double cs = rc[k], sr = rsr[k], si = rsi[k];
-      double2 (*M)[JP + 1] = mat ? J : G;
-      const double2 x = M[row][i], y = M[row][j];
-      // conj(s) y = (sr y.x + si y.y, sr y.y - si y.x);  s x = (sr x.x - si x.y, sr x.y + si x.x)
-      M[row][i] = make_double2(cs * x.x - (sr * y.x + si * y.y), cs * x.y - (sr * y.y - si * y.x));
-      M[row][j] = make_double2((sr * x.x - si * x.y) + cs * y.x, (sr * x.y + si * x.x) + cs * y.y);
+    {
+      int ik, jk, il, jl;
+      round_pair(JP, step, bk, ik, jk);
+      round_pair(JP, step, bl, il, jl);
+      const double ck = rc[bk], skr = rsr[bk], ski = rsi[bk];
+      const double cl = rc[bl], slr = rsr[bl], sli = rsi[bl];
+      const double2 b00 = G[ik][il], b01 = G[ik][jl], b10 = G[jk][il], b11 = G[jk][jl];
+      // columns: x' = cl x - conj(sl) y, y' = sl x + cl y (x = column il, y = column jl)
+      double2 c00, c01, c10, c11;
+      c00 = make_double2(cl * b00.x - (slr * b01.x + sli * b01.y), cl * b00.y - (slr * b01.y - sli * b01.x));
+      c10 = make_double2(cl * b10.x - (slr * b11.x + sli * b11.y), cl * b10.y - (slr * b11.y - sli * b11.x));
+      c01 = make_double2((slr * b00.x - sli * b00.y) + cl * b01.x, (slr * b00.y + sli * b00.x) + cl * b01.y);
+      c11 = make_double2((slr * b10.x - sli * b10.y) + cl * b11.x, (slr * b10.y + sli * b10.x) + cl * b11.y);
+      // rows: row_i' = ck row_i - sk row_j, row_j' = conj(sk) row_i + ck row_j
+      G[ik][il] = make_double2(ck * c00.x - (skr * c10.x - ski * c10.y), ck * c00.y - (skr * c10.y + ski * c10.x));
+      G[ik][jl] = make_double2(ck * c01.x - (skr * c11.x - ski * c11.y), ck * c01.y - (skr * c11.y + ski * c11.x));
+      G[jk][il] = make_double2((skr * c00.x + ski * c00.y) + ck * c10.x, (skr * c00.y - ski * c00.x) + ck * c10.y);
+      G[jk][jl] = make_double2((skr * c01.x + ski * c01.y) + ck * c11.x, (skr * c01.y - ski * c01.x) + ck * c11.y);
     }
-    __syncthreads();
-    // rows of G (left multiplication by J^H): row_i' = cs row_i - s row_j,
-    // row_j' = conj(s) row_i + cs row_j
-    for (int e = t; e < JB * JP; e += 256) {
-      const int k = e % JB, col = e / JB;
+    // J <- J R: column pair bl for rows 2*bk and 2*bk + 1
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int row = 2 * bk + h;
       int i, j;
-      round_pair(JP, step, k, i, j);
-      const double cs = rc[k], sr = rsr[k], si = rsi[k];
-      const double2 x = G[i][col], y = G[j][col];
-      G[i][col] = make_double2(cs * x.x - (sr * y.x - si * y.y), cs * x.y - (sr * y.y + si * y.x));
-      G[j][col] = make_double2((sr * x.x + si * x.y) + cs * y.x, (sr * x.y - si * x.x) + cs * y.y);
+      round_pair(JP, step, bl, i, j);
+      const double cs = rc[bl], sr = rsr[bl], si = rsi[bl];
+      const double2 x = J[row][i], y = J[row][j];
+      J[row][i] = make_double2(cs * x.x - (sr * y.x + si * y.y), cs * x.y - (sr * y.y - si * y.x));
+      J[row][j] = make_double2((sr * x.x - si * x.y) + cs * y.x, (sr * x.y + si * x.x) + cs * y.y);
     }
     __syncthreads();
   }
@@ -260,8 +275,8 @@ __global__ __launch_bounds__(256) void k_rank(int64_t q, const double* __restric
 // row-major outputs, pc = perm[c]:
 //   m >= n: U[i][c] = UB[i][pc],           Vh[c][j] = conj(X[j][pc]) / s_c
 //   m <  n: U[i][c] = X[i][pc] / s_c,       Vh[c][j] = conj(UB[j][pc])
-// X is column-major with leading dimension q; UB is row-major p x q.
-__global__ void k_svd_out(int64_t m, int64_t n, int64_t k, int adjoint, int64_t q,
+// X is column-major with leading dimension q; UB is column-major p x q.
+__global__ void k_svd_out(int64_t m, int64_t n, int64_t k, int adjoint, int64_t p, int64_t q,
                           const double2* __restrict__ X, const double2* __restrict__ UB,
                           const int64_t* __restrict__ perm, const double* __restrict__ s,
                           double2* __restrict__ u, int64_t ldu, double2* __restrict__ vh,
@@ -271,7 +286,7 @@ __global__ void k_svd_out(int64_t m, int64_t n, int64_t k, int adjoint, int64_t 
     const int64_t i = t / k, c = t % k, pc = perm[c];
     double2 val;
     if (!adjoint) {
-      val = UB[i * q + pc];
+      val = UB[pc * p + i];
     } else {
       const double2 x = X[pc * q + i];
       const double sc = s[c];
@@ -287,7 +302,7 @@ __global__ void k_svd_out(int64_t m, int64_t n, int64_t k, int adjoint, int64_t 
       const double sc = s[c];
       val = sc > 0 ? make_double2(x.x / sc, -x.y / sc) : make_double2(0.0, 0.0);
     } else {
-      const double2 v = UB[j * q + pc];
+      const double2 v = UB[pc * p + j];
       val = make_double2(v.x, -v.y);
     }
     vh[c * ldvh + j] = val;
@@ -304,18 +319,27 @@ __global__ void k_adjoint(int64_t m, int64_t n, const double2* __restrict__ a, i
   b[j * m + i] = make_double2(v.x, -v.y);
 }
 
-// X (column-major q x qp) = R^H for the row-major q x q R: X[c][r] = conj(R[c][r]) in
-// memory order; padding columns are zero
-__global__ void k_load_rh(int64_t q, int64_t qp, const double2* __restrict__ R,
-                          double2* __restrict__ X) {
+// X (column-major q x qp) = R^H, i.e. X[c*q + i] = conj(R[c][i]), with R from the
+// factorisation (strict upper triangle in the column-major p x q W, diagonal in
+// rdiag); padding columns are zero
+__global__ void k_load_rh(int64_t p, int64_t q, int64_t qp, const double2* __restrict__ W,
+                          const double2* __restrict__ rdiag, double2* __restrict__ X) {
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < q * qp;
        t += (int64_t)gridDim.x * blockDim.x) {
-    if (t < q * q) {
-      const double2 v = R[t];
-      X[t] = make_double2(v.x, -v.y);
-    } else {
-      X[t] = make_double2(0.0, 0.0);
-    }
+    const int64_t c = t / q, i = t % q;
+    double2 v = make_double2(0.0, 0.0);
+    if (c < q) v = (c < i) ? W[i * p + c] : (c == i ? rdiag[c] : v);
+    X[t] = make_double2(v.x, -v.y);
+  }
+}
+
+// UB (column-major p x q) = [V(0:q, 0:q); 0] for V column-major with leading dimension qp
+__global__ void k_embed_v(int64_t p, int64_t q, int64_t qp, const double2* __restrict__ V,
+                          double2* __restrict__ UB) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < p * q;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = t / p, i = t % p;
+    UB[t] = (i < q) ? V[c * qp + i] : make_double2(0.0, 0.0);
   }
 }
 
@@ -341,14 +365,10 @@ static int* pinned_counter() {
   return p;
 }
 
-size_t qr_ws_bytes(int64_t m, int64_t n);
-int qr_run(int64_t m, int64_t n, const void* a, int64_t lda, void* q, int64_t ldq, void* r,
-           int64_t ldr, void* ws, size_t ws_bytes, hipStream_t s);
-
 struct SvdLayout {
   int64_t p, q, qp, nseg;
   int nblk;
-  size_t bt, qm, rm, qrws, x, v, ub, gp, jg, rflag, nrm, perm, cnt, total;
+  size_t bt, fact, x, v, ub, gp, jg, rflag, nrm, perm, cnt, total;
 };
 
 static SvdLayout svd_layout(int64_t m, int64_t n) {
@@ -361,9 +381,7 @@ static SvdLayout svd_layout(int64_t m, int64_t n) {
   const int npairs = L.nblk / 2;
   size_t off = 0;
   L.bt = off; off += (m < n) ? al((size_t)L.p * L.q * 16) : 0;
-  L.qm = off; off += al((size_t)L.p * L.q * 16);
-  L.rm = off; off += al((size_t)L.q * L.q * 16);
-  L.qrws = off; off += al(qr_ws_bytes(L.p, L.q));
+  L.fact = off; off += al(qr_fact_bytes(L.p, L.q, L.q));
   L.x = off; off += al((size_t)L.q * L.qp * 16);
   L.v = off; off += al((size_t)L.qp * L.qp * 16);
   L.ub = off; off += al((size_t)L.p * L.q * 16);
@@ -392,8 +410,6 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
   if (npairs > 65535 || aseg > 65535) return PTN_ERR_ARG;
   const int adjoint = m < n;
   char* base = (char*)ws;
-  double2* Q = (double2*)(base + L.qm);
-  double2* R = (double2*)(base + L.rm);
   double2* X = (double2*)(base + L.x);
   double2* V = (double2*)(base + L.v);
   double2* UB = (double2*)(base + L.ub);
@@ -415,10 +431,10 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
     B = base + L.bt;
     ldb = m;
   }
-  if ((stt = qr_run(L.p, L.q, B, ldb, Q, L.q, R, L.q, base + L.qrws, al(qr_ws_bytes(L.p, L.q)), st)))
-    return stt;
+  QrFact f = qr_fact_layout(L.p, L.q, L.q, base + L.fact);
+  if ((stt = qr_factor(f, (const double2*)B, ldb, st))) return stt;
   // 2. one-sided Jacobi on X = R^H
-  k_load_rh<<<1024, 256, 0, st>>>(L.q, L.qp, R, X);
+  k_load_rh<<<1024, 256, 0, st>>>(L.p, L.q, L.qp, f.W, f.rdiag, X);
   k_eye<<<1024, 256, 0, st>>>(L.qp, V);
   PTN_LAUNCH_CHECK();
   int* host_rot = pinned_counter();
@@ -448,13 +464,14 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
     if (final_sweep) converged = 1;
     else if (*host_rot == 0) final_sweep = 1;
   }
-  // 3. U_B = Q V (V column-major with leading dimension qp)
-  ZGemm g = {L.p, L.q, L.q, Q, L.q, 1, 0, V, 1, L.qp, 0, UB, L.q, 1, 0, 1, 0, 0, 0};
-  if ((stt = zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, st))) return stt;
+  // 3. U_B = Q [V; 0] (V column-major with leading dimension qp)
+  k_embed_v<<<1024, 256, 0, st>>>(L.p, L.q, L.qp, V, UB);
+  PTN_LAUNCH_CHECK();
+  if ((stt = qr_apply_q(f, UB, L.p, L.q, false, st))) return stt;
   k_colnorms<<<(unsigned)L.q, 256, 0, st>>>(L.q, X, nrm);
   k_rank<<<1, 256, 0, st>>>(L.q, nrm, perm, s, counters + 1);
   const int64_t tot = (m * k > k * n) ? m * k : k * n;
-  k_svd_out<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(m, n, k, adjoint, L.q, X, UB, perm, s,
+  k_svd_out<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(m, n, k, adjoint, L.p, L.q, X, UB, perm, s,
                                                             (double2*)u, ldu, (double2*)vh, ldvh);
   if (info) k_svd_info<<<1, 64, 0, st>>>(info, sweeps, converged ? 0 : 1, counters);
   PTN_LAUNCH_CHECK();
