@@ -109,6 +109,21 @@ def heff_roofline(D, prec, reps=10):
     return ms, heff_flops(D, 4, D, 5, 5) / (ms * 1e-3) / 1e12
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heff_2site_D1024_hbm_traffic.txt")
+
+
+def profiled_traffic(prec, D):
+    """HBM bytes per two-site H_eff application at D=1024 from the committed
+    FETCH_SIZE / WRITE_SIZE counter runs (calibrated against a plain copy);
+    None for other sizes or when the summary is absent."""
+    if D != 1024 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith(f"traffic_bytes_{prec}:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def svd_time(D, reps=2):
     import torch
     from pytenet_amd import backend
@@ -285,7 +300,10 @@ def main():
                      "bound": "mfma", "achieved": round(heff_tf_mx, 2),
                      "peak": PEAK_TFLOPS[args.mixed], "unit": "TFLOP/s",
                      "frac": round(heff_tf_mx / PEAK_TFLOPS[args.mixed], 4),
-                     "traffic": None, "avg_ms": round(heff_ms_mx, 3),
+                     "traffic": profiled_traffic(args.mixed, args.D),
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated "
+                                       "(profiles/heff_2site_D1024_hbm_traffic.txt)",
+                     "avg_ms": round(heff_ms_mx, 3),
                      "fp64_achieved": round(heff_tf_64, 2),
                      "fp64_frac": round(heff_tf_64 / PEAK_TFLOPS["fp64"], 4)},
     }

@@ -23,18 +23,25 @@
 
 namespace ptn {
 
-__device__ double wg_sum(double v) {
-  __shared__ double red[256];
-  const int t = threadIdx.x;
-  red[t] = v;
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// sums of a and b over a 256-thread workgroup, returned to every thread; the
+// order of the additions is fixed, so every workgroup that reduces the same
+// data gets bitwise the same result
+__device__ void block_sum2(double& a, double& b) {
+  __shared__ double red[2][4];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { red[0][w] = a; red[1][w] = b; }
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w) red[t] += red[t + w];
-    __syncthreads();
-  }
-  double s = red[0];
+  a = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+  b = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
   __syncthreads();
-  return s;
 }
 
 // column-major copy: W[j*m + i] = A[i*lda + j]
@@ -60,7 +67,9 @@ __global__ __launch_bounds__(256) void k_qr_col(int64_t m, int64_t j, double2* _
     const double2 x = xj[i];
     s += x.x * x.x + x.y * x.y;
   }
-  const double xnorm = sqrt(wg_sum(s));
+  double unused = 0;
+  block_sum2(s, unused);
+  const double xnorm = sqrt(s);
   double2* vj = Vf + j * m;
   if (xnorm == 0.0 && alpha.y == 0.0) {
     // H_j = I: the other columns are unchanged
@@ -110,8 +119,7 @@ __global__ __launch_bounds__(256) void k_qr_col(int64_t m, int64_t j, double2* _
     wr += v.x * xi.x + v.y * xi.y;
     wi += v.x * xi.y - v.y * xi.x;
   }
-  wr = wg_sum(wr);
-  wi = wg_sum(wi);
+  block_sum2(wr, wi);
   const double fr = t.x * wr + t.y * wi, fi = t.x * wi - t.y * wr;   // conj(tau) w
   for (int64_t i = j + threadIdx.x; i < m; i += 256) {
     double2 v;
@@ -138,8 +146,7 @@ __global__ __launch_bounds__(256) void k_qr_vtv(int64_t m, int64_t j0, int nbp,
     sr += x.x * y.x + x.y * y.y;
     si += x.x * y.y - x.y * y.x;
   }
-  sr = wg_sum(sr);
-  si = wg_sum(si);
+  block_sum2(sr, si);
   if (threadIdx.x == 0) G[a * QR_NB + b] = make_double2(sr, si);
 }
 
@@ -149,9 +156,13 @@ __global__ __launch_bounds__(64) void k_qr_tmat(int nbp, const double2* __restri
                                                 const double2* __restrict__ tau,
                                                 double2* __restrict__ T) {
   __shared__ double2 Ts[QR_NB][QR_NB + 1];
+  __shared__ double2 Gs[QR_NB][QR_NB + 1];
   const int r = threadIdx.x;
-  if (r < QR_NB)
-    for (int c = 0; c < QR_NB; ++c) Ts[r][c] = make_double2(0.0, 0.0);
+  for (int e = r; e < QR_NB * QR_NB; e += 64) {
+    const int a = e / QR_NB, b = e % QR_NB;
+    Ts[a][b] = make_double2(0.0, 0.0);
+    Gs[a][b] = (a < b && b < nbp) ? G[e] : make_double2(0.0, 0.0);
+  }
   __syncthreads();
   for (int i = 0; i < nbp; ++i) {
     const double2 ti = tau[i];
@@ -159,7 +170,7 @@ __global__ __launch_bounds__(64) void k_qr_tmat(int nbp, const double2* __restri
     if (r < i) {
       double ar = 0, ai = 0;
       for (int s = r; s < i; ++s) {
-        const double2 a = Ts[r][s], g = G[s * QR_NB + i];
+        const double2 a = Ts[r][s], g = Gs[s][i];
         ar += a.x * g.x - a.y * g.y;
         ai += a.x * g.y + a.y * g.x;
       }
@@ -175,50 +186,46 @@ __global__ __launch_bounds__(64) void k_qr_tmat(int nbp, const double2* __restri
 }
 
 // Y = sum of the nparts partial products P[s] (QR_NB x nc, rows < nbp used);
-// Z = -T^H Y (adjoint = 1) or Z = -T Y (adjoint = 0); one thread per column
-__global__ __launch_bounds__(64) void k_qr_reduce(int nbp, int64_t nc, int nparts,
-                                                  const double2* __restrict__ P,
-                                                  const double2* __restrict__ T, int adjoint,
-                                                  double2* __restrict__ Z) {
+// Z = -T^H Y (adjoint = 1) or Z = -T Y (adjoint = 0). A workgroup handles 8
+// columns; thread (r, cl) sums row r of column cl, then forms row r of Z.
+__global__ __launch_bounds__(256) void k_qr_reduce(int nbp, int64_t nc, int nparts,
+                                                   const double2* __restrict__ P,
+                                                   const double2* __restrict__ T, int adjoint,
+                                                   double2* __restrict__ Z) {
   __shared__ double2 Ts[QR_NB][QR_NB + 1];
-  for (int e = threadIdx.x; e < QR_NB * QR_NB; e += 64) Ts[e / QR_NB][e % QR_NB] = T[e];
-  __syncthreads();
-  const int64_t c = (int64_t)blockIdx.x * 64 + threadIdx.x;
-  if (c >= nc) return;
-  double2 y[QR_NB];
-#pragma unroll
-  for (int r = 0; r < QR_NB; ++r) {
-    double sr = 0, si = 0;
-    if (r < nbp)
-      for (int s = 0; s < nparts; ++s) {
-        const double2 v = P[((int64_t)s * QR_NB + r) * nc + c];
-        sr += v.x;
-        si += v.y;
-      }
-    y[r] = make_double2(sr, si);
-  }
-#pragma unroll
-  for (int r = 0; r < QR_NB; ++r) {
-    double zr = 0, zi = 0;
-    if (adjoint) {
-      // (T^H y)_r = sum_{r' <= r} conj(T[r'][r]) y_r'
-#pragma unroll
-      for (int q = 0; q <= r; ++q) {
-        const double2 a = Ts[q][r];
-        zr += a.x * y[q].x + a.y * y[q].y;
-        zi += a.x * y[q].y - a.y * y[q].x;
-      }
-    } else {
-      // (T y)_r = sum_{r' >= r} T[r][r'] y_r'
-#pragma unroll
-      for (int q = r; q < QR_NB; ++q) {
-        const double2 a = Ts[r][q];
-        zr += a.x * y[q].x - a.y * y[q].y;
-        zi += a.x * y[q].y + a.y * y[q].x;
-      }
+  __shared__ double2 Ys[8][QR_NB + 1];
+  const int t = threadIdx.x;
+  for (int e = t; e < QR_NB * QR_NB; e += 256) Ts[e / QR_NB][e % QR_NB] = T[e];
+  const int cl = t & 7, r = t >> 3;
+  const int64_t c = (int64_t)blockIdx.x * 8 + cl;
+  const bool ok = c < nc && r < nbp;
+  double sr = 0, si = 0;
+  if (ok)
+    for (int s = 0; s < nparts; ++s) {
+      const double2 v = P[((int64_t)s * QR_NB + r) * nc + c];
+      sr += v.x;
+      si += v.y;
     }
-    if (r < nbp) Z[(int64_t)r * nc + c] = make_double2(-zr, -zi);
+  Ys[cl][r] = make_double2(sr, si);
+  __syncthreads();
+  if (!ok) return;
+  double zr = 0, zi = 0;
+  if (adjoint) {
+    // (T^H y)_r = sum_{q <= r} conj(T[q][r]) y_q
+    for (int q = 0; q <= r; ++q) {
+      const double2 a = Ts[q][r], y = Ys[cl][q];
+      zr += a.x * y.x + a.y * y.y;
+      zi += a.x * y.y - a.y * y.x;
+    }
+  } else {
+    // (T y)_r = sum_{q >= r} T[r][q] y_q
+    for (int q = r; q < nbp; ++q) {
+      const double2 a = Ts[r][q], y = Ys[cl][q];
+      zr += a.x * y.x - a.y * y.y;
+      zi += a.x * y.y + a.y * y.x;
+    }
   }
+  Z[(int64_t)r * nc + c] = make_double2(-zr, -zi);
 }
 
 static inline size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -273,7 +280,7 @@ static int apply_block(const QrFact& f, int64_t j0, int nbp, const double2* Tp, 
     if ((st = zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, s))) return st;
   }
   const int nparts = (int)(nfull + (rem > 0 ? 1 : 0));
-  k_qr_reduce<<<(unsigned)((nc + 63) / 64), 64, 0, s>>>(nbp, nc, nparts, f.P, Tp, adjoint, f.Z);
+  k_qr_reduce<<<(unsigned)((nc + 7) / 8), 256, 0, s>>>(nbp, nc, nparts, f.P, Tp, adjoint, f.Z);
   PTN_LAUNCH_CHECK();
   // C += V Z
   ZGemm g = {mr, nc, nbp, V, 1, m, 0, f.Z, nc, 1, 0, Cs, 1, ldc, 1, 1, 0, 0, 0};
