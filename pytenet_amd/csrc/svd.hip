@@ -2,13 +2,15 @@
 // one-sided (Hestenes) Jacobi, all in FP64.
 //
 // With B = A (m >= n) or B = A^H (m < n), a p x q matrix with p >= q:
-//   1. B = Q R by the blocked Householder QR of qr.hip (Q kept as reflectors);
-//   2. X = R^H (q x q, column-major, padded with zero columns to a multiple of
-//      2*JB) is orthogonalised by one-sided Jacobi, X V = U_X diag(sigma);
-//      starting from R^H instead of B cuts the number of sweeps several-fold
-//      for graded spectra (QR-preconditioned Jacobi SVD, Drmac & Veselic);
-//   3. B = (Q V) diag(sigma) U_X^H, so U_B = Q V (the reflectors applied to
-//      [V; 0] in compact WY blocks) and V_B = U_X.
+//   1. B = Q1 R by the blocked Householder QR of qr.hip (Q1 kept as reflectors);
+//   2. R^H = Q2 R2 (second QR), and X = R2^H (q x q, column-major, padded with
+//      zero columns to a multiple of 2*JB) is orthogonalised by one-sided
+//      Jacobi, X V = U_X diag(sigma); starting from R2^H instead of B cuts the
+//      number of sweeps several-fold for graded spectra (the two-sided QR
+//      preconditioning of the Jacobi SVD of Drmac & Veselic);
+//   3. B = Q1 R2^H Q2^H = (Q1 [U_X; 0]) diag(sigma) (Q2 V)^H: U_B and V_B are
+//      formed by applying the reflectors of the two factorisations in compact
+//      WY blocks.
 // Jacobi: columns are grouped in blocks of JB; a round of the block tournament
 // pairs every block with another and each block pair (2*JB columns) is
 // orthogonalised by
@@ -144,18 +146,21 @@ __global__ __launch_bounds__(256) void k_jac_solve(int nblk, const double2* __re
       round_pair(JP, step, t, i, j);
       const double a = G[i][i].x, b = G[j][j].x;
       const double2 c = G[i][j];
-      const double ac = hypot(c.x, c.y);
-      const double scale = sqrt(fmax(a, 0.0)) * sqrt(fmax(b, 0.0));
+      // |c| > tol sqrt(a b) compared in squares (Gram entries already hold
+      // squared column scales); one reciprocal square root gives 1/|c|
+      const double ac2 = c.x * c.x + c.y * c.y;
+      const double ab = fmax(a, 0.0) * fmax(b, 0.0);
       double cs = 1.0, sr = 0.0, si = 0.0;
-      if (ac > 0.0 && ac > tol * scale) {
-        const double zeta = (b - a) / (2.0 * ac);
+      if (ac2 > 0.0 && ac2 > (tol * tol) * ab) {
+        const double iac = rsqrt(ac2);
+        const double zeta = (b - a) * (0.5 * iac);
         const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        cs = 1.0 / sqrt(1.0 + tt * tt);
+        cs = rsqrt(1.0 + tt * tt);
         const double sn = cs * tt;
-        sr = sn * (c.x / ac);   // s = sn e^{i phi}
-        si = sn * (c.y / ac);
+        sr = sn * (c.x * iac);   // s = sn e^{i phi}
+        si = sn * (c.y * iac);
         rot_any = 1;
-        if (ac > big * scale) rot_big = 1;
+        if (ac2 > (big * big) * ab) rot_big = 1;
       }
       rc[t] = cs; rsr[t] = sr; rsi[t] = si;
     }
@@ -272,40 +277,23 @@ __global__ __launch_bounds__(256) void k_rank(int64_t q, const double* __restric
   }
 }
 
-// row-major outputs, pc = perm[c]:
-//   m >= n: U[i][c] = UB[i][pc],           Vh[c][j] = conj(X[j][pc]) / s_c
-//   m <  n: U[i][c] = X[i][pc] / s_c,       Vh[c][j] = conj(UB[j][pc])
-// X is column-major with leading dimension q; UB is column-major p x q.
+// row-major outputs, pc = perm[c], from U_B (column-major p x q) and V_B
+// (column-major q x q) of B = U_B diag(sigma) V_B^H:
+//   m >= n (B = A):    U[i][c] = UB[i][pc],  Vh[c][j] = conj(VB[j][pc])
+//   m <  n (B = A^H):  U[i][c] = VB[i][pc],  Vh[c][j] = conj(UB[j][pc])
 __global__ void k_svd_out(int64_t m, int64_t n, int64_t k, int adjoint, int64_t p, int64_t q,
-                          const double2* __restrict__ X, const double2* __restrict__ UB,
-                          const int64_t* __restrict__ perm, const double* __restrict__ s,
-                          double2* __restrict__ u, int64_t ldu, double2* __restrict__ vh,
-                          int64_t ldvh) {
+                          const double2* __restrict__ UB, const double2* __restrict__ VB,
+                          const int64_t* __restrict__ perm, double2* __restrict__ u, int64_t ldu,
+                          double2* __restrict__ vh, int64_t ldvh) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < m * k) {
     const int64_t i = t / k, c = t % k, pc = perm[c];
-    double2 val;
-    if (!adjoint) {
-      val = UB[pc * p + i];
-    } else {
-      const double2 x = X[pc * q + i];
-      const double sc = s[c];
-      val = sc > 0 ? make_double2(x.x / sc, x.y / sc) : make_double2(0.0, 0.0);
-    }
-    u[i * ldu + c] = val;
+    u[i * ldu + c] = adjoint ? VB[pc * q + i] : UB[pc * p + i];
   }
   if (t < k * n) {
     const int64_t c = t / n, j = t % n, pc = perm[c];
-    double2 val;
-    if (!adjoint) {
-      const double2 x = X[pc * q + j];
-      const double sc = s[c];
-      val = sc > 0 ? make_double2(x.x / sc, -x.y / sc) : make_double2(0.0, 0.0);
-    } else {
-      const double2 v = UB[pc * p + j];
-      val = make_double2(v.x, -v.y);
-    }
-    vh[c * ldvh + j] = val;
+    const double2 v = adjoint ? UB[pc * p + j] : VB[pc * q + j];
+    vh[c * ldvh + j] = make_double2(v.x, -v.y);
   }
 }
 
@@ -319,28 +307,42 @@ __global__ void k_adjoint(int64_t m, int64_t n, const double2* __restrict__ a, i
   b[j * m + i] = make_double2(v.x, -v.y);
 }
 
-// X (column-major q x qp) = R^H, i.e. X[c*q + i] = conj(R[c][i]), with R from the
-// factorisation (strict upper triangle in the column-major p x q W, diagonal in
-// rdiag); padding columns are zero
-__global__ void k_load_rh(int64_t p, int64_t q, int64_t qp, const double2* __restrict__ W,
-                          const double2* __restrict__ rdiag, double2* __restrict__ X) {
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < q * qp;
+// R^H of a factorisation (strict upper triangle of R in the column-major p x q
+// W, diagonal in rdiag): element (i, c) of R^H is conj(R[c][i]). Written either
+// column-major with leading dimension ld and zero padding columns up to ncols
+// (colmajor = 1), or row-major q x q (colmajor = 0).
+__global__ void k_rh(int64_t p, int64_t q, int64_t ld, int64_t ncols, int colmajor,
+                     const double2* __restrict__ W, const double2* __restrict__ rdiag,
+                     double2* __restrict__ out) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ld * ncols;
        t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t c = t / q, i = t % q;
+    const int64_t c = colmajor ? t / ld : t % ld, i = colmajor ? t % ld : t / ld;
     double2 v = make_double2(0.0, 0.0);
-    if (c < q) v = (c < i) ? W[i * p + c] : (c == i ? rdiag[c] : v);
-    X[t] = make_double2(v.x, -v.y);
+    if (c < q && i < q) v = (c < i) ? W[i * p + c] : (c == i ? rdiag[c] : v);
+    out[t] = make_double2(v.x, -v.y);
   }
 }
 
-// UB (column-major p x q) = [V(0:q, 0:q); 0] for V column-major with leading dimension qp
-__global__ void k_embed_v(int64_t p, int64_t q, int64_t qp, const double2* __restrict__ V,
-                          double2* __restrict__ UB) {
+// UB (column-major p x q) = [X / sigma; 0] for X column-major q x q (sigma = its
+// column norms; a zero column stays zero)
+__global__ void k_embed_u(int64_t p, int64_t q, const double2* __restrict__ X,
+                          const double* __restrict__ nrm, double2* __restrict__ UB) {
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < p * q;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t c = t / p, i = t % p;
-    UB[t] = (i < q) ? V[c * qp + i] : make_double2(0.0, 0.0);
+    double2 v = make_double2(0.0, 0.0);
+    const double sc = nrm[c];
+    if (i < q && sc > 0) { const double2 x = X[c * q + i]; v = make_double2(x.x / sc, x.y / sc); }
+    UB[t] = v;
   }
+}
+
+// VB (column-major q x q) = V(0:q, 0:q) for V column-major with leading dimension qp
+__global__ void k_copy_v(int64_t q, int64_t qp, const double2* __restrict__ V,
+                         double2* __restrict__ VB) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < q * q;
+       t += (int64_t)gridDim.x * blockDim.x)
+    VB[t] = V[(t / q) * qp + t % q];
 }
 
 __global__ void k_set_ints(int* p, int n, int v) {
@@ -368,7 +370,7 @@ static int* pinned_counter() {
 struct SvdLayout {
   int64_t p, q, qp, nseg;
   int nblk;
-  size_t bt, fact, x, v, ub, gp, jg, rflag, nrm, perm, cnt, total;
+  size_t bt, fact, fact2, rh, x, v, ub, vb, gp, jg, rflag, nrm, perm, cnt, total;
 };
 
 static SvdLayout svd_layout(int64_t m, int64_t n) {
@@ -382,9 +384,12 @@ static SvdLayout svd_layout(int64_t m, int64_t n) {
   size_t off = 0;
   L.bt = off; off += (m < n) ? al((size_t)L.p * L.q * 16) : 0;
   L.fact = off; off += al(qr_fact_bytes(L.p, L.q, L.q));
+  L.fact2 = off; off += al(qr_fact_bytes(L.q, L.q, L.q));
+  L.rh = off; off += al((size_t)L.q * L.q * 16);
   L.x = off; off += al((size_t)L.q * L.qp * 16);
   L.v = off; off += al((size_t)L.qp * L.qp * 16);
   L.ub = off; off += al((size_t)L.p * L.q * 16);
+  L.vb = off; off += al((size_t)L.q * L.q * 16);
   L.gp = off; off += al((size_t)npairs * L.nseg * JP * JP * 16);
   L.jg = off; off += al((size_t)npairs * JP * JP * 16);
   L.rflag = off; off += al((size_t)npairs * 4);
@@ -413,6 +418,7 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
   double2* X = (double2*)(base + L.x);
   double2* V = (double2*)(base + L.v);
   double2* UB = (double2*)(base + L.ub);
+  double2* VB = (double2*)(base + L.vb);
   double2* Gp = (double2*)(base + L.gp);
   double2* Jg = (double2*)(base + L.jg);
   int* rflag = (int*)(base + L.rflag);
@@ -433,8 +439,13 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
   }
   QrFact f = qr_fact_layout(L.p, L.q, L.q, base + L.fact);
   if ((stt = qr_factor(f, (const double2*)B, ldb, st))) return stt;
-  // 2. one-sided Jacobi on X = R^H
-  k_load_rh<<<1024, 256, 0, st>>>(L.p, L.q, L.qp, f.W, f.rdiag, X);
+  // 2. R^H = Q2 R2, and one-sided Jacobi on X = R2^H
+  QrFact f2 = qr_fact_layout(L.q, L.q, L.q, base + L.fact2);
+  double2* RH = (double2*)(base + L.rh);
+  k_rh<<<1024, 256, 0, st>>>(L.p, L.q, L.q, L.q, 0, f.W, f.rdiag, RH);
+  PTN_LAUNCH_CHECK();
+  if ((stt = qr_factor(f2, RH, L.q, st))) return stt;
+  k_rh<<<1024, 256, 0, st>>>(L.q, L.q, L.q, L.qp, 1, f2.W, f2.rdiag, X);
   k_eye<<<1024, 256, 0, st>>>(L.qp, V);
   PTN_LAUNCH_CHECK();
   int* host_rot = pinned_counter();
@@ -464,15 +475,18 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
     if (final_sweep) converged = 1;
     else if (*host_rot == 0) final_sweep = 1;
   }
-  // 3. U_B = Q [V; 0] (V column-major with leading dimension qp)
-  k_embed_v<<<1024, 256, 0, st>>>(L.p, L.q, L.qp, V, UB);
+  // 3. B = Q1 R2^H Q2^H = (Q1 [U_X; 0]) diag(sigma) (Q2 V)^H with U_X = X / sigma
+  k_colnorms<<<(unsigned)L.q, 256, 0, st>>>(L.q, X, nrm);
+  k_embed_u<<<1024, 256, 0, st>>>(L.p, L.q, X, nrm, UB);
+  k_copy_v<<<1024, 256, 0, st>>>(L.q, L.qp, V, VB);
   PTN_LAUNCH_CHECK();
   if ((stt = qr_apply_q(f, UB, L.p, L.q, false, st))) return stt;
-  k_colnorms<<<(unsigned)L.q, 256, 0, st>>>(L.q, X, nrm);
+  if ((stt = qr_apply_q(f2, VB, L.q, L.q, false, st))) return stt;
   k_rank<<<1, 256, 0, st>>>(L.q, nrm, perm, s, counters + 1);
   const int64_t tot = (m * k > k * n) ? m * k : k * n;
-  k_svd_out<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(m, n, k, adjoint, L.p, L.q, X, UB, perm, s,
-                                                            (double2*)u, ldu, (double2*)vh, ldvh);
+  k_svd_out<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(m, n, k, adjoint, L.p, L.q, UB, VB,
+                                                            perm, (double2*)u, ldu, (double2*)vh,
+                                                            ldvh);
   if (info) k_svd_info<<<1, 64, 0, st>>>(info, sweeps, converged ? 0 : 1, counters);
   PTN_LAUNCH_CHECK();
   return PTN_OK;
