@@ -14,10 +14,11 @@
 // Jacobi: columns are grouped in blocks of JB; a round of the block tournament
 // pairs every block with another and each block pair (2*JB columns) is
 // orthogonalised by
-//   k_jac_gram:  partial Gram matrices G = Y^H Y of the pair's columns over row segments;
+//   k_jac_gram:  partial Gram matrices G = Y^H Y of the pair's columns over row
+//                segments (FP64 MFMA);
 //   k_jac_solve: one cyclic sweep of two-sided Jacobi on G in LDS (parallel
 //                ordering, JB disjoint rotations per step), accumulating the unitary J;
-//   k_jac_apply: Y <- Y J and V <- V J, one row per thread.
+//   k_jac_apply: Y <- Y J and V <- V J (FP64 MFMA).
 // Sweeps repeat until no pair needs a rotation, i.e. every column pair
 // satisfies |x_i^H x_j| <= tol |x_i| |x_j|. Then sigma_j = |x_j| and
 // U_X = X / sigma. One-sided Jacobi computes small singular values to high
@@ -35,8 +36,9 @@ namespace ptn {
 
 constexpr int JB = 16;          // columns per block
 constexpr int JP = 2 * JB;      // columns per block pair
-constexpr int SEG = 128;        // rows per workgroup segment (apply: one row per thread)
+constexpr int GSEG = 256;       // rows per Gram workgroup (one partial Gram matrix each)
 constexpr int GCH = 64;         // rows per LDS chunk in the Gram kernel
+constexpr int AROWS = 64;       // rows per apply workgroup (16 per wave)
 
 __device__ double wg_sum3(double v, double* red) {
   const int t = threadIdx.x;
@@ -70,7 +72,15 @@ __device__ __forceinline__ int64_t pair_col(int bp, int bq, int c) {
   return (c < JB) ? (int64_t)bp * JB + c : (int64_t)bq * JB + (c - JB);
 }
 
-// partial Gram matrices: Gp[pair][seg][i][j] = sum_{rows in seg} conj(Y[row][i]) Y[row][j]
+// partial Gram matrices Gp[pair][seg][i][j] = sum_{rows in seg} conj(Y[row][i]) Y[row][j]
+// on FP64 MFMA (v_mfma_f64_16x16x4f64) through the real embedding of the
+// complex product with A = Y^H (real k = 2 row + part):
+//   A_real[i][2r] = Re Y[r][i], A_real[i][2r+1] = -Im Y[r][i];
+//   B_real[2r + q][2j + p] = (Re, Im; -Im, Re)[q][p] of Y[r][j].
+// The 32 x 64 real result is 8 tiles of 16 x 16; wave w owns row tile w & 1 and
+// column tiles 2 (w >> 1) and 2 (w >> 1) + 1. Rows are staged through LDS in
+// chunks of GCH.
+static_assert(JP == 32, "the Gram and apply kernels are laid out for 32-column block pairs");
 __global__ __launch_bounds__(256) void k_jac_gram(int64_t p, int nblk, int r,
                                                   const double2* __restrict__ X,
                                                   double2* __restrict__ Gp, int nseg) {
@@ -78,12 +88,14 @@ __global__ __launch_bounds__(256) void k_jac_gram(int64_t p, int nblk, int r,
   int bp, bq;
   round_pair(nblk, r, blockIdx.y, bp, bq);
   const int seg = blockIdx.x;
-  const int64_t r0 = (int64_t)seg * SEG;
-  const int64_t r1 = (r0 + SEG < p) ? r0 + SEG : p;
+  const int64_t r0 = (int64_t)seg * GSEG;
+  const int64_t r1 = (r0 + GSEG < p) ? r0 + GSEG : p;
   const int t = threadIdx.x;
-  // thread t owns the 4 entries (i, j) = (t / 8, 4 * (t % 8) + e)
-  const int gi = t >> 3, gj = 4 * (t & 7);
-  double gr[4] = {0, 0, 0, 0}, gim[4] = {0, 0, 0, 0};
+  const int lane = t & 63, wave = t >> 6;
+  const int l16 = lane & 15, kk = lane >> 4;
+  const int i0 = 16 * (wave & 1), jt0 = 2 * (wave >> 1);
+  const int q = kk & 1, pb = l16 & 1;
+  d4_t acc[2] = {d4_t{0, 0, 0, 0}, d4_t{0, 0, 0, 0}};
   for (int64_t c0 = r0; c0 < r1; c0 += GCH) {
     __syncthreads();
     for (int e = t; e < GCH * JP; e += 256) {
@@ -92,19 +104,41 @@ __global__ __launch_bounds__(256) void k_jac_gram(int64_t p, int nblk, int r,
       Y[row][col] = (grow < r1) ? X[pair_col(bp, bq, col) * p + grow] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    for (int row = 0; row < GCH; ++row) {
-      const double2 yi = Y[row][gi];
+#pragma unroll 4
+    for (int s = 0; s < GCH / 2; ++s) {
+      const int row = 2 * s + (kk >> 1);
+      const double2 ya = Y[row][i0 + l16];
+      const double a = q ? -ya.y : ya.x;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const double2 yj = Y[row][gj + e];
-        gr[e] += yi.x * yj.x + yi.y * yj.y;
-        gim[e] += yi.x * yj.y - yi.y * yj.x;
+      for (int h = 0; h < 2; ++h) {
+        const double2 yb = Y[row][8 * (jt0 + h) + (l16 >> 1)];
+        const double b = (q == pb) ? yb.x : (q ? -yb.y : yb.y);
+        acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[h], 0, 0, 0);
       }
     }
   }
+  // result element v of tile h: complex row i0 + kk + 4v, real column l16 of the
+  // tile (complex column 8 (jt0 + h) + l16 / 2, real part for even l16); the
+  // partner lane l16 ^ 1 holds the other part
   double2* out = Gp + ((int64_t)blockIdx.y * nseg + seg) * JP * JP;
 #pragma unroll
-  for (int e = 0; e < 4; ++e) out[gi * JP + gj + e] = make_double2(gr[e], gim[e]);
+  for (int h = 0; h < 2; ++h) {
+    double mine[4], other[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      mine[v] = acc[h][v];
+      other[v] = __shfl_xor(mine[v], 1);
+    }
+    const int j = 8 * (jt0 + h) + (l16 >> 1);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      // even lanes store elements 0, 1 and odd lanes elements 2, 3 (constant indices)
+      const int i = i0 + kk + 4 * (2 * pb + u);
+      const double re = pb ? other[2 + u] : mine[u];
+      const double im = pb ? mine[2 + u] : other[u];
+      out[i * JP + j] = make_double2(re, im);
+    }
+  }
 }
 
 // one cyclic Jacobi sweep on the pair's Gram matrix (sum of the partials,
@@ -205,39 +239,70 @@ __global__ __launch_bounds__(256) void k_jac_solve(int nblk, const double2* __re
   }
 }
 
-// Y <- Y J and V <- V J for the rows of segment blockIdx.x (one row per thread)
-__global__ __launch_bounds__(128) void k_jac_apply(int64_t p, int64_t qp, int nblk, int r,
+// Y <- Y J (pass blockIdx.z = 0, rows of X) or V <- V J (pass 1) for AROWS rows
+// per workgroup, 16 per wave, on FP64 MFMA: the rows' 32 complex entries are
+// the A operand (real k = 2 column + part, the interleaved complex layout) and
+// J's real embedding [[Re, Im], [-Im, Re]] the B operand, read from LDS.
+// Every wave loads all A operands of its rows before it stores any result.
+__global__ __launch_bounds__(256) void k_jac_apply(int64_t p, int64_t qp, int nblk, int r,
                                                    double2* __restrict__ X,
                                                    double2* __restrict__ V,
                                                    const double2* __restrict__ Jg,
                                                    const int* __restrict__ rotflag) {
   __shared__ double2 J[JP][JP + 1];
   if (!rotflag[blockIdx.y]) return;
+  const int pass = blockIdx.z;
+  double2* M = pass ? V : X;
+  const int64_t ld = pass ? qp : p;
+  const int64_t rowblk = (int64_t)blockIdx.x * AROWS;
+  if (rowblk >= ld) return;
   int bp, bq;
   round_pair(nblk, r, blockIdx.y, bp, bq);
   const int t = threadIdx.x;
   const double2* jsrc = Jg + (int64_t)blockIdx.y * JP * JP;
-  for (int e = t; e < JP * JP; e += SEG) J[e / JP][e % JP] = jsrc[e];
+  for (int e = t; e < JP * JP; e += 256) J[e / JP][e % JP] = jsrc[e];
   __syncthreads();
-  for (int pass = 0; pass < 2; ++pass) {
-    double2* M = pass ? V : X;
-    const int64_t ld = pass ? qp : p;
-    const int64_t row = (int64_t)blockIdx.x * SEG + t;
-    if (row < ld) {
-      double2 y[JP];
+  const int lane = t & 63, wave = t >> 6;
+  const int l16 = lane & 15, kk = lane >> 4;
+  const int q = kk & 1, pb = l16 & 1;
+  const int64_t row0 = rowblk + 16 * wave;
+  const int64_t arow = row0 + l16;
+  const bool aok = arow < ld;
+  double a[16];
 #pragma unroll
-      for (int c = 0; c < JP; ++c) y[c] = M[pair_col(bp, bq, c) * ld + row];
-#pragma unroll 4
-      for (int c = 0; c < JP; ++c) {
-        double sr = 0, si = 0;
+  for (int s = 0; s < 16; ++s) {
+    const int kc = 2 * s + (kk >> 1);
+    const double2 y = aok ? M[pair_col(bp, bq, kc) * ld + arow] : make_double2(0.0, 0.0);
+    a[s] = q ? y.y : y.x;
+  }
+  d4_t acc[4];
 #pragma unroll
-        for (int kk = 0; kk < JP; ++kk) {
-          const double2 jk = J[kk][c];
-          sr += y[kk].x * jk.x - y[kk].y * jk.y;
-          si += y[kk].x * jk.y + y[kk].y * jk.x;
-        }
-        M[pair_col(bp, bq, c) * ld + row] = make_double2(sr, si);
-      }
+  for (int tile = 0; tile < 4; ++tile) acc[tile] = d4_t{0, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int kc = 2 * s + (kk >> 1);
+#pragma unroll
+    for (int tile = 0; tile < 4; ++tile) {
+      const double2 jv = J[kc][8 * tile + (l16 >> 1)];
+      const double b = (q == pb) ? jv.x : (q ? -jv.y : jv.y);
+      acc[tile] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b, acc[tile], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int tile = 0; tile < 4; ++tile) {
+    double mine[4], other[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      mine[v] = acc[tile][v];
+      other[v] = __shfl_xor(mine[v], 1);
+    }
+    double2* col = M + pair_col(bp, bq, 8 * tile + (l16 >> 1)) * ld;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int64_t row = row0 + kk + 4 * (2 * pb + u);
+      const double re = pb ? other[2 + u] : mine[u];
+      const double im = pb ? mine[2 + u] : other[u];
+      if (row < ld) col[row] = make_double2(re, im);
     }
   }
 }
@@ -379,7 +444,7 @@ static SvdLayout svd_layout(int64_t m, int64_t n) {
   L.p = m < n ? n : m;
   L.qp = ((L.q + JP - 1) / JP) * JP;
   L.nblk = (int)(L.qp / JB);
-  L.nseg = (L.q + SEG - 1) / SEG;     // Jacobi runs on X = R^H with q rows
+  L.nseg = (L.q + GSEG - 1) / GSEG;   // Jacobi runs on X = R2^H with q rows
   const int npairs = L.nblk / 2;
   size_t off = 0;
   L.bt = off; off += (m < n) ? al((size_t)L.p * L.q * 16) : 0;
@@ -410,8 +475,7 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
   const SvdLayout L = svd_layout(m, n);
   if (ws_bytes < L.total) return PTN_ERR_WORKSPACE;
   const int npairs = L.nblk / 2;
-  const int64_t nvseg = (L.qp + SEG - 1) / SEG;
-  const int64_t aseg = L.nseg > nvseg ? L.nseg : nvseg;
+  const int64_t aseg = (L.qp + AROWS - 1) / AROWS;   // qp >= q rows
   if (npairs > 65535 || aseg > 65535) return PTN_ERR_ARG;
   const int adjoint = m < n;
   char* base = (char*)ws;
@@ -465,7 +529,7 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
                                                                            (int)L.nseg);
       k_jac_solve<<<(unsigned)npairs, 256, 0, st>>>(L.nblk, Gp, (int)L.nseg, tol, big, Jg, rflag,
                                                      counters);
-      k_jac_apply<<<dim3((unsigned)aseg, (unsigned)npairs), SEG, 0, st>>>(
+      k_jac_apply<<<dim3((unsigned)aseg, (unsigned)npairs, 2), 256, 0, st>>>(
           L.q, L.qp, L.nblk, r, X, V, Jg, rflag);
     }
     PTN_LAUNCH_CHECK();
