@@ -151,38 +151,43 @@ __global__ __launch_bounds__(256) void k_qr_vtv(int64_t m, int64_t j0, int nbp,
 }
 
 // compact WY factor (forward, columnwise): T[i][i] = tau_i,
-// T[0:i, i] = -tau_i T[0:i, 0:i] (V[:, 0:i]^H v_i); zero padded to QR_NB x QR_NB
+// T[0:i, i] = -tau_i T[0:i, 0:i] (V[:, 0:i]^H v_i); zero padded to QR_NB x QR_NB.
+// Thread r builds row r of T in registers (it only needs its own row and G),
+// so the column recurrence needs no barriers.
 __global__ __launch_bounds__(64) void k_qr_tmat(int nbp, const double2* __restrict__ G,
                                                 const double2* __restrict__ tau,
                                                 double2* __restrict__ T) {
-  __shared__ double2 Ts[QR_NB][QR_NB + 1];
   __shared__ double2 Gs[QR_NB][QR_NB + 1];
+  __shared__ double2 taus[QR_NB];
   const int r = threadIdx.x;
   for (int e = r; e < QR_NB * QR_NB; e += 64) {
     const int a = e / QR_NB, b = e % QR_NB;
-    Ts[a][b] = make_double2(0.0, 0.0);
     Gs[a][b] = (a < b && b < nbp) ? G[e] : make_double2(0.0, 0.0);
   }
+  if (r < QR_NB) taus[r] = (r < nbp) ? tau[r] : make_double2(0.0, 0.0);
   __syncthreads();
-  for (int i = 0; i < nbp; ++i) {
-    const double2 ti = tau[i];
-    double2 val = make_double2(0.0, 0.0);
-    if (r < i) {
+  if (r >= QR_NB) return;
+  double2 trow[QR_NB];
+#pragma unroll
+  for (int c = 0; c < QR_NB; ++c) trow[c] = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < QR_NB; ++i) {
+    if (i < nbp) {
+      const double2 ti = taus[i];
       double ar = 0, ai = 0;
-      for (int s = r; s < i; ++s) {
-        const double2 a = Ts[r][s], g = Gs[s][i];
+#pragma unroll
+      for (int s = 0; s < i; ++s) {
+        // T[r][s] is zero for s < r, so the full range gives the same sum
+        const double2 a = trow[s], g = Gs[s][i];
         ar += a.x * g.x - a.y * g.y;
         ai += a.x * g.y + a.y * g.x;
       }
-      val = make_double2(-(ti.x * ar - ti.y * ai), -(ti.x * ai + ti.y * ar));
+      if (r < i) trow[i] = make_double2(-(ti.x * ar - ti.y * ai), -(ti.x * ai + ti.y * ar));
+      else if (r == i) trow[i] = ti;
     }
-    __syncthreads();
-    if (r < i) Ts[r][i] = val;
-    if (r == i) Ts[i][i] = ti;
-    __syncthreads();
   }
-  if (r < QR_NB)
-    for (int c = 0; c < QR_NB; ++c) T[r * QR_NB + c] = Ts[r][c];
+#pragma unroll
+  for (int c = 0; c < QR_NB; ++c) T[r * QR_NB + c] = trow[c];
 }
 
 // Y = sum of the nparts partial products P[s] (QR_NB x nc, rows < nbp used);

@@ -36,7 +36,7 @@ namespace ptn {
 
 constexpr int JB = 16;          // columns per block
 constexpr int JP = 2 * JB;      // columns per block pair
-constexpr int GSEG = 256;       // rows per Gram workgroup (one partial Gram matrix each)
+constexpr int GSEG = 128;       // rows per Gram workgroup (one partial Gram matrix each)
 constexpr int GCH = 64;         // rows per LDS chunk in the Gram kernel
 constexpr int AROWS = 64;       // rows per apply workgroup (16 per wave)
 
@@ -96,14 +96,28 @@ __global__ __launch_bounds__(256) void k_jac_gram(int64_t p, int nblk, int r,
   const int i0 = 16 * (wave & 1), jt0 = 2 * (wave >> 1);
   const int q = kk & 1, pb = l16 & 1;
   d4_t acc[2] = {d4_t{0, 0, 0, 0}, d4_t{0, 0, 0, 0}};
-  for (int64_t c0 = r0; c0 < r1; c0 += GCH) {
-    __syncthreads();
-    for (int e = t; e < GCH * JP; e += 256) {
+  // the next chunk is loaded into registers while the current one is multiplied
+  constexpr int NL = GCH * JP / 256;
+  double2 pre[NL];
+  auto load_chunk = [&](int64_t c0) {
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+      const int e = t + 256 * u;
       const int col = e / GCH, row = e % GCH;
       const int64_t grow = c0 + row;
-      Y[row][col] = (grow < r1) ? X[pair_col(bp, bq, col) * p + grow] : make_double2(0.0, 0.0);
+      pre[u] = (grow < r1) ? X[pair_col(bp, bq, col) * p + grow] : make_double2(0.0, 0.0);
+    }
+  };
+  load_chunk(r0);
+  for (int64_t c0 = r0; c0 < r1; c0 += GCH) {
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+      const int e = t + 256 * u;
+      Y[e % GCH][e / GCH] = pre[u];
     }
     __syncthreads();
+    if (c0 + GCH < r1) load_chunk(c0 + GCH);
 #pragma unroll 4
     for (int s = 0; s < GCH / 2; ++s) {
       const int row = 2 * s + (kk >> 1);
@@ -161,15 +175,28 @@ __global__ __launch_bounds__(256) void k_jac_solve(int nblk, const double2* __re
   __shared__ int rot_any, rot_big;
   const int t = threadIdx.x;
   const double2* gsrc = Gp + (int64_t)blockIdx.x * nseg * JP * JP;
-  for (int e = t; e < JP * JP; e += 256) {
-    double sr = 0, si = 0;
+  {
+    // sum of the partial Gram matrices in a fixed order; the four elements of a
+    // thread and several partials are loaded before they are added
+    constexpr int NE = JP * JP / 256;
+    double sr[NE], si[NE];
+#pragma unroll
+    for (int u = 0; u < NE; ++u) { sr[u] = 0; si[u] = 0; }
+#pragma unroll 4
     for (int s = 0; s < nseg; ++s) {
-      const double2 v = gsrc[(int64_t)s * JP * JP + e];
-      sr += v.x;
-      si += v.y;
+#pragma unroll
+      for (int u = 0; u < NE; ++u) {
+        const double2 v = gsrc[(int64_t)s * JP * JP + t + 256 * u];
+        sr[u] += v.x;
+        si[u] += v.y;
+      }
     }
-    G[e / JP][e % JP] = make_double2(sr, si);
-    J[e / JP][e % JP] = make_double2((e / JP) == (e % JP) ? 1.0 : 0.0, 0.0);
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int e = t + 256 * u;
+      G[e / JP][e % JP] = make_double2(sr[u], si[u]);
+      J[e / JP][e % JP] = make_double2((e / JP) == (e % JP) ? 1.0 : 0.0, 0.0);
+    }
   }
   if (t == 0) { rot_any = 0; rot_big = 0; }
   __syncthreads();
