@@ -12,12 +12,13 @@ constexpr int QR_SPLITK = 8;    // K chunks of the V^H C products
 //   Vf    column-major m x k Householder vectors v_j (v_j[j] = 1, zero above)
 //   tau   k scalar factors, H_j = I - tau_j v_j v_j^H
 //   rdiag k diagonal entries of R
+//   rsup  k superdiagonal entries R[j][j+1] produced by two-reflector launches
 //   T     per panel the QR_NB x QR_NB upper-triangular factor of the compact
 //         WY form H_j0 ... H_j0+nb-1 = I - V T V^H (row-major, zero padded)
 // plus scratch for the blocked updates (Vf^H C partials, the reduced product).
 struct QrFact {
   int64_t m, n, k, npan, ncmax;
-  double2 *W, *Vf, *tau, *rdiag, *T, *G, *P, *Z;
+  double2 *W, *Vf, *tau, *rdiag, *rsup, *T, *G, *P, *Z;
 };
 
 size_t qr_fact_bytes(int64_t m, int64_t n, int64_t ncmax);

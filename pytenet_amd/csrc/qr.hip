@@ -9,9 +9,10 @@
 //
 // Blocked right-looking algorithm on a column-major copy (panels of QR_NB
 // columns):
-//   panel:    for each column j one launch (k_qr_col) computes reflector j in
-//             every workgroup (the same deterministic reduction) and applies
-//             it to one remaining column of the panel per workgroup;
+//   panel:    for each pair of columns j, j+1 one launch (k_qr_col2) computes
+//             reflectors j and j+1 in every workgroup (the same deterministic
+//             reductions) and applies them to one remaining column of the panel
+//             per workgroup (k_qr_col for a last odd column);
 //   WY form:  T from V^H V (k_qr_vtv, k_qr_tmat), H_j0 ... = I - V T V^H;
 //   trailing: A <- A - V (T^H (V^H A)): V^H A as a split-K complex product on
 //             MFMA, the sum of the K chunks and the small T^H product in
@@ -43,6 +44,53 @@ __device__ void block_sum2(double& a, double& b) {
   b = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
   __syncthreads();
 }
+
+// sums of N values over a 256-thread workgroup (fixed order, as block_sum2)
+template <int N> __device__ void block_sum(double (&v)[N]) {
+  __shared__ double red[N][4];
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    v[k] = wave_sum(v[k]);
+    if ((threadIdx.x & 63) == 0) red[k][w] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
+  __syncthreads();
+}
+
+// Householder reflector from alpha and |x| (LAPACK zlarfg conventions):
+// H = I when x = 0 and alpha is real (tau = 0, R entry alpha, v = e_j);
+// else beta = -sign(Re alpha) |(alpha, x)|, tau = (beta - alpha) / beta and
+// v = [1, x / (alpha - beta)], held as the complex factor sc = 1 / (alpha - beta)
+struct Refl {
+  double2 tau, rd, sc;
+};
+__device__ __forceinline__ Refl make_refl(double2 alpha, double xnorm) {
+  Refl h;
+  if (xnorm == 0.0 && alpha.y == 0.0) {
+    h.tau = make_double2(0.0, 0.0);
+    h.rd = alpha;
+    h.sc = make_double2(0.0, 0.0);
+    return h;
+  }
+  const double ar = fabs(alpha.x), ai = fabs(alpha.y);
+  const double mx = fmax(fmax(ar, ai), xnorm);
+  const double nrm = mx * sqrt((ar / mx) * (ar / mx) + (ai / mx) * (ai / mx) +
+                               (xnorm / mx) * (xnorm / mx));
+  const double beta = -copysign(nrm, alpha.x);
+  const double dr = alpha.x - beta, di = alpha.y;
+  const double den = dr * dr + di * di;
+  h.sc = make_double2(dr / den, -di / den);
+  h.tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+  h.rd = make_double2(beta, 0.0);
+  return h;
+}
+__device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 conj2(double2 a) { return make_double2(a.x, -a.y); }
 
 // column-major copy: W[j*m + i] = A[i*lda + j]
 __global__ void k_to_colmajor(int64_t m, int64_t n, const double2* __restrict__ a, int64_t lda,
@@ -130,6 +178,110 @@ __global__ __launch_bounds__(256) void k_qr_col(int64_t m, int64_t j, double2* _
     xi.y -= v.x * fi + v.y * fr;
     xc[i] = xi;
   }
+}
+
+// reflectors j and j+1 in one launch; workgroup b handles column c = j + b.
+// Every workgroup forms reflector j from column j, applies it to column j+1
+// on the fly (y = H_j^H x_{j+1}, never stored) and forms reflector j+1 from y,
+// all with the same deterministic reductions. Workgroup 0 stores v_j, tau_j
+// and R[j][j]; workgroup 1 stores v_{j+1}, tau_{j+1}, R[j+1][j+1] and R[j][j+1]
+// (into rsup: columns j and j+1 of W are read by every workgroup during the
+// launch and are not written); workgroups b >= 2 apply H_j^H then H_{j+1}^H to
+// their column.
+__global__ __launch_bounds__(256) void k_qr_col2(int64_t m, int64_t j, double2* __restrict__ W,
+                                                 double2* __restrict__ Vf,
+                                                 double2* __restrict__ tau,
+                                                 double2* __restrict__ rdiag,
+                                                 double2* __restrict__ rsup) {
+  const double2* xj = W + j * m;
+  const double2* xk = W + (j + 1) * m;
+  const int b = blockIdx.x;
+  const int t = threadIdx.x;
+  double n1[1] = {0};
+  for (int64_t i = j + 1 + t; i < m; i += 256) {
+    const double2 x = xj[i];
+    n1[0] += x.x * x.x + x.y * x.y;
+  }
+  block_sum<1>(n1);
+  const Refl h1 = make_refl(xj[j], sqrt(n1[0]));
+  auto v1 = [&](int64_t i) -> double2 {
+    return i == j ? make_double2(1.0, 0.0) : cmul2(xj[i], h1.sc);   // i >= j
+  };
+  if (b == 0) {
+    double2* vj = Vf + j * m;
+    for (int64_t i = t; i < m; i += 256) vj[i] = (i < j) ? make_double2(0.0, 0.0) : v1(i);
+    if (t == 0) { tau[j] = h1.tau; rdiag[j] = h1.rd; }
+    return;
+  }
+  double2* xc = W + (j + b) * m;
+  const bool own = b >= 2;
+  // w1 = v1^H x_{j+1}, wc = v1^H x_c (rows >= j)
+  double d2[4] = {0, 0, 0, 0};
+  for (int64_t i = j + t; i < m; i += 256) {
+    const double2 v = v1(i), x = xk[i];
+    d2[0] += v.x * x.x + v.y * x.y;
+    d2[1] += v.x * x.y - v.y * x.x;
+    if (own) {
+      const double2 z = xc[i];
+      d2[2] += v.x * z.x + v.y * z.y;
+      d2[3] += v.x * z.y - v.y * z.x;
+    }
+  }
+  block_sum<4>(d2);
+  const double2 f1 = cmul2(conj2(h1.tau), make_double2(d2[0], d2[1]));
+  const double2 fc = cmul2(conj2(h1.tau), make_double2(d2[2], d2[3]));
+  auto y = [&](int64_t i) -> double2 {   // H_j^H x_{j+1}, rows >= j
+    const double2 v = v1(i), g = cmul2(f1, v), x = xk[i];
+    return make_double2(x.x - g.x, x.y - g.y);
+  };
+  auto z = [&](int64_t i) -> double2 {   // H_j^H x_c, rows >= j
+    const double2 v = v1(i), g = cmul2(fc, v), x = xc[i];
+    return make_double2(x.x - g.x, x.y - g.y);
+  };
+  // |y| below row j+1, and y^H z below row j+1
+  double d3[3] = {0, 0, 0};
+  for (int64_t i = j + 2 + t; i < m; i += 256) {
+    const double2 yi = y(i);
+    d3[0] += yi.x * yi.x + yi.y * yi.y;
+    if (own) {
+      const double2 zi = z(i);
+      d3[1] += yi.x * zi.x + yi.y * zi.y;
+      d3[2] += yi.x * zi.y - yi.y * zi.x;
+    }
+  }
+  block_sum<3>(d3);
+  const double2 alpha2 = y(j + 1);
+  const Refl h2 = make_refl(alpha2, sqrt(d3[0]));
+  auto v2 = [&](int64_t i) -> double2 {   // i >= j+1
+    return i == j + 1 ? make_double2(1.0, 0.0) : cmul2(y(i), h2.sc);
+  };
+  if (b == 1) {
+    double2* vk = Vf + (j + 1) * m;
+    for (int64_t i = t; i < m; i += 256) vk[i] = (i <= j) ? make_double2(0.0, 0.0) : v2(i);
+    if (t == 0) { tau[j + 1] = h2.tau; rdiag[j + 1] = h2.rd; rsup[j] = y(j); }
+    return;
+  }
+  // w2 = v2^H z = z_{j+1} + conj(sc2) (y^H z)_{> j+1};  x_c <- z - conj(tau2) w2 v2
+  const double2 zk = (j + 1 < m) ? z(j + 1) : make_double2(0.0, 0.0);
+  const double2 yz = cmul2(conj2(h2.sc), make_double2(d3[1], d3[2]));
+  const double2 f2 = cmul2(conj2(h2.tau), make_double2(zk.x + yz.x, zk.y + yz.y));
+  // every row is read into registers before the row is written (same thread)
+  for (int64_t i = j + t; i < m; i += 256) {
+    double2 o = z(i);
+    if (i >= j + 1) {
+      const double2 g = cmul2(f2, v2(i));
+      o.x -= g.x;
+      o.y -= g.y;
+    }
+    xc[i] = o;
+  }
+}
+
+// R[jj][jj+1] of the two-reflector launches of a panel, written after the panel
+__global__ void k_qr_fixsup(int64_t m, int64_t j0, int64_t pe, const double2* __restrict__ rsup,
+                            double2* __restrict__ W) {
+  const int64_t jj = j0 + 2 * (int64_t)threadIdx.x;
+  if (jj + 1 < pe) W[(jj + 1) * m + jj] = rsup[jj];
 }
 
 // G[a][b] = v_a^H v_b for the panel's reflectors (rows j0 .. m-1), one workgroup per entry
@@ -239,7 +391,7 @@ size_t qr_fact_bytes(int64_t m, int64_t n, int64_t ncmax) {
   const int64_t k = m < n ? m : n;
   const int64_t npan = (k + QR_NB - 1) / QR_NB;
   const int64_t nc = ncmax > n ? ncmax : n;
-  return al((size_t)m * n * 16) + al((size_t)m * k * 16) + 2 * al((size_t)(k + 1) * 16) +
+  return al((size_t)m * n * 16) + al((size_t)m * k * 16) + 3 * al((size_t)(k + 1) * 16) +
          al((size_t)(npan + 1) * QR_NB * QR_NB * 16) + al((size_t)QR_NB * QR_NB * 16) +
          al((size_t)(QR_SPLITK + 1) * QR_NB * nc * 16) + al((size_t)QR_NB * nc * 16);
 }
@@ -254,6 +406,7 @@ QrFact qr_fact_layout(int64_t m, int64_t n, int64_t ncmax, void* ws) {
   f.Vf = (double2*)p; p += al((size_t)m * f.k * 16);
   f.tau = (double2*)p; p += al((size_t)(f.k + 1) * 16);
   f.rdiag = (double2*)p; p += al((size_t)(f.k + 1) * 16);
+  f.rsup = (double2*)p; p += al((size_t)(f.k + 1) * 16);
   f.T = (double2*)p; p += al((size_t)(f.npan + 1) * QR_NB * QR_NB * 16);
   f.G = (double2*)p; p += al((size_t)QR_NB * QR_NB * 16);
   f.P = (double2*)p; p += al((size_t)(QR_SPLITK + 1) * QR_NB * f.ncmax * 16);
@@ -301,8 +454,11 @@ int qr_factor(QrFact& f, const double2* a, int64_t lda, hipStream_t s) {
     const int64_t j0 = p * QR_NB;
     const int64_t pe = (j0 + QR_NB < k) ? j0 + QR_NB : k;
     const int nbp = (int)(pe - j0);
-    for (int64_t j = j0; j < pe; ++j)
-      k_qr_col<<<(unsigned)(pe - j), 256, 0, s>>>(m, j, f.W, f.Vf, f.tau, f.rdiag);
+    int64_t j = j0;
+    for (; j + 1 < pe; j += 2)
+      k_qr_col2<<<(unsigned)(pe - j), 256, 0, s>>>(m, j, f.W, f.Vf, f.tau, f.rdiag, f.rsup);
+    if (j < pe) k_qr_col<<<(unsigned)(pe - j), 256, 0, s>>>(m, j, f.W, f.Vf, f.tau, f.rdiag);
+    if (pe - j0 >= 2) k_qr_fixsup<<<1, QR_NB / 2, 0, s>>>(m, j0, pe, f.rsup, f.W);
     PTN_LAUNCH_CHECK();
     double2* Tp = f.T + p * QR_NB * QR_NB;
     k_qr_vtv<<<(unsigned)(nbp * nbp), 256, 0, s>>>(m, j0, nbp, f.Vf, f.G);
