@@ -46,6 +46,14 @@ extern "C" {
 #define PTN_FP64    0
 #define PTN_FP32    1
 #define PTN_BF16X3  2
+/* flag OR-ed into a precision code: the operands are block sparse (U(1)
+ * quantum numbers, the reference's dense arrays with zero blocks;
+ * pytenet/block_sparse_util.py:56-64). The contractions then build per-tile
+ * nonzero masks of their operands and skip every tile and k-stage whose
+ * product is identically zero; results equal the dense products. The gain
+ * needs bonds whose quantum numbers are grouped into contiguous sectors (as the
+ * bond-splitting SVD produces them). */
+#define PTN_BLOCK_SPARSE 0x100
 
 /* local operator kinds for the Krylov solvers */
 #define PTN_OP_HEFF 0   /* L.W.A.R (one or two sites; for two sites pass d = d0*d1 and W = merged pair) */
@@ -104,7 +112,7 @@ int ptn_zgemm(int64_t M, int64_t N, int64_t K,
  * reference's matvec closure (pytenet/tdvp.py:223-238, pytenet/dmrg.py:181-189). */
 typedef struct {
   int kind;                 /* PTN_OP_HEFF or PTN_OP_BOND */
-  int precision;            /* PTN_FP64 / PTN_FP32 / PTN_BF16X3 */
+  int precision;            /* PTN_FP64 / PTN_FP32 / PTN_BF16X3, optionally | PTN_BLOCK_SPARSE */
   int64_t Dl, d, Dr, Ml, Mr;/* bond operator: d = 1, Ml = Mr = M */
   const void* l;            /* L (Dl, Ml, Dl) */
   const void* w;            /* W (Ml, d, d, Mr); NULL for the bond operator */

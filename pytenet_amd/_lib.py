@@ -19,6 +19,7 @@ PTN_ERR_UNSUPPORTED = 4
 PTN_FP64 = 0
 PTN_FP32 = 1
 PTN_BF16X3 = 2
+PTN_BLOCK_SPARSE = 0x100
 
 PTN_OP_HEFF = 0
 PTN_OP_BOND = 1

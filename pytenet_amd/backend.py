@@ -32,10 +32,12 @@ def _check(status, what):
         raise PtnError(f"{what} failed ({status}: {msg})")
 
 
-def precision_code(precision):
+def precision_code(precision, block_sparse=False):
+    """Precision code of the C ABI; `block_sparse` adds the flag that makes the
+    contractions skip the identically-zero tiles of U(1) block-sparse operands."""
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}")
-    return PRECISIONS[precision]
+    return PRECISIONS[precision] | (_lib.PTN_BLOCK_SPARSE if block_sparse else 0)
 
 
 def device():
@@ -102,14 +104,14 @@ def to_host(x):
 # ---------------------------------------------------------------------------
 # local contractions (K1-K4)
 
-def heff_apply(a, w, l, r, precision="fp64"):
+def heff_apply(a, w, l, r, precision="fp64", block_sparse=False):
     """out[lb,sb,rb] = sum L[l,m,lb] W[m,sb,s,m'] A[l,s,r] R[r,m',rb]."""
     a, w, l, r = as_dev(a), as_dev(w), as_dev(l), as_dev(r)
     Dl, d, Dr = a.shape
     Ml, Mr = w.shape[0], w.shape[3]
     _shape_check(w.shape == (Ml, d, d, Mr) and l.shape == (Dl, Ml, Dl) and r.shape == (Dr, Mr, Dr),
                  "apply_local_hamiltonian", a, w, l, r)
-    prec = precision_code(precision)
+    prec = precision_code(precision, block_sparse)
     lib = _lib.load()
     out = torch.empty((Dl, d, Dr), dtype=torch.complex128, device=a.device)
     nb = lib.ptn_heff_workspace(prec, Dl, d, Dr, Ml, Mr)
@@ -119,14 +121,14 @@ def heff_apply(a, w, l, r, precision="fp64"):
     return out
 
 
-def bond_apply(c, l, r, precision="fp64"):
+def bond_apply(c, l, r, precision="fp64", block_sparse=False):
     """out[lb,rb] = sum L[l,m,lb] C[l,r] R[r,m,rb]."""
     c, l, r = as_dev(c), as_dev(l), as_dev(r)
     Dl, Dr = c.shape
     M = l.shape[1]
     _shape_check(l.shape == (Dl, M, Dl) and r.shape == (Dr, M, Dr),
                  "apply_local_bond_contraction", c, l, r)
-    prec = precision_code(precision)
+    prec = precision_code(precision, block_sparse)
     lib = _lib.load()
     out = torch.empty((Dl, Dr), dtype=torch.complex128, device=c.device)
     ws = workspace(lib.ptn_bond_workspace(prec, Dl, Dr, M))
@@ -135,14 +137,14 @@ def bond_apply(c, l, r, precision="fp64"):
     return out
 
 
-def env_left(a, b, w, l, precision="fp64"):
+def env_left(a, b, w, l, precision="fp64", block_sparse=False):
     """out[r,m',rb] = sum A[l,s,r] W[m,sb,s,m'] L[l,m,lb] conj(B[lb,sb,rb])."""
     a, b, w, l = as_dev(a), as_dev(b), as_dev(w), as_dev(l)
     Dl, d, Dr = a.shape
     Ml, Mr = w.shape[0], w.shape[3]
     _shape_check(b.shape == a.shape and w.shape == (Ml, d, d, Mr) and l.shape == (Dl, Ml, Dl),
                  "contraction_operator_step_left", a, b, w, l)
-    prec = precision_code(precision)
+    prec = precision_code(precision, block_sparse)
     lib = _lib.load()
     out = torch.empty((Dr, Mr, Dr), dtype=torch.complex128, device=a.device)
     ws = workspace(lib.ptn_env_workspace(prec, Dl, d, Dr, Ml, Mr))
@@ -151,14 +153,14 @@ def env_left(a, b, w, l, precision="fp64"):
     return out
 
 
-def env_right(a, b, w, r, precision="fp64"):
+def env_right(a, b, w, r, precision="fp64", block_sparse=False):
     """out[l,m,lb] = sum A[l,s,r] W[m,sb,s,m'] R[r,m',rb] conj(B[lb,sb,rb])."""
     a, b, w, r = as_dev(a), as_dev(b), as_dev(w), as_dev(r)
     Dl, d, Dr = a.shape
     Ml, Mr = w.shape[0], w.shape[3]
     _shape_check(b.shape == a.shape and w.shape == (Ml, d, d, Mr) and r.shape == (Dr, Mr, Dr),
                  "contraction_operator_step_right", a, b, w, r)
-    prec = precision_code(precision)
+    prec = precision_code(precision, block_sparse)
     lib = _lib.load()
     out = torch.empty((Dl, Ml, Dl), dtype=torch.complex128, device=a.device)
     ws = workspace(lib.ptn_env_workspace(prec, Dl, d, Dr, Ml, Mr))
@@ -205,10 +207,10 @@ def merge_pair(a0, a1):
 # ---------------------------------------------------------------------------
 # Krylov solvers (K8-K10)
 
-def _local_op(kind, precision, l, w, r, Dl, d, Dr):
+def _local_op(kind, precision, l, w, r, Dl, d, Dr, block_sparse=False):
     op = _lib.LocalOp()
     op.kind = kind
-    op.precision = precision_code(precision)
+    op.precision = precision_code(precision, block_sparse)
     op.Dl, op.d, op.Dr = Dl, d, Dr
     op.Ml = l.shape[1]
     op.Mr = r.shape[1]
@@ -218,7 +220,8 @@ def _local_op(kind, precision, l, w, r, Dl, d, Dr):
     return op
 
 
-def krylov_local(kind, v, l, w, r, numiter, mode, dt=0.0, precision="fp64", info=None):
+def krylov_local(kind, v, l, w, r, numiter, mode, dt=0.0, precision="fp64", info=None,
+                 block_sparse=False):
     """Lanczos on the local operator (PTN_OP_HEFF or PTN_OP_BOND):
     mode "expm" -> expm(dt*H) v; mode "eigh" -> (lowest Ritz vector, energy tensor).
     `info` (int32[4] device tensor) receives the status words; returns
@@ -230,7 +233,7 @@ def krylov_local(kind, v, l, w, r, numiter, mode, dt=0.0, precision="fp64", info
         Dl, d, Dr = v.shape
     else:
         (Dl, Dr), d = v.shape, 1
-    op = _local_op(kind, precision, l, w, r, Dl, d, Dr)
+    op = _local_op(kind, precision, l, w, r, Dl, d, Dr, block_sparse)
     lib = _lib.load()
     nb = lib.ptn_krylov_workspace(ctypes.byref(op), int(numiter))
     ws = workspace(nb)

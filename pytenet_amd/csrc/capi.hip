@@ -67,6 +67,7 @@ __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 using namespace ptn;
 
 static inline bool precision_ok(int p) {
+  p &= ~PTN_BLOCK_SPARSE;
   return p == PTN_FP64 || p == PTN_FP32 || p == PTN_BF16X3;
 }
 

@@ -121,12 +121,15 @@ __global__ __launch_bounds__(256) void zgemm_bf16x3_kernel(GemmKArgs g) {
   const int pb = lane & 1;            // real column parity: 0 -> Re C, 1 -> Im C
 
   const int nk = (int)((g.K + BK - 1) / BK);
-  load_tiles(0);
-  for (int t = 0; t < nk; ++t) {
+  int t = next_stage(g, tm, tn, 0, nk);
+  if (t < nk) load_tiles((int64_t)t * BK);
+  while (t < nk) {
     __syncthreads();
     store_tiles();
     __syncthreads();
-    if (t + 1 < nk) load_tiles((int64_t)(t + 1) * BK);
+    const int tnext = next_stage(g, tm, tn, t + 1, nk);
+    if (tnext < nk) load_tiles((int64_t)tnext * BK);
+    t = tnext;
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
       const int kof = ks * 32 + 8 * grp;   // real k offset within the stage
@@ -199,7 +202,7 @@ __global__ __launch_bounds__(256) void zgemm_bf16x3_kernel(GemmKArgs g) {
 }
 
 template <typename TA, typename TB, typename TC, int TM, int TN>
-inline int launch_cfg_bf16(const ZGemm& z, hipStream_t s) {
+inline int launch_cfg_bf16(const ZGemm& z, hipStream_t s, void* mws, size_t mws_bytes) {
   typedef TileCfgB<TM, TN> T;
   GemmKArgs g;
   g.M = z.M; g.N = z.N; g.K = z.K;
@@ -212,6 +215,8 @@ inline int launch_cfg_bf16(const ZGemm& z, hipStream_t s) {
   g.tiles_n = (int)((z.N + T::BN - 1) / T::BN);
   int64_t nwg = (int64_t)g.tiles_m * g.tiles_n;
   if (nwg > 0x7fffffff || z.batch > 65535) return PTN_ERR_ARG;
+  int st = build_masks<TA, TB>(z, g, T::BM, T::BN, T::BK, mws, mws_bytes, s);
+  if (st) return st;
   dim3 grid((unsigned)nwg, 1, (unsigned)z.batch);
   const bool ak = (z.a_k == 1) || (z.a_m != 1);
   const bool bn = (z.b_n == 1) || (z.b_k != 1);
@@ -228,20 +233,21 @@ inline int launch_cfg_bf16(const ZGemm& z, hipStream_t s) {
 }
 
 template <typename TA, typename TB, typename TC>
-inline int launch_typed_bf16(const ZGemm& z, hipStream_t s) {
+inline int launch_typed_bf16(const ZGemm& z, hipStream_t s, void* mws, size_t mws_bytes) {
   int64_t big = ((z.M + 127) / 128) * ((z.N + 63) / 64) * z.batch;
-  if (big >= 512) return launch_cfg_bf16<TA, TB, TC, 4, 4>(z, s);
-  return launch_cfg_bf16<TA, TB, TC, 2, 2>(z, s);
+  if (big >= 512) return launch_cfg_bf16<TA, TB, TC, 4, 4>(z, s, mws, mws_bytes);
+  return launch_cfg_bf16<TA, TB, TC, 2, 2>(z, s, mws, mws_bytes);
 }
 
-int zgemm_bf16x3(const ZGemm& z, int ta, int tb, int tc, hipStream_t s) {
+int zgemm_bf16x3(const ZGemm& z, int ta, int tb, int tc, hipStream_t s, void* mws,
+                 size_t mws_bytes) {
   const int key = (ta << 2) | (tb << 1) | tc;
   switch (key) {
-    case 1: return launch_typed_bf16<double2, double2, float2>(z, s);
-    case 2: return launch_typed_bf16<double2, float2, double2>(z, s);
-    case 3: return launch_typed_bf16<double2, float2, float2>(z, s);
-    case 4: return launch_typed_bf16<float2, double2, double2>(z, s);
-    case 6: return launch_typed_bf16<float2, float2, double2>(z, s);
+    case 1: return launch_typed_bf16<double2, double2, float2>(z, s, mws, mws_bytes);
+    case 2: return launch_typed_bf16<double2, float2, double2>(z, s, mws, mws_bytes);
+    case 3: return launch_typed_bf16<double2, float2, float2>(z, s, mws, mws_bytes);
+    case 4: return launch_typed_bf16<float2, double2, double2>(z, s, mws, mws_bytes);
+    case 6: return launch_typed_bf16<float2, float2, double2>(z, s, mws, mws_bytes);
   }
   return PTN_ERR_UNSUPPORTED;
 }

@@ -52,7 +52,72 @@ struct GemmKArgs {
   int64_t bz_a, bz_b, bz_c;
   int tiles_n;          // number of column tiles
   int tiles_m;
+  // optional block-sparsity masks (batch 1 only): bit t of word w of row tile
+  // tm (column tile tn) is set when the A rows of tm (B columns of tn) have a
+  // nonzero entry in k-stage 64 w + t; stages where the two masks do not
+  // overlap contribute exactly zero and are skipped
+  const uint64_t* maskA;
+  const uint64_t* maskB;
+  int mwords;
 };
+
+// first k-stage >= from whose A and B masks overlap (every stage when unmasked)
+__device__ __forceinline__ int next_stage(const GemmKArgs& g, int tm, int tn, int from, int nk) {
+  if (!g.maskA) return from;
+  for (int w = from >> 6; w < g.mwords; ++w) {
+    uint64_t bits = g.maskA[(int64_t)tm * g.mwords + w] & g.maskB[(int64_t)tn * g.mwords + w];
+    if (w == (from >> 6)) bits &= ~0ull << (from & 63);
+    if (bits) {
+      const int t = (w << 6) + __builtin_ctzll(bits);
+      return t < nk ? t : nk;
+    }
+  }
+  return nk;
+}
+
+// Tile masks of a GEMM operand. The operand has `ntile_items` items (A rows or
+// B columns) split into tiles of `tile`, and K split into k-stages of `bk`;
+// element (item, k) is at p[item * s_item + k * s_k] (+ batch 0 only).
+// One workgroup per (tile, 64-stage word); threads walk the block along its
+// contiguous dimension and OR the stage bits of nonzero entries in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void tile_mask_kernel(int64_t nitems, int64_t K, const T* __restrict__ p,
+                                                        int64_t s_item, int64_t s_k, int tile, int bk,
+                                                        int mwords, uint64_t* __restrict__ mask) {
+  __shared__ unsigned int bits[2];
+  const int tix = blockIdx.x, w = blockIdx.y;
+  if (threadIdx.x < 2) bits[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t i0 = (int64_t)tix * tile;
+  const int64_t k0 = (int64_t)w * 64 * bk;
+  const int64_t ni = (i0 + tile < nitems ? i0 + tile : nitems) - i0;
+  const int64_t nkk = (k0 + 64 * bk < K ? k0 + 64 * bk : K) - k0;
+  const bool kfast = (s_k == 1) || (s_item != 1);
+  unsigned int lo = 0, hi = 0;
+  if (ni > 0 && nkk > 0) {
+    for (int64_t e = threadIdx.x; e < ni * nkk; e += 256) {
+      const int64_t it = kfast ? e / nkk : e % ni;
+      const int64_t kk = kfast ? e % nkk : e / ni;
+      const T v = p[(i0 + it) * s_item + (k0 + kk) * s_k];
+      if (v.x != 0 || v.y != 0) {
+        const int st = (int)(kk / bk);
+        if (st < 32) lo |= 1u << st; else hi |= 1u << (st - 32);
+      }
+    }
+  }
+  if (lo) atomicOr(&bits[0], lo);
+  if (hi) atomicOr(&bits[1], hi);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    mask[(int64_t)tix * mwords + w] = ((uint64_t)bits[1] << 32) | bits[0];
+}
+
+// bytes of mask workspace for an M x N x K product at any tile choice
+inline size_t gemm_mask_bytes(int64_t M, int64_t N, int64_t K) {
+  const int64_t words = ((K + 7) / 8 + 63) / 64;
+  const int64_t tiles = (M + 63) / 64 + (N + 31) / 32;
+  return (size_t)(tiles * words * 8 + 255) & ~(size_t)255;
+}
 
 // map the linear workgroup id so that the workgroups that run on one XCD
 // (hardware assigns id % 8 to XCD id % 8) walk a contiguous range of tiles,
@@ -154,12 +219,15 @@ __global__ __launch_bounds__(256) void zgemm_kernel(GemmKArgs g) {
   const Tc im_sign = (pa == 1) ? (Tc)-1 : (Tc)1;
 
   const int nk = (int)((g.K + BK - 1) / BK);
-  load_tiles(0);
-  for (int t = 0; t < nk; ++t) {
+  int t = next_stage(g, tm, tn, 0, nk);
+  if (t < nk) load_tiles((int64_t)t * BK);
+  while (t < nk) {
     __syncthreads();
     store_tiles();
     __syncthreads();
-    if (t + 1 < nk) load_tiles((int64_t)(t + 1) * BK);
+    const int tnext = next_stage(g, tm, tn, t + 1, nk);
+    if (tnext < nk) load_tiles((int64_t)tnext * BK);
+    t = tnext;
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       Tc av[TM], bv[TN];
@@ -214,8 +282,30 @@ __global__ __launch_bounds__(256) void zgemm_kernel(GemmKArgs g) {
   }
 }
 
+// masks for the chosen tile shape (BM rows of A, BN columns of B, BK-wide
+// k-stages) in `mws` (gemm_mask_bytes bytes), batch 1 only
+template <typename TA, typename TB>
+inline int build_masks(const ZGemm& z, GemmKArgs& g, int BM, int BN, int BK, void* mws,
+                       size_t mws_bytes, hipStream_t s) {
+  g.maskA = nullptr; g.maskB = nullptr; g.mwords = 0;
+  if (!mws || z.batch != 1 || z.K == 0) return PTN_OK;
+  const int64_t nk = (z.K + BK - 1) / BK;
+  const int words = (int)((nk + 63) / 64);
+  const size_t need = (size_t)((int64_t)g.tiles_m + g.tiles_n) * words * 8;
+  if (mws_bytes < need) return PTN_ERR_WORKSPACE;
+  uint64_t* ma = (uint64_t*)mws;
+  uint64_t* mb = ma + (int64_t)g.tiles_m * words;
+  tile_mask_kernel<TA><<<dim3((unsigned)g.tiles_m, (unsigned)words), 256, 0, s>>>(
+      z.M, z.K, (const TA*)z.A, z.a_m, z.a_k, BM, BK, words, ma);
+  tile_mask_kernel<TB><<<dim3((unsigned)g.tiles_n, (unsigned)words), 256, 0, s>>>(
+      z.N, z.K, (const TB*)z.B, z.b_n, z.b_k, BN, BK, words, mb);
+  PTN_LAUNCH_CHECK();
+  g.maskA = ma; g.maskB = mb; g.mwords = words;
+  return PTN_OK;
+}
+
 template <typename Tc, typename TA, typename TB, typename TC, int TM, int TN>
-inline int launch_cfg(const ZGemm& z, hipStream_t s) {
+inline int launch_cfg(const ZGemm& z, hipStream_t s, void* mws = nullptr, size_t mws_bytes = 0) {
   typedef TileCfg<TM, TN> T;
   GemmKArgs g;
   g.M = z.M; g.N = z.N; g.K = z.K;
@@ -228,6 +318,8 @@ inline int launch_cfg(const ZGemm& z, hipStream_t s) {
   g.tiles_n = (int)((z.N + T::BN - 1) / T::BN);
   int64_t nwg = (int64_t)g.tiles_m * g.tiles_n;
   if (nwg > 0x7fffffff || z.batch > 65535) return PTN_ERR_ARG;
+  int st = build_masks<TA, TB>(z, g, T::BM, T::BN, T::BK, mws, mws_bytes, s);
+  if (st) return st;
   dim3 grid((unsigned)nwg, 1, (unsigned)z.batch);
   // a K-contiguous A when its k stride is 1 (or K == 1), else M-contiguous
   const bool ak = (z.a_k == 1) || (z.a_m != 1);
@@ -247,10 +339,10 @@ inline int launch_cfg(const ZGemm& z, hipStream_t s) {
 // tile choice: the large tile (128 x 64 complex) when it still gives at least
 // two workgroups per CU, else the small one (64 x 32)
 template <typename Tc, typename TA, typename TB, typename TC>
-inline int launch_typed(const ZGemm& z, hipStream_t s) {
+inline int launch_typed(const ZGemm& z, hipStream_t s, void* mws = nullptr, size_t mws_bytes = 0) {
   int64_t big = ((z.M + 127) / 128) * ((z.N + 63) / 64) * z.batch;
-  if (big >= 512) return launch_cfg<Tc, TA, TB, TC, 4, 4>(z, s);
-  return launch_cfg<Tc, TA, TB, TC, 2, 2>(z, s);
+  if (big >= 512) return launch_cfg<Tc, TA, TB, TC, 4, 4>(z, s, mws, mws_bytes);
+  return launch_cfg<Tc, TA, TB, TC, 2, 2>(z, s, mws, mws_bytes);
 }
 
 }  // namespace ptn

@@ -349,7 +349,7 @@ static KrylovLayout krylov_layout(const ptn_local_op* op, int numiter) {
   k.part_a = off; off += al(RED_BLOCKS * 8);
   k.part_b = off; off += al(RED_BLOCKS * 8);
   k.state = off; off += al(sizeof(LzState));
-  const bool lowp = op->precision != PTN_FP64;
+  const bool lowp = (op->precision & ~PTN_BLOCK_SPARSE) != PTN_FP64;
   const int64_t nl = op->Dl * op->Ml * op->Dl, nr = op->Dr * op->Mr * op->Dr;
   k.l64 = off; off += lowp ? al((size_t)nl * 8) : 0;
   k.r64 = off; off += lowp ? al((size_t)nr * 8) : 0;
@@ -373,8 +373,8 @@ int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, do
                void* ws, size_t ws_bytes, hipStream_t s) {
   if (!op || numiter < 1 || numiter > LZ_MAX) return PTN_ERR_ARG;
   if (op->kind != PTN_OP_HEFF && op->kind != PTN_OP_BOND) return PTN_ERR_ARG;
-  if (op->precision != PTN_FP64 && op->precision != PTN_FP32 && op->precision != PTN_BF16X3)
-    return PTN_ERR_UNSUPPORTED;
+  const int bprec = op->precision & ~PTN_BLOCK_SPARSE;
+  if (bprec != PTN_FP64 && bprec != PTN_FP32 && bprec != PTN_BF16X3) return PTN_ERR_UNSUPPORTED;
   if (op->Dl <= 0 || op->Dr <= 0 || op->Ml <= 0 || op->Mr <= 0) return PTN_ERR_ARG;
   if (op->kind == PTN_OP_HEFF && (op->d <= 0 || !op->w)) return PTN_ERR_ARG;
   if (op->kind == PTN_OP_BOND && op->Ml != op->Mr) return PTN_ERR_ARG;
@@ -396,7 +396,7 @@ int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, do
   const void* R = op->r;
   int tl = PTN_T_C128, tr = PTN_T_C128;
   int st_;
-  if (op->precision != PTN_FP64) {
+  if (bprec != PTN_FP64) {
     const int64_t nl = op->Dl * op->Ml * op->Dl, nr = op->Dr * op->Mr * op->Dr;
     if ((st_ = convert_c128_c64(nl, op->l, base + K.l64, s))) return st_;
     if ((st_ = convert_c128_c64(nr, op->r, base + K.r64, s))) return st_;

@@ -7,11 +7,20 @@
 // MPO tensor for every (left bond, right bond) pair; it is HBM-bound and
 // keeps the two products dense and MFMA-shaped.
 #include "common.h"
+#include "gemm_impl.h"
 #include "ops.h"
 
 namespace ptn {
 
 static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+// precision code with the block-sparse flag removed, and the flag itself
+static inline int base_prec(int prec) { return prec & ~PTN_BLOCK_SPARSE; }
+static inline bool is_sparse(int prec) { return (prec & PTN_BLOCK_SPARSE) != 0; }
+static inline size_t mask_bytes2(int64_t m1, int64_t n1, int64_t k1, int64_t m3, int64_t n3,
+                                 int64_t k3) {
+  size_t a = gemm_mask_bytes(m1, n1, k1), b = gemm_mask_bytes(m3, n3, k3);
+  return a > b ? a : b;
+}
 static inline size_t csize(int prec) { return prec == PTN_FP64 ? 16 : 8; }
 static inline int ctype(int prec) { return prec == PTN_FP64 ? PTN_T_C128 : PTN_T_C64; }
 static inline int gemm_compute(int prec) { return prec; }
@@ -45,22 +54,29 @@ int convert_c128_c64(int64_t n, const void* x, void* y, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 size_t heff_ws_bytes(int prec, const Dims& D) {
+  prec = base_prec(prec);
   size_t t1 = (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
   size_t t2 = (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
-  return align256(t1) + align256(t2) + 256;
+  return align256(t1) + align256(t2) +
+         mask_bytes2(D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl, D.d * D.Dr, D.Dl * D.Ml) + 256;
 }
 
 size_t env_ws_bytes(int prec, const Dims& D) {
+  prec = base_prec(prec);
   // right step: T1 (Dl*d, Mr*Dr), T2 (Dl*Ml, d*Dr); left step: T1 (Dl*Ml, d*Dr),
   // T2 (Dl*d, Mr*Dr); plus the transposed MPO tensor
   size_t a = (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
   size_t b = (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
   size_t wt = (size_t)(D.Ml * D.d * D.d * D.Mr) * 16;
-  return 2 * align256(a > b ? a : b) + align256(wt) + 256;
+  const size_t mr = mask_bytes2(D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl * D.Ml, D.Dl, D.d * D.Dr);
+  const size_t ml = mask_bytes2(D.Dl * D.Ml, D.d * D.Dr, D.Dl, D.Dr, D.Mr * D.Dr, D.Dl * D.d);
+  return 2 * align256(a > b ? a : b) + align256(wt) + (mr > ml ? mr : ml) + 256;
 }
 
 size_t bond_ws_bytes(int prec, int64_t Dl, int64_t Dr, int64_t M) {
-  return align256((size_t)Dl * (size_t)(M * Dr) * csize(prec)) + 256;
+  prec = base_prec(prec);
+  return align256((size_t)Dl * (size_t)(M * Dr) * csize(prec)) +
+         mask_bytes2(Dl, M * Dr, Dr, Dl, Dr, Dl * M) + 256;
 }
 
 static bool dims_ok(const Dims& D) {
@@ -75,29 +91,36 @@ int heff_apply(int prec, const Dims& D, const void* l, int tl, const void* w, co
                hipStream_t s) {
   if (!dims_ok(D)) return PTN_ERR_ARG;
   if (ws_bytes < heff_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;
+  const bool sp = is_sparse(prec);
+  prec = base_prec(prec);
   const int ct = ctype(prec), cp = gemm_compute(prec);
   char* p = (char*)ws;
   void* t1 = p;
   p += align256((size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec));
   void* t2 = p;
+  p += align256((size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec));
+  const size_t mb = mask_bytes2(D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl, D.d * D.Dr, D.Dl * D.Ml);
+  void* mws = sp ? p : nullptr;
   int st;
   ZGemm g1 = {D.Dl * D.d, D.Mr * D.Dr, D.Dr,
               a, D.Dr, 1, 0,
               r, D.Mr * D.Dr, 1, 0,
               t1, D.Mr * D.Dr, 1, 0, 1, 0, 0, 0};
-  if ((st = zgemm(g1, cp, ta, tr, ct, nullptr, 0, s))) return st;
+  if ((st = zgemm(g1, cp, ta, tr, ct, mws, mb, s))) return st;
   if ((st = mode_mix(D.Dl, D.d * D.Mr, D.Ml * D.d, D.Dr, w, t1, ct, t2, ct, s))) return st;
   ZGemm g3 = {D.Dl, D.d * D.Dr, D.Dl * D.Ml,
               l, 1, D.Dl, 0,
               t2, D.d * D.Dr, 1, 0,
               out, D.d * D.Dr, 1, 0, 1, 0, 0, 0};
-  return zgemm(g3, cp, tl, ct, tout, nullptr, 0, s);
+  return zgemm(g3, cp, tl, ct, tout, mws, mb, s);
 }
 
 int env_right(int prec, const Dims& D, const void* a, const void* b, const void* w, const void* r,
               int tr, void* out, void* ws, size_t ws_bytes, hipStream_t s) {
   if (!dims_ok(D)) return PTN_ERR_ARG;
   if (ws_bytes < env_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;
+  const bool sp = is_sparse(prec);
+  prec = base_prec(prec);
   const int ct = ctype(prec), cp = gemm_compute(prec);
   size_t ta = (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
   size_t tb = (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
@@ -105,25 +128,30 @@ int env_right(int prec, const Dims& D, const void* a, const void* b, const void*
   void* t1 = p;
   p += align256(ta > tb ? ta : tb);
   void* t2 = p;
+  const size_t wtb = align256((size_t)(D.Ml * D.d * D.d * D.Mr) * 16);
+  const size_t mb = mask_bytes2(D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl * D.Ml, D.Dl, D.d * D.Dr);
+  void* mws = sp ? (char*)ws + 2 * align256(ta > tb ? ta : tb) + wtb : nullptr;
   int st;
   ZGemm g1 = {D.Dl * D.d, D.Mr * D.Dr, D.Dr,
               a, D.Dr, 1, 0,
               r, D.Mr * D.Dr, 1, 0,
               t1, D.Mr * D.Dr, 1, 0, 1, 0, 0, 0};
-  if ((st = zgemm(g1, cp, PTN_T_C128, tr, ct, nullptr, 0, s))) return st;
+  if ((st = zgemm(g1, cp, PTN_T_C128, tr, ct, mws, mb, s))) return st;
   if ((st = mode_mix(D.Dl, D.d * D.Mr, D.Ml * D.d, D.Dr, w, t1, ct, t2, ct, s))) return st;
   // out[(l, m), lb] = sum_{(sb, rb)} T2[(l, m), (sb, rb)] conj(B[lb, (sb, rb)])
   ZGemm g3 = {D.Dl * D.Ml, D.Dl, D.d * D.Dr,
               t2, D.d * D.Dr, 1, 0,
               b, 1, D.d * D.Dr, 1,
               out, D.Dl, 1, 0, 1, 0, 0, 0};
-  return zgemm(g3, cp, ct, PTN_T_C128, PTN_T_C128, nullptr, 0, s);
+  return zgemm(g3, cp, ct, PTN_T_C128, PTN_T_C128, mws, mb, s);
 }
 
 int env_left(int prec, const Dims& D, const void* a, const void* b, const void* w, const void* l,
              int tl, void* out, void* ws, size_t ws_bytes, hipStream_t s) {
   if (!dims_ok(D)) return PTN_ERR_ARG;
   if (ws_bytes < env_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;
+  const bool sp = is_sparse(prec);
+  prec = base_prec(prec);
   const int ct = ctype(prec), cp = gemm_compute(prec);
   size_t ta = (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
   size_t tb = (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
@@ -132,6 +160,9 @@ int env_left(int prec, const Dims& D, const void* a, const void* b, const void* 
   void* t1 = p;
   void* t2 = p + big;
   void* wt = p + 2 * big;
+  const size_t wtb = align256((size_t)(D.Ml * D.d * D.d * D.Mr) * 16);
+  const size_t mb = mask_bytes2(D.Dl * D.Ml, D.d * D.Dr, D.Dl, D.Dr, D.Mr * D.Dr, D.Dl * D.d);
+  void* mws = sp ? p + 2 * big + wtb : nullptr;
   int64_t rows = D.Ml * D.d, cols = D.d * D.Mr;
   transpose_w_kernel<<<(unsigned)((rows * cols + 255) / 256), 256, 0, s>>>(
       rows, cols, (const double2*)w, (double2*)wt);
@@ -142,7 +173,7 @@ int env_left(int prec, const Dims& D, const void* a, const void* b, const void* 
               l, D.Dl, 1, 0,
               b, D.d * D.Dr, 1, 1,
               t1, D.d * D.Dr, 1, 0, 1, 0, 0, 0};
-  if ((st = zgemm(g1, cp, tl, PTN_T_C128, ct, nullptr, 0, s))) return st;
+  if ((st = zgemm(g1, cp, tl, PTN_T_C128, ct, mws, mb, s))) return st;
   // T2[l, (s, m'), rb] = sum_{(m, sb)} W[m, sb, s, m'] T1[l, (m, sb), rb]
   if ((st = mode_mix(D.Dl, D.Ml * D.d, D.d * D.Mr, D.Dr, wt, t1, ct, t2, ct, s))) return st;
   // out[r, (m', rb)] = sum_{(l, s)} A[(l, s), r] T2[(l, s), (m', rb)]
@@ -150,7 +181,7 @@ int env_left(int prec, const Dims& D, const void* a, const void* b, const void* 
               a, 1, D.Dr, 0,
               t2, D.Mr * D.Dr, 1, 0,
               out, D.Mr * D.Dr, 1, 0, 1, 0, 0, 0};
-  return zgemm(g3, cp, PTN_T_C128, ct, PTN_T_C128, nullptr, 0, s);
+  return zgemm(g3, cp, PTN_T_C128, ct, PTN_T_C128, mws, mb, s);
 }
 
 int bond_apply(int prec, int64_t Dl, int64_t Dr, int64_t M, const void* l, int tl, const void* r,
@@ -158,21 +189,25 @@ int bond_apply(int prec, int64_t Dl, int64_t Dr, int64_t M, const void* l, int t
                hipStream_t s) {
   if (Dl <= 0 || Dr <= 0 || M <= 0) return PTN_ERR_ARG;
   if (ws_bytes < bond_ws_bytes(prec, Dl, Dr, M)) return PTN_ERR_WORKSPACE;
+  const bool sp = is_sparse(prec);
+  prec = base_prec(prec);
   const int ct = ctype(prec), cp = gemm_compute(prec);
   void* t = ws;
+  const size_t mb = mask_bytes2(Dl, M * Dr, Dr, Dl, Dr, Dl * M);
+  void* mws = sp ? (char*)ws + align256((size_t)Dl * (size_t)(M * Dr) * csize(prec)) : nullptr;
   int st;
   // T[l, (m, rb)] = sum_r C[l, r] R[r, (m, rb)]
   ZGemm g1 = {Dl, M * Dr, Dr,
               c, Dr, 1, 0,
               r, M * Dr, 1, 0,
               t, M * Dr, 1, 0, 1, 0, 0, 0};
-  if ((st = zgemm(g1, cp, tcin, tr, ct, nullptr, 0, s))) return st;
+  if ((st = zgemm(g1, cp, tcin, tr, ct, mws, mb, s))) return st;
   // out[lb, rb] = sum_{(l, m)} L[(l, m), lb] T[(l, m), rb]
   ZGemm g3 = {Dl, Dr, Dl * M,
               l, 1, Dl, 0,
               t, Dr, 1, 0,
               out, Dr, 1, 0, 1, 0, 0, 0};
-  return zgemm(g3, cp, tl, ct, tout, nullptr, 0, s);
+  return zgemm(g3, cp, tl, ct, tout, mws, mb, s);
 }
 
 }  // namespace ptn

@@ -26,6 +26,11 @@ class SweepContext:
         self.host_input = not psi.on_device
         psi.to_device()
         self._status = []
+        # nontrivial quantum numbers make every tensor of the sweep block sparse
+        # (the reference keeps them as dense arrays with zero blocks): let the
+        # contractions skip the zero tiles
+        self.block_sparse = bool(np.any(np.asarray(psi.qsite) != 0) or
+                                 any(np.any(np.asarray(q) != 0) for q in psi.qbonds))
 
     def one(self):
         return torch.ones((1, 1, 1), dtype=torch.complex128, device=self.dev)
@@ -33,14 +38,15 @@ class SweepContext:
     def site_expm(self, a, l, w, r, dt):
         """expm(dt * H_eff) a (the reference passes -dt to expm_krylov, tdvp.py:227)."""
         out, _, info = backend.krylov_local(_lib.PTN_OP_HEFF, a, l, w, r, self.numiter, "expm",
-                                            dt, self.precision)
+                                            dt, self.precision, block_sparse=self.block_sparse)
         self._status.append(info)
         return out
 
     def bond_expm(self, c, l, r, dt):
         """expm(dt * H_bond) c (tdvp.py:232-238)."""
         out, _, info = backend.krylov_local(_lib.PTN_OP_BOND, c, l, None, r, self.numiter,
-                                            "expm", dt, self.precision)
+                                            "expm", dt, self.precision,
+                                            block_sparse=self.block_sparse)
         self._status.append(info)
         return out
 
@@ -48,15 +54,15 @@ class SweepContext:
         """Lowest Ritz pair of H_eff started from a (dmrg.py:181-189);
         the energy stays on the device."""
         out, en, info = backend.krylov_local(_lib.PTN_OP_HEFF, a, l, w, r, self.numiter, "eigh",
-                                             0.0, self.precision)
+                                             0.0, self.precision, block_sparse=self.block_sparse)
         self._status.append(info)
         return en, out
 
     def env_left(self, a, w, l):
-        return backend.env_left(a, a, w, l, self.precision)
+        return backend.env_left(a, a, w, l, self.precision, self.block_sparse)
 
     def env_right(self, a, w, r):
-        return backend.env_right(a, a, w, r, self.precision)
+        return backend.env_right(a, a, w, r, self.precision, self.block_sparse)
 
     def flush(self):
         """Read the Krylov status words of the calls so far (one synchronisation)

@@ -13,6 +13,7 @@ import torch
 import oracle as orc
 from conftest import load_fixture
 from pytenet_amd import backend, _lib
+from pytenet_amd.hamiltonian import heisenberg_xxz_1d_mpo
 
 pytestmark = pytest.mark.gpu
 
@@ -255,3 +256,52 @@ def test_qsparse_kernels():
     ref[mask] = 0
     assert np.array_equal(backend.to_host(t), ref)
     assert backend.qsparse_violation(t, [q0, q1, q2]) == 0
+
+
+def _sorted_bond(rng, D, spread):
+    """Quantum numbers of a bond grouped into contiguous ascending sectors
+    (as the bond-splitting SVD produces them), drawn around zero."""
+    return np.sort(rng.binomial(2 * spread, 0.5, size=D) - spread)
+
+
+def _qsparse(rng, shape, qnums):
+    x = orc.crandn(shape, rng)
+    orc.enforce_qsparsity(x, qnums)
+    return x
+
+
+@pytest.mark.parametrize("Dl,Dr,two_site", [(200, 180, False), (260, 300, True), (37, 41, True)])
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL),
+                                           ("bf16x3", BF16X3_TOL)])
+def test_block_sparse_contractions_match_dense(Dl, Dr, two_site, precision, tol):
+    """U(1) block-sparse operands (XXZ MPO with quantum numbers, sector-sorted
+    bonds): the tile-skipping contractions equal the dense ones and the oracle."""
+    rng = np.random.default_rng(Dl + Dr)
+    H = heisenberg_xxz_1d_mpo(4, 1.0, 0.8, -0.1)
+    qd = np.asarray(H.qsite)
+    w = np.asarray(H.a[1], dtype=complex)
+    qm_l, qm_r = np.asarray(H.qbonds[1]), np.asarray(H.qbonds[2])
+    qs = qd
+    if two_site:
+        w = orc.mpo_merge_tensor_pair(w, np.asarray(H.a[2], dtype=complex))
+        qm_r = np.asarray(H.qbonds[3])
+        qs = orc.qnumber_flatten([qd, qd])
+    ql, qr = _sorted_bond(rng, Dl, 4), _sorted_bond(rng, Dr, 4)
+    d = len(qs)
+    a = _qsparse(rng, (Dl, d, Dr), [ql, qs, -qr])
+    l = _qsparse(rng, (Dl, len(qm_l), Dl), [ql, qm_l, -ql])
+    r = _qsparse(rng, (Dr, len(qm_r), Dr), [qr, qm_r, -qr])
+    assert np.count_nonzero(a) < a.size // 3       # genuinely sparse operands
+    dense = backend.to_host(backend.heff_apply(a, w, l, r, precision))
+    sparse = backend.to_host(backend.heff_apply(a, w, l, r, precision, block_sparse=True))
+    ref = orc.apply_local_hamiltonian(a, w, l, r)
+    assert relerr(sparse, ref) < tol
+    assert np.abs(sparse - dense).max() <= 1e-13 * np.abs(dense).max()
+    assert relerr(backend.env_left(a, a, w, l, precision, block_sparse=True),
+                  orc.contraction_operator_step_left(a, a, w, l)) < tol
+    assert relerr(backend.env_right(a, a, w, r, precision, block_sparse=True),
+                  orc.contraction_operator_step_right(a, a, w, r)) < tol
+    c = _qsparse(rng, (Dl, Dl), [ql, -ql])
+    rb = _qsparse(rng, (Dl, len(qm_l), Dl), [ql, qm_l, -ql])
+    assert relerr(backend.bond_apply(c, l, rb, precision, block_sparse=True),
+                  orc.apply_local_bond_contraction(c, l, rb)) < tol
