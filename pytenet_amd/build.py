@@ -17,8 +17,12 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libptn_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
+# -amdgpu-mfma-vgpr-form: MFMA accumulators in VGPRs. Without it the GEMM
+# main loops keep their loop-carried accumulators in VGPRs but run the MFMAs on
+# AGPR copies, i.e. 2 x 128 register moves per k-stage that drain the MFMA pipe.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-amdgpu-mfma-vgpr-form",
          "-Wno-unused-result", "-I" + os.path.join(HERE, "..", "include")]
 
 

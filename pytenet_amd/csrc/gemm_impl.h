@@ -33,15 +33,20 @@ template <> struct acc4<float> {
 };
 
 // Workgroup = 4 waves as 2 x 2; each wave owns (16*TM) rows x (8*TN) complex columns.
-template <int TM, int TN> struct TileCfg {
+// BK complex k per LDS stage (2 BK real k = BK / 2 MFMA k-steps); LDS holds two
+// stages (double buffer).
+template <int TM, int TN, int BK_ = 8> struct TileCfg {
   static constexpr int WM = 16 * TM;
   static constexpr int WN = 8 * TN;
   static constexpr int BM = 2 * WM;
   static constexpr int BN = 2 * WN;
-  static constexpr int BK = 8;            // complex k per LDS stage = 4 MFMA k-steps
+  static constexpr int BK = BK_;
   static constexpr int NA = BM * BK / 256;
   static constexpr int NB = BN * BK / 256;
 };
+// FP32 stages twice as deep as FP64 ones: same LDS bytes per stage
+template <typename Tc> struct StageK { static constexpr int value = 8; };
+template <> struct StageK<float> { static constexpr int value = 16; };
 
 struct GemmKArgs {
   int64_t M, N, K;
@@ -60,6 +65,9 @@ struct GemmKArgs {
   const uint64_t* maskB;
   int mwords;
 };
+
+// masked products keep their stage list in LDS: K / BK at most this many stages
+constexpr int MAX_MASKED_STAGES = 8192;
 
 // first k-stage >= from whose A and B masks overlap (every stage when unmasked)
 __device__ __forceinline__ int next_stage(const GemmKArgs& g, int tm, int tn, int from, int nk) {
@@ -139,12 +147,12 @@ __device__ __forceinline__ void tile_coords(int wg, int nwg, int tiles_m, int ti
 template <typename Tc, typename TA, typename TB, typename TC, int TM, int TN, bool A_KCONTIG,
           bool B_NCONTIG>
 __global__ __launch_bounds__(256) void zgemm_kernel(GemmKArgs g) {
-  typedef TileCfg<TM, TN> T;
+  typedef TileCfg<TM, TN, StageK<Tc>::value> T;
   constexpr int BM = T::BM, BN = T::BN, BK = T::BK, NA = T::NA, NB = T::NB;
   typedef typename acc4<Tc>::type acc_t;
 
-  __shared__ Tc As[2 * BK][BM + 1];
-  __shared__ Tc Bs[BK][2 * BN + 2];
+  __shared__ Tc As[2][2 * BK][BM + 1];
+  __shared__ Tc Bs[2][BK][2 * BN + 2];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -186,22 +194,22 @@ __global__ __launch_bounds__(256) void zgemm_kernel(GemmKArgs g) {
       rb[i][0] = re; rb[i][1] = sb * im;
     }
   };
-  auto store_tiles = [&]() {
+  auto store_tiles = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
       int e = tid + 256 * i;
       int row, kc;
       if (A_KCONTIG) { row = e / BK; kc = e % BK; } else { kc = e / BM; row = e % BM; }
-      As[2 * kc][row] = ra[i][0];
-      As[2 * kc + 1][row] = ra[i][1];
+      As[buf][2 * kc][row] = ra[i][0];
+      As[buf][2 * kc + 1][row] = ra[i][1];
     }
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       int e = tid + 256 * i;
       int col, kc;
       if (B_NCONTIG) { kc = e / BN; col = e % BN; } else { col = e / BK; kc = e % BK; }
-      Bs[kc][2 * col] = rb[i][0];
-      Bs[kc][2 * col + 1] = rb[i][1];
+      Bs[buf][kc][2 * col] = rb[i][0];
+      Bs[buf][kc][2 * col + 1] = rb[i][1];
     }
   };
 
@@ -218,24 +226,43 @@ __global__ __launch_bounds__(256) void zgemm_kernel(GemmKArgs g) {
   const bool take_im = (pa != pb);
   const Tc im_sign = (pa == 1) ? (Tc)-1 : (Tc)1;
 
+  // k-stages to visit: all of them, or (with masks) the list of stages where
+  // the A and B tile masks overlap, built once in LDS so that the main loop is
+  // a plain counted loop (keeps the accumulators in place across iterations)
   const int nk = (int)((g.K + BK - 1) / BK);
-  int t = next_stage(g, tm, tn, 0, nk);
-  if (t < nk) load_tiles((int64_t)t * BK);
-  while (t < nk) {
+  __shared__ unsigned short stages[MAX_MASKED_STAGES];
+  __shared__ int nstage_s;
+  int nst = nk;
+  if (g.maskA) {
+    if (tid == 0) {
+      int c = 0;
+      for (int t = next_stage(g, tm, tn, 0, nk); t < nk; t = next_stage(g, tm, tn, t + 1, nk))
+        stages[c++] = (unsigned short)t;
+      nstage_s = c;
+    }
     __syncthreads();
-    store_tiles();
-    __syncthreads();
-    const int tnext = next_stage(g, tm, tn, t + 1, nk);
-    if (tnext < nk) load_tiles((int64_t)tnext * BK);
-    t = tnext;
+    nst = nstage_s;
+  }
+  auto stage_of = [&](int it) -> int { return g.maskA ? (int)stages[it] : it; };
+  // stage pipeline: global loads of the next stage into registers, MFMAs on
+  // the current LDS buffer, registers into the other buffer, one barrier
+  int buf = 0;
+  if (nst > 0) {
+    load_tiles((int64_t)stage_of(0) * BK);
+    store_tiles(0);
+  }
+  __syncthreads();
+  for (int it = 0; it < nst; ++it) {
+    const bool more = it + 1 < nst;
+    if (more) load_tiles((int64_t)stage_of(it + 1) * BK);
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
+    for (int ks = 0; ks < BK / 2; ++ks) {
       Tc av[TM], bv[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) av[i] = As[4 * ks + kk][wm0 + 16 * i + (lane & 15)];
+      for (int i = 0; i < TM; ++i) av[i] = As[buf][4 * ks + kk][wm0 + 16 * i + (lane & 15)];
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
-        const Tc* p = &Bs[2 * ks + (kk >> 1)][2 * (wn0 + 8 * j + ((lane & 15) >> 1))];
+        const Tc* p = &Bs[buf][2 * ks + (kk >> 1)][2 * (wn0 + 8 * j + ((lane & 15) >> 1))];
         Tc re = p[0], im = p[1];
         bv[j] = take_im ? im_sign * im : re;
       }
@@ -244,6 +271,9 @@ __global__ __launch_bounds__(256) void zgemm_kernel(GemmKArgs g) {
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = mfma16(av[i], bv[j], acc[i][j]);
     }
+    if (more) store_tiles(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
   }
 
   // epilogue: lane holds the real (pb = 0) or imaginary (pb = 1) part of
@@ -290,6 +320,7 @@ inline int build_masks(const ZGemm& z, GemmKArgs& g, int BM, int BN, int BK, voi
   g.maskA = nullptr; g.maskB = nullptr; g.mwords = 0;
   if (!mws || z.batch != 1 || z.K == 0) return PTN_OK;
   const int64_t nk = (z.K + BK - 1) / BK;
+  if (nk > MAX_MASKED_STAGES) return PTN_OK;   // dense product (correct, no skipping)
   const int words = (int)((nk + 63) / 64);
   const size_t need = (size_t)((int64_t)g.tiles_m + g.tiles_n) * words * 8;
   if (mws_bytes < need) return PTN_ERR_WORKSPACE;
@@ -306,7 +337,7 @@ inline int build_masks(const ZGemm& z, GemmKArgs& g, int BM, int BN, int BK, voi
 
 template <typename Tc, typename TA, typename TB, typename TC, int TM, int TN>
 inline int launch_cfg(const ZGemm& z, hipStream_t s, void* mws = nullptr, size_t mws_bytes = 0) {
-  typedef TileCfg<TM, TN> T;
+  typedef TileCfg<TM, TN, StageK<Tc>::value> T;
   GemmKArgs g;
   g.M = z.M; g.N = z.N; g.K = z.K;
   g.A = z.A; g.a_m = z.a_m; g.a_k = z.a_k; g.conjA = z.conjA;
