@@ -88,10 +88,19 @@ const char* ptn_status_string(int status) {
 
 const char* ptn_last_hip_error(void) { return hipGetErrorString(g_last_hip); }
 
+// In the FP32 and split-bf16 modes the environment blocks are converted to
+// complex64 once per call (one pass), as the Krylov solver does once per
+// local solve: the products then stream half the operand bytes.
+static inline bool lowp(int p) { return (p & ~PTN_BLOCK_SPARSE) != PTN_FP64; }
+static inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+static inline size_t env64_bytes(int64_t D, int64_t M) { return al256((size_t)(D * M * D) * 8); }
+
 size_t ptn_heff_workspace(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml,
                           int64_t Mr) {
   Dims D = {Dl, d, Dr, Ml, Mr};
-  return heff_ws_bytes(precision, D);
+  size_t b = al256(heff_ws_bytes(precision, D));
+  if (lowp(precision)) b += env64_bytes(Dl, Ml) + env64_bytes(Dr, Mr);
+  return b;
 }
 
 int ptn_heff_apply(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr,
@@ -99,25 +108,56 @@ int ptn_heff_apply(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml,
                    size_t ws_bytes, void* stream) {
   if (!precision_ok(precision)) return PTN_ERR_UNSUPPORTED;
   Dims D = {Dl, d, Dr, Ml, Mr};
-  return heff_apply(precision, D, l, PTN_T_C128, w, r, PTN_T_C128, a, PTN_T_C128, out,
-                    PTN_T_C128, ws, ws_bytes, (hipStream_t)stream);
+  if (Dl <= 0 || d <= 0 || Dr <= 0 || Ml <= 0 || Mr <= 0) return PTN_ERR_ARG;
+  if (ws_bytes < ptn_heff_workspace(precision, Dl, d, Dr, Ml, Mr)) return PTN_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t base = al256(heff_ws_bytes(precision, D));
+  if (!lowp(precision))
+    return heff_apply(precision, D, l, PTN_T_C128, w, r, PTN_T_C128, a, PTN_T_C128, out,
+                      PTN_T_C128, ws, base, s);
+  char* l64 = (char*)ws + base;
+  char* r64 = l64 + env64_bytes(Dl, Ml);
+  int st;
+  if ((st = convert_c128_c64(Dl * Ml * Dl, l, l64, s))) return st;
+  if ((st = convert_c128_c64(Dr * Mr * Dr, r, r64, s))) return st;
+  return heff_apply(precision, D, l64, PTN_T_C64, w, r64, PTN_T_C64, a, PTN_T_C128, out,
+                    PTN_T_C128, ws, base, s);
 }
 
 size_t ptn_bond_workspace(int precision, int64_t Dl, int64_t Dr, int64_t M) {
-  return bond_ws_bytes(precision, Dl, Dr, M);
+  size_t b = al256(bond_ws_bytes(precision, Dl, Dr, M));
+  if (lowp(precision)) b += env64_bytes(Dl, M) + env64_bytes(Dr, M);
+  return b;
 }
 
 int ptn_bond_apply(int precision, int64_t Dl, int64_t Dr, int64_t M, const void* l, const void* r,
                    const void* c, void* out, void* ws, size_t ws_bytes, void* stream) {
   if (!precision_ok(precision)) return PTN_ERR_UNSUPPORTED;
-  return bond_apply(precision, Dl, Dr, M, l, PTN_T_C128, r, PTN_T_C128, c, PTN_T_C128, out,
-                    PTN_T_C128, ws, ws_bytes, (hipStream_t)stream);
+  if (Dl <= 0 || Dr <= 0 || M <= 0) return PTN_ERR_ARG;
+  if (ws_bytes < ptn_bond_workspace(precision, Dl, Dr, M)) return PTN_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t base = al256(bond_ws_bytes(precision, Dl, Dr, M));
+  if (!lowp(precision))
+    return bond_apply(precision, Dl, Dr, M, l, PTN_T_C128, r, PTN_T_C128, c, PTN_T_C128, out,
+                      PTN_T_C128, ws, base, s);
+  char* l64 = (char*)ws + base;
+  char* r64 = l64 + env64_bytes(Dl, M);
+  int st;
+  if ((st = convert_c128_c64(Dl * M * Dl, l, l64, s))) return st;
+  if ((st = convert_c128_c64(Dr * M * Dr, r, r64, s))) return st;
+  return bond_apply(precision, Dl, Dr, M, l64, PTN_T_C64, r64, PTN_T_C64, c, PTN_T_C128, out,
+                    PTN_T_C128, ws, base, s);
 }
 
 size_t ptn_env_workspace(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml,
                          int64_t Mr) {
   Dims D = {Dl, d, Dr, Ml, Mr};
-  return env_ws_bytes(precision, D);
+  size_t b = al256(env_ws_bytes(precision, D));
+  if (lowp(precision)) {
+    const size_t el = env64_bytes(Dl, Ml), er = env64_bytes(Dr, Mr);
+    b += el > er ? el : er;
+  }
+  return b;
 }
 
 int ptn_env_left(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr,
@@ -125,7 +165,16 @@ int ptn_env_left(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, i
                  size_t ws_bytes, void* stream) {
   if (!precision_ok(precision)) return PTN_ERR_UNSUPPORTED;
   Dims D = {Dl, d, Dr, Ml, Mr};
-  return env_left(precision, D, a, b, w, l, PTN_T_C128, out, ws, ws_bytes, (hipStream_t)stream);
+  if (Dl <= 0 || d <= 0 || Dr <= 0 || Ml <= 0 || Mr <= 0) return PTN_ERR_ARG;
+  if (ws_bytes < ptn_env_workspace(precision, Dl, d, Dr, Ml, Mr)) return PTN_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t base = al256(env_ws_bytes(precision, D));
+  if (!lowp(precision))
+    return env_left(precision, D, a, b, w, l, PTN_T_C128, out, ws, base, s);
+  char* l64 = (char*)ws + base;
+  int st;
+  if ((st = convert_c128_c64(Dl * Ml * Dl, l, l64, s))) return st;
+  return env_left(precision, D, a, b, w, l64, PTN_T_C64, out, ws, base, s);
 }
 
 int ptn_env_right(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr,
@@ -133,7 +182,16 @@ int ptn_env_right(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, 
                   size_t ws_bytes, void* stream) {
   if (!precision_ok(precision)) return PTN_ERR_UNSUPPORTED;
   Dims D = {Dl, d, Dr, Ml, Mr};
-  return env_right(precision, D, a, b, w, r, PTN_T_C128, out, ws, ws_bytes, (hipStream_t)stream);
+  if (Dl <= 0 || d <= 0 || Dr <= 0 || Ml <= 0 || Mr <= 0) return PTN_ERR_ARG;
+  if (ws_bytes < ptn_env_workspace(precision, Dl, d, Dr, Ml, Mr)) return PTN_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t base = al256(env_ws_bytes(precision, D));
+  if (!lowp(precision))
+    return env_right(precision, D, a, b, w, r, PTN_T_C128, out, ws, base, s);
+  char* r64 = (char*)ws + base;
+  int st;
+  if ((st = convert_c128_c64(Dr * Mr * Dr, r, r64, s))) return st;
+  return env_right(precision, D, a, b, w, r64, PTN_T_C64, out, ws, base, s);
 }
 
 size_t ptn_zgemm_workspace(int64_t M, int64_t N, int64_t K) {

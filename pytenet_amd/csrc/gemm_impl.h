@@ -67,7 +67,7 @@ struct GemmKArgs {
 };
 
 // masked products keep their stage list in LDS: K / BK at most this many stages
-constexpr int MAX_MASKED_STAGES = 8192;
+constexpr int MAX_MASKED_STAGES = 2048;
 
 // first k-stage >= from whose A and B masks overlap (every stage when unmasked)
 __device__ __forceinline__ int next_stage(const GemmKArgs& g, int tm, int tn, int from, int nk) {
@@ -371,6 +371,12 @@ inline int launch_cfg(const ZGemm& z, hipStream_t s, void* mws = nullptr, size_t
 // two workgroups per CU, else the small one (64 x 32)
 template <typename Tc, typename TA, typename TB, typename TC>
 inline int launch_typed(const ZGemm& z, hipStream_t s, void* mws = nullptr, size_t mws_bytes = 0) {
+  // FP32: 128 x 128 complex tiles (half the operand bytes per flop of 128 x 64)
+  // when they still give two workgroups per CU
+  if constexpr (sizeof(Tc) == 4) {
+    int64_t wide = ((z.M + 127) / 128) * ((z.N + 127) / 128) * z.batch;
+    if (wide >= 512) return launch_cfg<Tc, TA, TB, TC, 4, 8>(z, s, mws, mws_bytes);
+  }
   int64_t big = ((z.M + 127) / 128) * ((z.N + 63) / 64) * z.batch;
   if (big >= 512) return launch_cfg<Tc, TA, TB, TC, 4, 4>(z, s, mws, mws_bytes);
   return launch_cfg<Tc, TA, TB, TC, 2, 2>(z, s, mws, mws_bytes);
