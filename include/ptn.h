@@ -163,6 +163,36 @@ int ptn_scale_axis(int64_t n0, int64_t n1, int64_t n2, void* a, const double* s,
 int ptn_qsparse3(int64_t n0, int64_t n1, int64_t n2, const int64_t* q0, const int64_t* q1,
                  const int64_t* q2, void* a, int zero_them, double* out, void* stream);
 
+/* ---- Batched forms for many independent chains (no counterpart in the
+ * reference, which evolves one chain per call; they replace a loop of the
+ * reference's tdvp_singlesite over chains, pytenet/tdvp.py:26-118). `batch`
+ * problems of identical shape, every tensor stacked contiguously along a
+ * leading batch axis; status words / energies per problem. */
+int ptn_zgemm_batched(int64_t batch, int64_t M, int64_t N, int64_t K,
+                      const void* A, int64_t a_m, int64_t a_k, int64_t bz_a, int conjA,
+                      const void* B, int64_t b_k, int64_t b_n, int64_t bz_b, int conjB,
+                      void* C, int64_t c_m, int64_t c_n, int64_t bz_c, int accumulate,
+                      void* stream);
+size_t ptn_krylov_workspace_batched(const ptn_local_op* op, int numiter, int64_t batch);
+int ptn_krylov_expm_batched(const ptn_local_op* op, int64_t batch, const void* v, double dt_re,
+                            double dt_im, int numiter, void* out, int32_t* info,
+                            void* ws, size_t ws_bytes, void* stream);
+int ptn_krylov_eigh_batched(const ptn_local_op* op, int64_t batch, const void* v, int numiter,
+                            void* out, double* energy, int32_t* info,
+                            void* ws, size_t ws_bytes, void* stream);
+size_t ptn_env_workspace_batched(int precision, int64_t batch, int64_t Dl, int64_t d, int64_t Dr,
+                                 int64_t Ml, int64_t Mr);
+int ptn_env_left_batched(int precision, int64_t batch, int64_t Dl, int64_t d, int64_t Dr,
+                         int64_t Ml, int64_t Mr, const void* a, const void* b, const void* w,
+                         const void* l, void* out, void* ws, size_t ws_bytes, void* stream);
+int ptn_env_right_batched(int precision, int64_t batch, int64_t Dl, int64_t d, int64_t Dr,
+                          int64_t Ml, int64_t Mr, const void* a, const void* b, const void* w,
+                          const void* r, void* out, void* ws, size_t ws_bytes, void* stream);
+/* m x n row-major matrices, Q m x k (ldq = k), R k x n (ldr = n) */
+size_t ptn_qr_workspace_batched(int64_t batch, int64_t m, int64_t n);
+int ptn_qr_batched(int64_t batch, int64_t m, int64_t n, const void* a, void* q, void* r,
+                   void* ws, size_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,3 +23,4 @@ from .tdvp import *
 from .dmrg import *
 
 __version__ = "0.1.0"
+from .batched import *

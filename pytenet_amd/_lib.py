@@ -68,6 +68,21 @@ SIGNATURES = [
                         c_p, c_sz, c_p]),
     ("ptn_scale_axis", c_int, [c_i64, c_i64, c_i64, c_p, c_p, c_int, c_p]),
     ("ptn_qsparse3", c_int, [c_i64, c_i64, c_i64, c_p, c_p, c_p, c_p, c_int, c_p, c_p]),
+    ("ptn_zgemm_batched", c_int, [c_i64, c_i64, c_i64, c_i64, c_p, c_i64, c_i64, c_i64, c_int,
+                                  c_p, c_i64, c_i64, c_i64, c_int, c_p, c_i64, c_i64, c_i64,
+                                  c_int, c_p]),
+    ("ptn_krylov_workspace_batched", c_sz, [ctypes.POINTER(LocalOp), c_int, c_i64]),
+    ("ptn_krylov_expm_batched", c_int, [ctypes.POINTER(LocalOp), c_i64, c_p, c_dbl, c_dbl, c_int,
+                                        c_p, c_p, c_p, c_sz, c_p]),
+    ("ptn_krylov_eigh_batched", c_int, [ctypes.POINTER(LocalOp), c_i64, c_p, c_int, c_p, c_p, c_p,
+                                        c_p, c_sz, c_p]),
+    ("ptn_env_workspace_batched", c_sz, [c_int, c_i64, c_i64, c_i64, c_i64, c_i64, c_i64]),
+    ("ptn_env_left_batched", c_int, [c_int, c_i64, c_i64, c_i64, c_i64, c_i64, c_i64,
+                                     c_p, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
+    ("ptn_env_right_batched", c_int, [c_int, c_i64, c_i64, c_i64, c_i64, c_i64, c_i64,
+                                      c_p, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
+    ("ptn_qr_workspace_batched", c_sz, [c_i64, c_i64, c_i64]),
+    ("ptn_qr_batched", c_int, [c_i64, c_i64, c_i64, c_p, c_p, c_p, c_p, c_sz, c_p]),
 ]
 
 _lib = None

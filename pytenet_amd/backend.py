@@ -455,3 +455,106 @@ def block_sparse_svd(a, q0, q1):
         k += kk
     s = torch.cat(s_parts).cpu().numpy()
     return u[:, :k].contiguous(), s, v[:k, :].contiguous(), qi[:k]
+
+
+# ---------------------------------------------------------------------------
+# batched forms: many independent problems of identical shape, stacked along a
+# leading batch axis (one launch sequence for all of them)
+
+def matmul_batched(A, B, conjA=False, conjB=False):
+    """C[b] = op(A[b]) @ op(B[b]) for 3-D complex128 device tensors of any strides."""
+    assert A.dim() == 3 and B.dim() == 3 and A.shape[0] == B.shape[0] and A.shape[2] == B.shape[1]
+    assert A.dtype == torch.complex128 and B.dtype == torch.complex128
+    nb, M, K = A.shape
+    N = B.shape[2]
+    out = torch.empty((nb, M, N), dtype=torch.complex128, device=A.device)
+    lib = _lib.load()
+    _check(lib.ptn_zgemm_batched(nb, M, N, K, ptr(A), A.stride(1), A.stride(2), A.stride(0),
+                                 int(conjA), ptr(B), B.stride(1), B.stride(2), B.stride(0),
+                                 int(conjB), ptr(out), out.stride(1), out.stride(2),
+                                 out.stride(0), 0, stream()), "ptn_zgemm_batched")
+    return out
+
+
+def env_left_batched(a, w, l, precision="fp64"):
+    """Batched contraction_operator_step_left with the bra equal to the ket:
+    a (B, Dl, d, Dr), w (B, Ml, d, d, Mr), l (B, Dl, Ml, Dl) -> (B, Dr, Mr, Dr)."""
+    a, w, l = as_dev(a), as_dev(w), as_dev(l)
+    nb, Dl, d, Dr = a.shape
+    Ml, Mr = w.shape[1], w.shape[4]
+    prec = precision_code(precision)
+    lib = _lib.load()
+    out = torch.empty((nb, Dr, Mr, Dr), dtype=torch.complex128, device=a.device)
+    ws = workspace(lib.ptn_env_workspace_batched(prec, nb, Dl, d, Dr, Ml, Mr))
+    _check(lib.ptn_env_left_batched(prec, nb, Dl, d, Dr, Ml, Mr, ptr(a), ptr(a), ptr(w), ptr(l),
+                                    ptr(out), ptr(ws), ws.numel(), stream()),
+           "ptn_env_left_batched")
+    return out
+
+
+def env_right_batched(a, w, r, precision="fp64"):
+    """Batched contraction_operator_step_right with the bra equal to the ket:
+    a (B, Dl, d, Dr), w (B, Ml, d, d, Mr), r (B, Dr, Mr, Dr) -> (B, Dl, Ml, Dl)."""
+    a, w, r = as_dev(a), as_dev(w), as_dev(r)
+    nb, Dl, d, Dr = a.shape
+    Ml, Mr = w.shape[1], w.shape[4]
+    prec = precision_code(precision)
+    lib = _lib.load()
+    out = torch.empty((nb, Dl, Ml, Dl), dtype=torch.complex128, device=a.device)
+    ws = workspace(lib.ptn_env_workspace_batched(prec, nb, Dl, d, Dr, Ml, Mr))
+    _check(lib.ptn_env_right_batched(prec, nb, Dl, d, Dr, Ml, Mr, ptr(a), ptr(a), ptr(w), ptr(r),
+                                     ptr(out), ptr(ws), ws.numel(), stream()),
+           "ptn_env_right_batched")
+    return out
+
+
+def krylov_local_batched(kind, v, l, w, r, numiter, mode, dt=0.0, precision="fp64"):
+    """Batched krylov_local: v (B, ...), l (B, Dl, Ml, Dl), w (B, Ml, d, d, Mr) or None,
+    r (B, Dr, Mr, Dr). Returns (out, energies (B,) or None, info (B, 4))."""
+    v, l, r = as_dev(v), as_dev(l), as_dev(r)
+    if w is not None:
+        w = as_dev(w)
+    nb = v.shape[0]
+    if kind == _lib.PTN_OP_HEFF:
+        Dl, d, Dr = v.shape[1:]
+    else:
+        (Dl, Dr), d = v.shape[1:], 1
+    op = _lib.LocalOp()
+    op.kind = kind
+    op.precision = precision_code(precision)
+    op.Dl, op.d, op.Dr = Dl, d, Dr
+    op.Ml, op.Mr = l.shape[2], r.shape[2]
+    op.l = l.data_ptr()
+    op.w = w.data_ptr() if w is not None else None
+    op.r = r.data_ptr()
+    lib = _lib.load()
+    ws = workspace(lib.ptn_krylov_workspace_batched(ctypes.byref(op), int(numiter), nb))
+    out = torch.empty_like(v)
+    info = torch.zeros((nb, 4), dtype=torch.int32, device=v.device)
+    if mode == "expm":
+        dt = complex(dt)
+        _check(lib.ptn_krylov_expm_batched(ctypes.byref(op), nb, ptr(v), dt.real, dt.imag,
+                                           int(numiter), ptr(out), ptr(info), ptr(ws), ws.numel(),
+                                           stream()), "ptn_krylov_expm_batched")
+        return out, None, info
+    energy = torch.zeros(nb, dtype=torch.float64, device=v.device)
+    _check(lib.ptn_krylov_eigh_batched(ctypes.byref(op), nb, ptr(v), int(numiter), ptr(out),
+                                       ptr(energy), ptr(info), ptr(ws), ws.numel(), stream()),
+           "ptn_krylov_eigh_batched")
+    return out, energy, info
+
+
+def qr_batched(a):
+    """Reduced Householder QR of every matrix of a (B, m, n) complex128 device tensor."""
+    a = as_dev(a)
+    nb, m, n = a.shape
+    k = min(m, n)
+    q = torch.empty((nb, m, k), dtype=torch.complex128, device=a.device)
+    r = torch.empty((nb, k, n), dtype=torch.complex128, device=a.device)
+    if m == 0 or n == 0 or nb == 0:
+        return q, r
+    lib = _lib.load()
+    ws = workspace(lib.ptn_qr_workspace_batched(nb, m, n))
+    _check(lib.ptn_qr_batched(nb, m, n, ptr(a), ptr(q), ptr(r), ptr(ws), ws.numel(), stream()),
+           "ptn_qr_batched")
+    return q, r

@@ -6,13 +6,13 @@
 
 namespace ptn {
 
-size_t krylov_ws_bytes(const ptn_local_op* op, int numiter);
+size_t krylov_ws_bytes(const ptn_local_op* op, int numiter, int64_t batch);
 int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, double dt_im,
                int numiter, void* out, double* energy, int32_t* info, double* alpha_beta,
-               void* ws, size_t ws_bytes, hipStream_t s);
-size_t qr_ws_bytes(int64_t m, int64_t n);
+               void* ws, size_t ws_bytes, hipStream_t s, int64_t batch);
+size_t qr_ws_bytes(int64_t m, int64_t n, int64_t batch);
 int qr_run(int64_t m, int64_t n, const void* a, int64_t lda, void* q, int64_t ldq, void* r,
-           int64_t ldr, void* ws, size_t ws_bytes, hipStream_t s);
+           int64_t ldr, void* ws, size_t ws_bytes, hipStream_t s, int64_t batch);
 size_t svd_ws_bytes(int64_t m, int64_t n);
 int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t ldu, double* s,
             void* vh, int64_t ldvh, int32_t* info, void* ws, size_t ws_bytes, hipStream_t st);
@@ -149,49 +149,75 @@ int ptn_bond_apply(int precision, int64_t Dl, int64_t Dr, int64_t M, const void*
                     PTN_T_C128, ws, base, s);
 }
 
-size_t ptn_env_workspace(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml,
-                         int64_t Mr) {
+static size_t env_workspace_b(int precision, int64_t batch, int64_t Dl, int64_t d, int64_t Dr,
+                              int64_t Ml, int64_t Mr) {
   Dims D = {Dl, d, Dr, Ml, Mr};
-  size_t b = al256(env_ws_bytes(precision, D));
+  size_t b = al256(env_ws_bytes(precision, D, batch));
   if (lowp(precision)) {
     const size_t el = env64_bytes(Dl, Ml), er = env64_bytes(Dr, Mr);
-    b += el > er ? el : er;
+    b += (size_t)batch * (el > er ? el : er);
   }
   return b;
+}
+
+static int env_step_b(int left, int precision, int64_t batch, int64_t Dl, int64_t d, int64_t Dr,
+                      int64_t Ml, int64_t Mr, const void* a, const void* b, const void* w,
+                      const void* blk, void* out, void* ws, size_t ws_bytes, hipStream_t s) {
+  if (!precision_ok(precision)) return PTN_ERR_UNSUPPORTED;
+  Dims D = {Dl, d, Dr, Ml, Mr};
+  if (Dl <= 0 || d <= 0 || Dr <= 0 || Ml <= 0 || Mr <= 0 || batch < 1) return PTN_ERR_ARG;
+  if (ws_bytes < env_workspace_b(precision, batch, Dl, d, Dr, Ml, Mr)) return PTN_ERR_WORKSPACE;
+  const size_t base = al256(env_ws_bytes(precision, D, batch));
+  const void* e = blk;
+  int te = PTN_T_C128, st;
+  if (lowp(precision)) {
+    // the environment block stack in complex64, converted once (contiguous)
+    char* e64 = (char*)ws + base;
+    const int64_t ne = left ? Dl * Ml * Dl : Dr * Mr * Dr;
+    if ((st = convert_c128_c64(batch * ne, blk, e64, s))) return st;
+    e = e64;
+    te = PTN_T_C64;
+  }
+  if (left) return env_left(precision, D, a, b, w, e, te, out, ws, base, s, batch);
+  return env_right(precision, D, a, b, w, e, te, out, ws, base, s, batch);
+}
+
+size_t ptn_env_workspace(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml,
+                         int64_t Mr) {
+  return env_workspace_b(precision, 1, Dl, d, Dr, Ml, Mr);
 }
 
 int ptn_env_left(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr,
                  const void* a, const void* b, const void* w, const void* l, void* out, void* ws,
                  size_t ws_bytes, void* stream) {
-  if (!precision_ok(precision)) return PTN_ERR_UNSUPPORTED;
-  Dims D = {Dl, d, Dr, Ml, Mr};
-  if (Dl <= 0 || d <= 0 || Dr <= 0 || Ml <= 0 || Mr <= 0) return PTN_ERR_ARG;
-  if (ws_bytes < ptn_env_workspace(precision, Dl, d, Dr, Ml, Mr)) return PTN_ERR_WORKSPACE;
-  hipStream_t s = (hipStream_t)stream;
-  const size_t base = al256(env_ws_bytes(precision, D));
-  if (!lowp(precision))
-    return env_left(precision, D, a, b, w, l, PTN_T_C128, out, ws, base, s);
-  char* l64 = (char*)ws + base;
-  int st;
-  if ((st = convert_c128_c64(Dl * Ml * Dl, l, l64, s))) return st;
-  return env_left(precision, D, a, b, w, l64, PTN_T_C64, out, ws, base, s);
+  return env_step_b(1, precision, 1, Dl, d, Dr, Ml, Mr, a, b, w, l, out, ws, ws_bytes,
+                    (hipStream_t)stream);
 }
 
 int ptn_env_right(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, int64_t Mr,
                   const void* a, const void* b, const void* w, const void* r, void* out, void* ws,
                   size_t ws_bytes, void* stream) {
-  if (!precision_ok(precision)) return PTN_ERR_UNSUPPORTED;
-  Dims D = {Dl, d, Dr, Ml, Mr};
-  if (Dl <= 0 || d <= 0 || Dr <= 0 || Ml <= 0 || Mr <= 0) return PTN_ERR_ARG;
-  if (ws_bytes < ptn_env_workspace(precision, Dl, d, Dr, Ml, Mr)) return PTN_ERR_WORKSPACE;
-  hipStream_t s = (hipStream_t)stream;
-  const size_t base = al256(env_ws_bytes(precision, D));
-  if (!lowp(precision))
-    return env_right(precision, D, a, b, w, r, PTN_T_C128, out, ws, base, s);
-  char* r64 = (char*)ws + base;
-  int st;
-  if ((st = convert_c128_c64(Dr * Mr * Dr, r, r64, s))) return st;
-  return env_right(precision, D, a, b, w, r64, PTN_T_C64, out, ws, base, s);
+  return env_step_b(0, precision, 1, Dl, d, Dr, Ml, Mr, a, b, w, r, out, ws, ws_bytes,
+                    (hipStream_t)stream);
+}
+
+size_t ptn_env_workspace_batched(int precision, int64_t batch, int64_t Dl, int64_t d, int64_t Dr,
+                                 int64_t Ml, int64_t Mr) {
+  return env_workspace_b(precision, batch, Dl, d, Dr, Ml, Mr);
+}
+
+int ptn_env_left_batched(int precision, int64_t batch, int64_t Dl, int64_t d, int64_t Dr,
+                         int64_t Ml, int64_t Mr, const void* a, const void* b, const void* w,
+                         const void* l, void* out, void* ws, size_t ws_bytes, void* stream) {
+  return env_step_b(1, precision, batch, Dl, d, Dr, Ml, Mr, a, b, w, l, out, ws, ws_bytes,
+                    (hipStream_t)stream);
+}
+
+int ptn_env_right_batched(int precision, int64_t batch, int64_t Dl, int64_t d, int64_t Dr,
+                          int64_t Ml, int64_t Mr, const void* a, const void* b, const void* w,
+                          const void* r, void* out, void* ws, size_t ws_bytes, void* stream) {
+  return env_step_b(0, precision, batch, Dl, d, Dr, Ml, Mr, a, b, w, r, out, ws, ws_bytes,
+                    (hipStream_t)stream);
 }
 
 size_t ptn_zgemm_workspace(int64_t M, int64_t N, int64_t K) {
@@ -208,27 +234,27 @@ int ptn_zgemm(int64_t M, int64_t N, int64_t K, const void* A, int64_t a_m, int64
 }
 
 size_t ptn_krylov_workspace(const ptn_local_op* op, int numiter) {
-  return krylov_ws_bytes(op, numiter);
+  return krylov_ws_bytes(op, numiter, 1);
 }
 
 int ptn_krylov_expm(const ptn_local_op* op, const void* v, double dt_re, double dt_im, int numiter,
                     void* out, int32_t* info, double* alpha_beta, void* ws, size_t ws_bytes,
                     void* stream) {
   return krylov_run(op, 0, v, dt_re, dt_im, numiter, out, nullptr, info, alpha_beta, ws, ws_bytes,
-                    (hipStream_t)stream);
+                    (hipStream_t)stream, 1);
 }
 
 int ptn_krylov_eigh(const ptn_local_op* op, const void* v, int numiter, void* out, double* energy,
                     int32_t* info, double* alpha_beta, void* ws, size_t ws_bytes, void* stream) {
   return krylov_run(op, 1, v, 0.0, 0.0, numiter, out, energy, info, alpha_beta, ws, ws_bytes,
-                    (hipStream_t)stream);
+                    (hipStream_t)stream, 1);
 }
 
-size_t ptn_qr_workspace(int64_t m, int64_t n) { return qr_ws_bytes(m, n); }
+size_t ptn_qr_workspace(int64_t m, int64_t n) { return qr_ws_bytes(m, n, 1); }
 
 int ptn_qr(int64_t m, int64_t n, const void* a, int64_t lda, void* q, int64_t ldq, void* r,
            int64_t ldr, void* ws, size_t ws_bytes, void* stream) {
-  return qr_run(m, n, a, lda, q, ldq, r, ldr, ws, ws_bytes, (hipStream_t)stream);
+  return qr_run(m, n, a, lda, q, ldq, r, ldr, ws, ws_bytes, (hipStream_t)stream, 1);
 }
 
 size_t ptn_svd_workspace(int64_t m, int64_t n) { return svd_ws_bytes(m, n); }
@@ -236,6 +262,44 @@ size_t ptn_svd_workspace(int64_t m, int64_t n) { return svd_ws_bytes(m, n); }
 int ptn_svd(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t ldu, double* s,
             void* vh, int64_t ldvh, int32_t* info, void* ws, size_t ws_bytes, void* stream) {
   return svd_run(m, n, a, lda, u, ldu, s, vh, ldvh, info, ws, ws_bytes, (hipStream_t)stream);
+}
+
+int ptn_zgemm_batched(int64_t batch, int64_t M, int64_t N, int64_t K, const void* A, int64_t a_m,
+                      int64_t a_k, int64_t bz_a, int conjA, const void* B, int64_t b_k,
+                      int64_t b_n, int64_t bz_b, int conjB, void* C, int64_t c_m, int64_t c_n,
+                      int64_t bz_c, int accumulate, void* stream) {
+  if (batch < 1 || batch > 65535) return PTN_ERR_ARG;
+  ZGemm g = {M, N, K, A, a_m, a_k, conjA, B, b_k, b_n, conjB, C, c_m, c_n, accumulate,
+             batch, bz_a, bz_b, bz_c};
+  return zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, (hipStream_t)stream);
+}
+
+size_t ptn_krylov_workspace_batched(const ptn_local_op* op, int numiter, int64_t batch) {
+  return krylov_ws_bytes(op, numiter, batch);
+}
+
+int ptn_krylov_expm_batched(const ptn_local_op* op, int64_t batch, const void* v, double dt_re,
+                            double dt_im, int numiter, void* out, int32_t* info, void* ws,
+                            size_t ws_bytes, void* stream) {
+  return krylov_run(op, 0, v, dt_re, dt_im, numiter, out, nullptr, info, nullptr, ws, ws_bytes,
+                    (hipStream_t)stream, batch);
+}
+
+int ptn_krylov_eigh_batched(const ptn_local_op* op, int64_t batch, const void* v, int numiter,
+                            void* out, double* energy, int32_t* info, void* ws, size_t ws_bytes,
+                            void* stream) {
+  return krylov_run(op, 1, v, 0.0, 0.0, numiter, out, energy, info, nullptr, ws, ws_bytes,
+                    (hipStream_t)stream, batch);
+}
+
+size_t ptn_qr_workspace_batched(int64_t batch, int64_t m, int64_t n) {
+  return qr_ws_bytes(m, n, batch);
+}
+
+int ptn_qr_batched(int64_t batch, int64_t m, int64_t n, const void* a, void* q, void* r, void* ws,
+                   size_t ws_bytes, void* stream) {
+  const int64_t k = m < n ? m : n;
+  return qr_run(m, n, a, n, q, k, r, n, ws, ws_bytes, (hipStream_t)stream, batch);
 }
 
 int ptn_scale_axis(int64_t n0, int64_t n1, int64_t n2, void* a, const double* s, int axis,

@@ -66,8 +66,9 @@ enum { PTN_T_C128 = 0, PTN_T_C64 = 1 };
 int zgemm(const ZGemm& g, int compute /*PTN_FP64 | PTN_FP32*/, int ta, int tb, int tc,
           void* ws, size_t ws_bytes, hipStream_t s);
 
-// Y[l, o, r] = sum_i Wm[o, i] X[l, i, r]   (complex; Wm complex128 on device)
+// Y[l, o, r] = sum_i Wm[o, i] X[l, i, r]   (complex; Wm complex128 on device);
+// `batch` independent problems with contiguous X and Y and Wm stride bw
 int mode_mix(int64_t nl, int64_t ni, int64_t no, int64_t nr, const void* wm, const void* x, int tx,
-             void* y, int ty, hipStream_t s);
+             void* y, int ty, hipStream_t s, int64_t batch = 1, int64_t bw = 0);
 
 }  // namespace ptn

@@ -15,7 +15,7 @@
 namespace ptn {
 
 constexpr int LZ_MAX = 64;        // largest supported numiter
-constexpr int RED_BLOCKS = 512;   // fixed reduction width: deterministic sums
+constexpr int RED_BLOCKS = 512;   // largest reduction width (partial sums per reduction)
 
 struct LzState {
   double alpha[LZ_MAX];
@@ -32,12 +32,19 @@ struct LzState {
   int noconv;      // tridiagonal QL iteration did not converge
 };
 
-// sum of the RED_BLOCKS partials in a fixed order; every block of a kernel
-// computes the identical value (256 threads)
-__device__ double block_sum_partials(const double* __restrict__ part) {
+// Batched form: blockIdx.y selects the problem of a batch; vector arrays are
+// stacked with a per-problem stride, LzState and partial-sum arrays are
+// indexed by problem. `rb` partial sums per reduction (rb = min(512,
+// ceil(n / 256)), a function of n only, so every sum has a fixed order).
+
+// sum of the rb partials in a fixed order; every block of a kernel computes
+// the identical value (256 threads)
+__device__ double block_sum_partials(const double* __restrict__ part, int rb) {
   __shared__ double red[256];
   const int t = threadIdx.x;
-  red[t] = part[t] + part[t + 256];
+  double v = 0;
+  for (int i = t; i < rb; i += 256) v += part[i];
+  red[t] = v;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (t < w) red[t] += red[t + w];
@@ -61,6 +68,7 @@ __device__ void block_store_partial(double v, double* out) {
 }
 
 __global__ __launch_bounds__(256) void k_state_init(LzState* st, int numiter) {
+  st += blockIdx.x;
   int t = threadIdx.x;
   if (t < LZ_MAX) {
     st->alpha[t] = 0; st->beta[t] = 0; st->coef_re[t] = 0; st->coef_im[t] = 0;
@@ -72,9 +80,12 @@ __global__ __launch_bounds__(256) void k_state_init(LzState* st, int numiter) {
 }
 
 __global__ __launch_bounds__(256) void k_sumsq_partial(int64_t n, const double2* __restrict__ x,
-                                                       double* __restrict__ part) {
+                                                       int64_t xs, double* __restrict__ part,
+                                                       int rb) {
+  x += blockIdx.y * xs;
+  part += blockIdx.y * rb;
   double s = 0;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * 256) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)rb * 256) {
     double2 v = x[i];
     s += v.x * v.x + v.y * v.y;
   }
@@ -83,10 +94,14 @@ __global__ __launch_bounds__(256) void k_sumsq_partial(int64_t n, const double2*
 
 // Re <w, v> partials
 __global__ __launch_bounds__(256) void k_dot_partial(int64_t n, const double2* __restrict__ w,
-                                                     const double2* __restrict__ v,
-                                                     double* __restrict__ part) {
+                                                     int64_t ws_, const double2* __restrict__ v,
+                                                     int64_t vs, double* __restrict__ part,
+                                                     int rb) {
+  w += blockIdx.y * ws_;
+  v += blockIdx.y * vs;
+  part += blockIdx.y * rb;
   double s = 0;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * 256) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)rb * 256) {
     double2 a = w[i], b = v[i];
     s += a.x * b.x + a.y * b.y;
   }
@@ -95,9 +110,14 @@ __global__ __launch_bounds__(256) void k_dot_partial(int64_t n, const double2* _
 
 // v0 = x / |x|  (krylov.py:29-31)
 __global__ __launch_bounds__(256) void k_start(int64_t n, const double2* __restrict__ x,
-                                               double2* __restrict__ v0,
-                                               const double* __restrict__ part, LzState* st) {
-  double nrm = sqrt(block_sum_partials(part));
+                                               int64_t xs, double2* __restrict__ v0, int64_t vs,
+                                               const double* __restrict__ part, int rb,
+                                               LzState* st) {
+  x += blockIdx.y * xs;
+  v0 += blockIdx.y * vs;
+  part += blockIdx.y * rb;
+  st += blockIdx.y;
+  double nrm = sqrt(block_sum_partials(part, rb));
   bool bad = !(nrm > 0.0) || !isfinite(nrm);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->nrm0 = nrm;
@@ -116,12 +136,21 @@ __global__ __launch_bounds__(256) void k_start(int64_t n, const double2* __restr
 
 // alpha_j = Re <w, v_j>;  w -= alpha_j v_j + beta_{j-1} v_{j-1};  partials of |w|^2
 __global__ __launch_bounds__(256) void k_alpha_update(int64_t n, double2* __restrict__ w,
+                                                      int64_t ws_,
                                                       const double2* __restrict__ vj,
-                                                      const double2* __restrict__ vjm1, int j,
+                                                      const double2* __restrict__ vjm1,
+                                                      int64_t vs, int j,
                                                       const double* __restrict__ part_in,
-                                                      double* __restrict__ part_out, LzState* st) {
+                                                      double* __restrict__ part_out, int rb,
+                                                      LzState* st) {
+  st += blockIdx.y;
   if (st->stopped) return;
-  const double a = block_sum_partials(part_in);
+  w += blockIdx.y * ws_;
+  vj += blockIdx.y * vs;
+  vjm1 += blockIdx.y * vs;
+  part_in += blockIdx.y * rb;
+  part_out += blockIdx.y * rb;
+  const double a = block_sum_partials(part_in, rb);
   if (!isfinite(a)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->nonfinite = 1; st->stopped = 1; st->nvalid = j + 1; st->alpha[j] = a;
@@ -131,7 +160,7 @@ __global__ __launch_bounds__(256) void k_alpha_update(int64_t n, double2* __rest
   if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha[j] = a;
   const double b = (j > 0) ? st->beta[j - 1] : 0.0;
   double s = 0;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * 256) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)rb * 256) {
     double2 x = w[i], v = vj[i];
     double tr = a * v.x, ti = a * v.y;
     if (j > 0) {
@@ -147,11 +176,16 @@ __global__ __launch_bounds__(256) void k_alpha_update(int64_t n, double2* __rest
 
 // beta_j = |w|; breakdown test; v_{j+1} = w / beta_j
 __global__ __launch_bounds__(256) void k_beta_next(int64_t n, const double2* __restrict__ w,
-                                                   double2* __restrict__ vnext, int j,
-                                                   const double* __restrict__ part_in,
+                                                   int64_t ws_, double2* __restrict__ vnext,
+                                                   int64_t vs, int j,
+                                                   const double* __restrict__ part_in, int rb,
                                                    double thresh, LzState* st) {
+  st += blockIdx.y;
   if (st->stopped) return;
-  const double b = sqrt(block_sum_partials(part_in));
+  w += blockIdx.y * ws_;
+  vnext += blockIdx.y * vs;
+  part_in += blockIdx.y * rb;
+  const double b = sqrt(block_sum_partials(part_in, rb));
   const bool nf = !isfinite(b);
   const bool brk = !nf && (b < thresh);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -166,11 +200,12 @@ __global__ __launch_bounds__(256) void k_beta_next(int64_t n, const double2* __r
   }
 }
 
-// last alpha (krylov.py:54-56)
+// last alpha (krylov.py:54-56); one block per problem
 __global__ __launch_bounds__(256) void k_alpha_last(int j, const double* __restrict__ part_in,
-                                                    LzState* st) {
+                                                    int rb, LzState* st) {
+  st += blockIdx.x;
   if (st->stopped) return;
-  const double a = block_sum_partials(part_in);
+  const double a = block_sum_partials(part_in + blockIdx.x * rb, rb);
   if (threadIdx.x == 0) {
     st->alpha[j] = a;
     if (!isfinite(a)) st->nonfinite = 1;
@@ -186,6 +221,7 @@ __global__ __launch_bounds__(256) void k_alpha_last(int j, const double* __restr
 //   mode 1 (eigh):  c = U[:, 0], energy = lambda_0          (krylov.py:116-119)
 __global__ __launch_bounds__(64) void k_tridiag(LzState* st, int mode, double dt_re,
                                                 double dt_im) {
+  st += blockIdx.x;
   __shared__ double d[LZ_MAX], e[LZ_MAX];
   __shared__ double Z[LZ_MAX][LZ_MAX + 1];
   __shared__ double tre[LZ_MAX], tim[LZ_MAX];
@@ -290,10 +326,15 @@ __global__ __launch_bounds__(64) void k_tridiag(LzState* st, int mode, double dt
 }
 
 // out = sum_{j < nvalid} c_j V_j  (krylov.py:118, :136)
+// V_j of problem b at V + j * js + b * Vs
 __global__ __launch_bounds__(256) void k_combine(int64_t n, const double2* __restrict__ V,
+                                                 int64_t js, int64_t Vs,
                                                  const LzState* __restrict__ st,
-                                                 double2* __restrict__ out) {
+                                                 double2* __restrict__ out, int64_t os) {
   __shared__ double cr[LZ_MAX], ci[LZ_MAX];
+  V += blockIdx.y * Vs;
+  out += blockIdx.y * os;
+  st += blockIdx.y;
   const int nv = st->nonfinite ? 0 : st->nvalid;
   if (threadIdx.x < LZ_MAX) {
     cr[threadIdx.x] = st->coef_re[threadIdx.x];
@@ -303,7 +344,7 @@ __global__ __launch_bounds__(256) void k_combine(int64_t n, const double2* __res
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
     double sr = 0, si = 0;
     for (int j = 0; j < nv; ++j) {
-      const double2 v = V[(int64_t)j * n + i];
+      const double2 v = V[(int64_t)j * js + i];
       sr += cr[j] * v.x - ci[j] * v.y;
       si += cr[j] * v.y + ci[j] * v.x;
     }
@@ -313,6 +354,11 @@ __global__ __launch_bounds__(256) void k_combine(int64_t n, const double2* __res
 
 __global__ void k_export(const LzState* __restrict__ st, int32_t* info, double* alpha_beta,
                          double* energy, int numiter) {
+  const int b = blockIdx.x;
+  st += b;
+  if (info) info += 4 * b;
+  if (energy) energy += b;
+  if (alpha_beta) alpha_beta += 2 * numiter * b;
   const int t = threadIdx.x;
   if (t == 0) {
     if (info) {
@@ -340,45 +386,57 @@ static int64_t op_vec_len(const ptn_local_op* op) {
   return op->kind == PTN_OP_HEFF ? op->Dl * op->d * op->Dr : op->Dl * op->Dr;
 }
 
-static KrylovLayout krylov_layout(const ptn_local_op* op, int numiter) {
+static int red_width(int64_t n) {
+  int64_t rb = (n + 255) / 256;
+  if (rb > RED_BLOCKS) rb = RED_BLOCKS;
+  return rb < 1 ? 1 : (int)rb;
+}
+
+// Krylov vectors are stored as [j][problem][n], so that V_j of all problems is
+// one contiguous batch for the batched matvec
+static KrylovLayout krylov_layout(const ptn_local_op* op, int numiter, int64_t batch) {
   KrylovLayout k;
   const int64_t n = op_vec_len(op);
+  const int rb = red_width(n);
   size_t off = 0;
-  k.v = off; off += al((size_t)numiter * n * 16);
-  k.w = off; off += al((size_t)n * 16);
-  k.part_a = off; off += al(RED_BLOCKS * 8);
-  k.part_b = off; off += al(RED_BLOCKS * 8);
-  k.state = off; off += al(sizeof(LzState));
+  k.v = off; off += al((size_t)batch * numiter * n * 16);
+  k.w = off; off += al((size_t)batch * n * 16);
+  k.part_a = off; off += al((size_t)batch * rb * 8);
+  k.part_b = off; off += al((size_t)batch * rb * 8);
+  k.state = off; off += al((size_t)batch * sizeof(LzState));
   const bool lowp = (op->precision & ~PTN_BLOCK_SPARSE) != PTN_FP64;
   const int64_t nl = op->Dl * op->Ml * op->Dl, nr = op->Dr * op->Mr * op->Dr;
-  k.l64 = off; off += lowp ? al((size_t)nl * 8) : 0;
-  k.r64 = off; off += lowp ? al((size_t)nr * 8) : 0;
+  k.l64 = off; off += lowp ? al((size_t)batch * nl * 8) : 0;
+  k.r64 = off; off += lowp ? al((size_t)batch * nr * 8) : 0;
   k.mv = off;
   if (op->kind == PTN_OP_HEFF) {
     Dims D = {op->Dl, op->d, op->Dr, op->Ml, op->Mr};
-    off += al(heff_ws_bytes(op->precision, D));
+    off += al(heff_ws_bytes(op->precision, D, batch));
   } else {
-    off += al(bond_ws_bytes(op->precision, op->Dl, op->Dr, op->Ml));
+    off += al(bond_ws_bytes(op->precision, op->Dl, op->Dr, op->Ml, batch));
   }
   k.total = off;
   return k;
 }
 
-size_t krylov_ws_bytes(const ptn_local_op* op, int numiter) {
-  return krylov_layout(op, numiter).total;
+size_t krylov_ws_bytes(const ptn_local_op* op, int numiter, int64_t batch) {
+  return krylov_layout(op, numiter, batch).total;
 }
 
+// `batch` independent local problems of identical shape: l, w, r, v and out
+// stacked contiguously along a leading batch axis; info / energy /
+// alpha_beta per problem (4, 1, 2 numiter entries)
 int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, double dt_im,
                int numiter, void* out, double* energy, int32_t* info, double* alpha_beta,
-               void* ws, size_t ws_bytes, hipStream_t s) {
-  if (!op || numiter < 1 || numiter > LZ_MAX) return PTN_ERR_ARG;
+               void* ws, size_t ws_bytes, hipStream_t s, int64_t batch) {
+  if (!op || numiter < 1 || numiter > LZ_MAX || batch < 1 || batch > 65535) return PTN_ERR_ARG;
   if (op->kind != PTN_OP_HEFF && op->kind != PTN_OP_BOND) return PTN_ERR_ARG;
   const int bprec = op->precision & ~PTN_BLOCK_SPARSE;
   if (bprec != PTN_FP64 && bprec != PTN_FP32 && bprec != PTN_BF16X3) return PTN_ERR_UNSUPPORTED;
   if (op->Dl <= 0 || op->Dr <= 0 || op->Ml <= 0 || op->Mr <= 0) return PTN_ERR_ARG;
   if (op->kind == PTN_OP_HEFF && (op->d <= 0 || !op->w)) return PTN_ERR_ARG;
   if (op->kind == PTN_OP_BOND && op->Ml != op->Mr) return PTN_ERR_ARG;
-  const KrylovLayout K = krylov_layout(op, numiter);
+  const KrylovLayout K = krylov_layout(op, numiter, batch);
   if (ws_bytes < K.total) return PTN_ERR_WORKSPACE;
   char* base = (char*)ws;
   double2* V = (double2*)(base + K.v);
@@ -389,6 +447,8 @@ int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, do
   void* mvws = base + K.mv;
   const size_t mvbytes = K.total - K.mv;
   const int64_t n = op_vec_len(op);
+  const int rb = red_width(n);
+  const int64_t js = batch * n;   // stride between V_j and V_{j+1}
   const double thresh = 100.0 * (double)n * DBL_EPSILON;
 
   // environment blocks in the matvec's operand precision, converted once per solve
@@ -398,8 +458,8 @@ int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, do
   int st_;
   if (bprec != PTN_FP64) {
     const int64_t nl = op->Dl * op->Ml * op->Dl, nr = op->Dr * op->Mr * op->Dr;
-    if ((st_ = convert_c128_c64(nl, op->l, base + K.l64, s))) return st_;
-    if ((st_ = convert_c128_c64(nr, op->r, base + K.r64, s))) return st_;
+    if ((st_ = convert_c128_c64(batch * nl, op->l, base + K.l64, s))) return st_;
+    if ((st_ = convert_c128_c64(batch * nr, op->r, base + K.r64, s))) return st_;
     L = base + K.l64; R = base + K.r64;
     tl = tr = PTN_T_C64;
   }
@@ -407,36 +467,39 @@ int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, do
   auto matvec = [&](const double2* x, double2* y) -> int {
     if (op->kind == PTN_OP_HEFF)
       return heff_apply(op->precision, D, L, tl, op->w, R, tr, x, PTN_T_C128, y, PTN_T_C128,
-                        mvws, mvbytes, s);
+                        mvws, mvbytes, s, batch);
     return bond_apply(op->precision, op->Dl, op->Dr, op->Ml, L, tl, R, tr, x, PTN_T_C128, y,
-                      PTN_T_C128, mvws, mvbytes, s);
+                      PTN_T_C128, mvws, mvbytes, s, batch);
   };
   int64_t gblocks = (n + 255) / 256;
   if (gblocks > 2048) gblocks = 2048;
   if (gblocks < 1) gblocks = 1;
+  const unsigned B = (unsigned)batch;
 
-  k_state_init<<<1, 256, 0, s>>>(st, numiter);
-  k_sumsq_partial<<<RED_BLOCKS, 256, 0, s>>>(n, (const double2*)v, pa);
-  k_start<<<(unsigned)gblocks, 256, 0, s>>>(n, (const double2*)v, V, pa, st);
+  k_state_init<<<B, 256, 0, s>>>(st, numiter);
+  k_sumsq_partial<<<dim3(rb, B), 256, 0, s>>>(n, (const double2*)v, n, pa, rb);
+  k_start<<<dim3((unsigned)gblocks, B), 256, 0, s>>>(n, (const double2*)v, n, V, n, pa, rb, st);
   PTN_LAUNCH_CHECK();
   for (int j = 0; j < numiter - 1; ++j) {
-    double2* vj = V + (int64_t)j * n;
+    double2* vj = V + (int64_t)j * js;
     if ((st_ = matvec(vj, w))) return st_;
-    k_dot_partial<<<RED_BLOCKS, 256, 0, s>>>(n, w, vj, pa);
-    k_alpha_update<<<RED_BLOCKS, 256, 0, s>>>(n, w, vj, j > 0 ? vj - n : vj, j, pa, pb, st);
-    k_beta_next<<<(unsigned)gblocks, 256, 0, s>>>(n, w, vj + n, j, pb, thresh, st);
+    k_dot_partial<<<dim3(rb, B), 256, 0, s>>>(n, w, n, vj, n, pa, rb);
+    k_alpha_update<<<dim3(rb, B), 256, 0, s>>>(n, w, n, vj, j > 0 ? vj - js : vj, n, j, pa, pb,
+                                                rb, st);
+    k_beta_next<<<dim3((unsigned)gblocks, B), 256, 0, s>>>(n, w, n, vj + js, n, j, pb, rb,
+                                                            thresh, st);
     PTN_LAUNCH_CHECK();
   }
   {
     const int j = numiter - 1;
-    double2* vj = V + (int64_t)j * n;
+    double2* vj = V + (int64_t)j * js;
     if ((st_ = matvec(vj, w))) return st_;
-    k_dot_partial<<<RED_BLOCKS, 256, 0, s>>>(n, w, vj, pa);
-    k_alpha_last<<<1, 256, 0, s>>>(j, pa, st);
+    k_dot_partial<<<dim3(rb, B), 256, 0, s>>>(n, w, n, vj, n, pa, rb);
+    k_alpha_last<<<B, 256, 0, s>>>(j, pa, rb, st);
   }
-  k_tridiag<<<1, 64, 0, s>>>(st, mode, dt_re, dt_im);
-  k_combine<<<(unsigned)gblocks, 256, 0, s>>>(n, V, st, (double2*)out);
-  k_export<<<1, 64, 0, s>>>(st, info, alpha_beta, energy, numiter);
+  k_tridiag<<<B, 64, 0, s>>>(st, mode, dt_re, dt_im);
+  k_combine<<<dim3((unsigned)gblocks, B), 256, 0, s>>>(n, V, js, n, st, (double2*)out, n);
+  k_export<<<B, 64, 0, s>>>(st, info, alpha_beta, energy, numiter);
   PTN_LAUNCH_CHECK();
   return PTN_OK;
 }

@@ -10,12 +10,17 @@ namespace ptn {
 // workgroup-uniform branch. HBM-bound: reads ni and writes no complex values per (l, r).
 constexpr int MIX_MAX = 32;
 
+// blockIdx.z selects the problem of a batch (strides in complex elements).
 template <typename TX, typename TY>
 __global__ __launch_bounds__(256) void mode_mix_kernel(int64_t nl, int ni, int no, int64_t nr,
                                                        const double2* __restrict__ wm,
                                                        const TX* __restrict__ x,
-                                                       TY* __restrict__ y) {
+                                                       TY* __restrict__ y, int64_t bw,
+                                                       int64_t bx, int64_t by) {
   __shared__ double2 W[MIX_MAX * MIX_MAX];
+  wm += blockIdx.z * bw;
+  x += blockIdx.z * bx;
+  y += blockIdx.z * by;
   for (int t = threadIdx.x; t < ni * no; t += blockDim.x) W[t] = wm[t];
   __syncthreads();
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -44,27 +49,28 @@ __global__ __launch_bounds__(256) void mode_mix_kernel(int64_t nl, int ni, int n
 }
 
 int mode_mix(int64_t nl, int64_t ni, int64_t no, int64_t nr, const void* wm, const void* x, int tx,
-             void* y, int ty, hipStream_t s) {
+             void* y, int ty, hipStream_t s, int64_t batch, int64_t bw) {
   if (ni > MIX_MAX || no > MIX_MAX || ni < 1 || no < 1) return PTN_ERR_UNSUPPORTED;
-  if (nl == 0 || nr == 0) return PTN_OK;
-  if (nl > 65535) return PTN_ERR_ARG;
-  dim3 grid((unsigned)((nr + 255) / 256), (unsigned)nl);
+  if (nl == 0 || nr == 0 || batch == 0) return PTN_OK;
+  if (nl > 65535 || batch > 65535) return PTN_ERR_ARG;
+  dim3 grid((unsigned)((nr + 255) / 256), (unsigned)nl, (unsigned)batch);
+  const int64_t bx = nl * ni * nr, by = nl * no * nr;
   if (tx == PTN_T_C128 && ty == PTN_T_C128)
     mode_mix_kernel<double2, double2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
                                                            (const double2*)wm, (const double2*)x,
-                                                           (double2*)y);
+                                                           (double2*)y, bw, bx, by);
   else if (tx == PTN_T_C64 && ty == PTN_T_C64)
     mode_mix_kernel<float2, float2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
                                                          (const double2*)wm, (const float2*)x,
-                                                         (float2*)y);
+                                                         (float2*)y, bw, bx, by);
   else if (tx == PTN_T_C128 && ty == PTN_T_C64)
     mode_mix_kernel<double2, float2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
                                                           (const double2*)wm, (const double2*)x,
-                                                          (float2*)y);
+                                                          (float2*)y, bw, bx, by);
   else
     mode_mix_kernel<float2, double2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
                                                           (const double2*)wm, (const float2*)x,
-                                                          (double2*)y);
+                                                          (double2*)y, bw, bx, by);
   PTN_LAUNCH_CHECK();
   return PTN_OK;
 }

@@ -6,20 +6,22 @@ namespace ptn {
 
 struct Dims { int64_t Dl, d, Dr, Ml, Mr; };
 
-size_t heff_ws_bytes(int prec, const Dims& D);
-size_t env_ws_bytes(int prec, const Dims& D);
-size_t bond_ws_bytes(int prec, int64_t Dl, int64_t Dr, int64_t M);
+// `batch` independent problems of identical shape, every tensor stacked
+// contiguously along a leading batch axis (batch 1 = the single problem)
+size_t heff_ws_bytes(int prec, const Dims& D, int64_t batch = 1);
+size_t env_ws_bytes(int prec, const Dims& D, int64_t batch = 1);
+size_t bond_ws_bytes(int prec, int64_t Dl, int64_t Dr, int64_t M, int64_t batch = 1);
 
 int heff_apply(int prec, const Dims& D, const void* l, int tl, const void* w, const void* r, int tr,
                const void* a, int ta, void* out, int tout, void* ws, size_t ws_bytes,
-               hipStream_t s);
+               hipStream_t s, int64_t batch = 1);
 int bond_apply(int prec, int64_t Dl, int64_t Dr, int64_t M, const void* l, int tl, const void* r,
                int tr, const void* c, int tcin, void* out, int tout, void* ws, size_t ws_bytes,
-               hipStream_t s);
+               hipStream_t s, int64_t batch = 1);
 int env_right(int prec, const Dims& D, const void* a, const void* b, const void* w, const void* r,
-              int tr, void* out, void* ws, size_t ws_bytes, hipStream_t s);
+              int tr, void* out, void* ws, size_t ws_bytes, hipStream_t s, int64_t batch = 1);
 int env_left(int prec, const Dims& D, const void* a, const void* b, const void* w, const void* l,
-             int tl, void* out, void* ws, size_t ws_bytes, hipStream_t s);
+             int tl, void* out, void* ws, size_t ws_bytes, hipStream_t s, int64_t batch = 1);
 int convert_c128_c64(int64_t n, const void* x, void* y, hipStream_t s);
 
 }  // namespace ptn

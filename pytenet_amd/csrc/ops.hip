@@ -32,6 +32,8 @@ __global__ void transpose_w_kernel(int64_t rows, int64_t cols, const double2* __
                                    double2* __restrict__ wt) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= rows * cols) return;
+  w += blockIdx.y * rows * cols;     // problem of a batch
+  wt += blockIdx.y * rows * cols;
   int64_t i = t / cols, j = t % cols;
   wt[j * rows + i] = w[t];
 }
@@ -53,29 +55,29 @@ int convert_c128_c64(int64_t n, const void* x, void* y, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-size_t heff_ws_bytes(int prec, const Dims& D) {
+size_t heff_ws_bytes(int prec, const Dims& D, int64_t batch) {
   prec = base_prec(prec);
-  size_t t1 = (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
-  size_t t2 = (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
+  size_t t1 = (size_t)batch * (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
+  size_t t2 = (size_t)batch * (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
   return align256(t1) + align256(t2) +
          mask_bytes2(D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl, D.d * D.Dr, D.Dl * D.Ml) + 256;
 }
 
-size_t env_ws_bytes(int prec, const Dims& D) {
+size_t env_ws_bytes(int prec, const Dims& D, int64_t batch) {
   prec = base_prec(prec);
   // right step: T1 (Dl*d, Mr*Dr), T2 (Dl*Ml, d*Dr); left step: T1 (Dl*Ml, d*Dr),
   // T2 (Dl*d, Mr*Dr); plus the transposed MPO tensor
-  size_t a = (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
-  size_t b = (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
-  size_t wt = (size_t)(D.Ml * D.d * D.d * D.Mr) * 16;
+  size_t a = (size_t)batch * (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
+  size_t b = (size_t)batch * (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
+  size_t wt = (size_t)batch * (size_t)(D.Ml * D.d * D.d * D.Mr) * 16;
   const size_t mr = mask_bytes2(D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl * D.Ml, D.Dl, D.d * D.Dr);
   const size_t ml = mask_bytes2(D.Dl * D.Ml, D.d * D.Dr, D.Dl, D.Dr, D.Mr * D.Dr, D.Dl * D.d);
   return 2 * align256(a > b ? a : b) + align256(wt) + (mr > ml ? mr : ml) + 256;
 }
 
-size_t bond_ws_bytes(int prec, int64_t Dl, int64_t Dr, int64_t M) {
+size_t bond_ws_bytes(int prec, int64_t Dl, int64_t Dr, int64_t M, int64_t batch) {
   prec = base_prec(prec);
-  return align256((size_t)Dl * (size_t)(M * Dr) * csize(prec)) +
+  return align256((size_t)batch * (size_t)Dl * (size_t)(M * Dr) * csize(prec)) +
          mask_bytes2(Dl, M * Dr, Dr, Dl, Dr, Dl * M) + 256;
 }
 
@@ -88,125 +90,138 @@ static bool dims_ok(const Dims& D) {
 // complex64 as given by ta / tout; L and R have types tl / tr.
 int heff_apply(int prec, const Dims& D, const void* l, int tl, const void* w, const void* r, int tr,
                const void* a, int ta, void* out, int tout, void* ws, size_t ws_bytes,
-               hipStream_t s) {
-  if (!dims_ok(D)) return PTN_ERR_ARG;
-  if (ws_bytes < heff_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;
-  const bool sp = is_sparse(prec);
+               hipStream_t s, int64_t batch) {
+  if (!dims_ok(D) || batch < 1) return PTN_ERR_ARG;
+  if (ws_bytes < heff_ws_bytes(prec, D, batch)) return PTN_ERR_WORKSPACE;
+  const bool sp = is_sparse(prec) && batch == 1;
   prec = base_prec(prec);
   const int ct = ctype(prec), cp = gemm_compute(prec);
+  const int64_t n1 = (D.Dl * D.d) * (D.Mr * D.Dr), n2 = (D.Dl * D.Ml) * (D.d * D.Dr);
   char* p = (char*)ws;
   void* t1 = p;
-  p += align256((size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec));
+  p += align256((size_t)batch * n1 * csize(prec));
   void* t2 = p;
-  p += align256((size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec));
+  p += align256((size_t)batch * n2 * csize(prec));
   const size_t mb = mask_bytes2(D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl, D.d * D.Dr, D.Dl * D.Ml);
   void* mws = sp ? p : nullptr;
+  const int64_t nA = D.Dl * D.d * D.Dr, nL = D.Dl * D.Ml * D.Dl, nR = D.Dr * D.Mr * D.Dr;
   int st;
   ZGemm g1 = {D.Dl * D.d, D.Mr * D.Dr, D.Dr,
               a, D.Dr, 1, 0,
               r, D.Mr * D.Dr, 1, 0,
-              t1, D.Mr * D.Dr, 1, 0, 1, 0, 0, 0};
+              t1, D.Mr * D.Dr, 1, 0, batch, nA, nR, n1};
   if ((st = zgemm(g1, cp, ta, tr, ct, mws, mb, s))) return st;
-  if ((st = mode_mix(D.Dl, D.d * D.Mr, D.Ml * D.d, D.Dr, w, t1, ct, t2, ct, s))) return st;
+  if ((st = mode_mix(D.Dl, D.d * D.Mr, D.Ml * D.d, D.Dr, w, t1, ct, t2, ct, s, batch,
+                     D.Ml * D.d * D.d * D.Mr)))
+    return st;
   ZGemm g3 = {D.Dl, D.d * D.Dr, D.Dl * D.Ml,
               l, 1, D.Dl, 0,
               t2, D.d * D.Dr, 1, 0,
-              out, D.d * D.Dr, 1, 0, 1, 0, 0, 0};
+              out, D.d * D.Dr, 1, 0, batch, nL, n2, nA};
   return zgemm(g3, cp, tl, ct, tout, mws, mb, s);
 }
 
 int env_right(int prec, const Dims& D, const void* a, const void* b, const void* w, const void* r,
-              int tr, void* out, void* ws, size_t ws_bytes, hipStream_t s) {
-  if (!dims_ok(D)) return PTN_ERR_ARG;
-  if (ws_bytes < env_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;
-  const bool sp = is_sparse(prec);
+              int tr, void* out, void* ws, size_t ws_bytes, hipStream_t s, int64_t batch) {
+  if (!dims_ok(D) || batch < 1) return PTN_ERR_ARG;
+  if (ws_bytes < env_ws_bytes(prec, D, batch)) return PTN_ERR_WORKSPACE;
+  const bool sp = is_sparse(prec) && batch == 1;
   prec = base_prec(prec);
   const int ct = ctype(prec), cp = gemm_compute(prec);
-  size_t ta = (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
-  size_t tb = (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
+  const int64_t n1 = (D.Dl * D.d) * (D.Mr * D.Dr), n2 = (D.Dl * D.Ml) * (D.d * D.Dr);
+  size_t ta = (size_t)batch * n1 * csize(prec);
+  size_t tb = (size_t)batch * n2 * csize(prec);
   char* p = (char*)ws;
   void* t1 = p;
   p += align256(ta > tb ? ta : tb);
   void* t2 = p;
-  const size_t wtb = align256((size_t)(D.Ml * D.d * D.d * D.Mr) * 16);
+  const size_t wtb = align256((size_t)batch * (size_t)(D.Ml * D.d * D.d * D.Mr) * 16);
   const size_t mb = mask_bytes2(D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl * D.Ml, D.Dl, D.d * D.Dr);
   void* mws = sp ? (char*)ws + 2 * align256(ta > tb ? ta : tb) + wtb : nullptr;
+  const int64_t nA = D.Dl * D.d * D.Dr, nR = D.Dr * D.Mr * D.Dr, nO = D.Dl * D.Ml * D.Dl;
   int st;
   ZGemm g1 = {D.Dl * D.d, D.Mr * D.Dr, D.Dr,
               a, D.Dr, 1, 0,
               r, D.Mr * D.Dr, 1, 0,
-              t1, D.Mr * D.Dr, 1, 0, 1, 0, 0, 0};
+              t1, D.Mr * D.Dr, 1, 0, batch, nA, nR, n1};
   if ((st = zgemm(g1, cp, PTN_T_C128, tr, ct, mws, mb, s))) return st;
-  if ((st = mode_mix(D.Dl, D.d * D.Mr, D.Ml * D.d, D.Dr, w, t1, ct, t2, ct, s))) return st;
+  if ((st = mode_mix(D.Dl, D.d * D.Mr, D.Ml * D.d, D.Dr, w, t1, ct, t2, ct, s, batch,
+                     D.Ml * D.d * D.d * D.Mr)))
+    return st;
   // out[(l, m), lb] = sum_{(sb, rb)} T2[(l, m), (sb, rb)] conj(B[lb, (sb, rb)])
   ZGemm g3 = {D.Dl * D.Ml, D.Dl, D.d * D.Dr,
               t2, D.d * D.Dr, 1, 0,
               b, 1, D.d * D.Dr, 1,
-              out, D.Dl, 1, 0, 1, 0, 0, 0};
+              out, D.Dl, 1, 0, batch, n2, nA, nO};
   return zgemm(g3, cp, ct, PTN_T_C128, PTN_T_C128, mws, mb, s);
 }
 
 int env_left(int prec, const Dims& D, const void* a, const void* b, const void* w, const void* l,
-             int tl, void* out, void* ws, size_t ws_bytes, hipStream_t s) {
-  if (!dims_ok(D)) return PTN_ERR_ARG;
-  if (ws_bytes < env_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;
-  const bool sp = is_sparse(prec);
+             int tl, void* out, void* ws, size_t ws_bytes, hipStream_t s, int64_t batch) {
+  if (!dims_ok(D) || batch < 1) return PTN_ERR_ARG;
+  if (ws_bytes < env_ws_bytes(prec, D, batch)) return PTN_ERR_WORKSPACE;
+  const bool sp = is_sparse(prec) && batch == 1;
   prec = base_prec(prec);
   const int ct = ctype(prec), cp = gemm_compute(prec);
-  size_t ta = (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
-  size_t tb = (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
+  const int64_t n1 = (D.Dl * D.Ml) * (D.d * D.Dr), n2 = (D.Dl * D.d) * (D.Mr * D.Dr);
+  size_t ta = (size_t)batch * n2 * csize(prec);
+  size_t tb = (size_t)batch * n1 * csize(prec);
   size_t big = align256(ta > tb ? ta : tb);
   char* p = (char*)ws;
   void* t1 = p;
   void* t2 = p + big;
   void* wt = p + 2 * big;
-  const size_t wtb = align256((size_t)(D.Ml * D.d * D.d * D.Mr) * 16);
+  const int64_t nW = D.Ml * D.d * D.d * D.Mr;
+  const size_t wtb = align256((size_t)batch * (size_t)nW * 16);
   const size_t mb = mask_bytes2(D.Dl * D.Ml, D.d * D.Dr, D.Dl, D.Dr, D.Mr * D.Dr, D.Dl * D.d);
   void* mws = sp ? p + 2 * big + wtb : nullptr;
   int64_t rows = D.Ml * D.d, cols = D.d * D.Mr;
-  transpose_w_kernel<<<(unsigned)((rows * cols + 255) / 256), 256, 0, s>>>(
+  transpose_w_kernel<<<dim3((unsigned)((rows * cols + 255) / 256), (unsigned)batch), 256, 0, s>>>(
       rows, cols, (const double2*)w, (double2*)wt);
   PTN_LAUNCH_CHECK();
+  const int64_t nA = D.Dl * D.d * D.Dr, nL = D.Dl * D.Ml * D.Dl, nO = D.Dr * D.Mr * D.Dr;
   int st;
   // T1[(l, m), (sb, rb)] = sum_lb L[(l, m), lb] conj(B[lb, (sb, rb)])
   ZGemm g1 = {D.Dl * D.Ml, D.d * D.Dr, D.Dl,
               l, D.Dl, 1, 0,
               b, D.d * D.Dr, 1, 1,
-              t1, D.d * D.Dr, 1, 0, 1, 0, 0, 0};
+              t1, D.d * D.Dr, 1, 0, batch, nL, nA, n1};
   if ((st = zgemm(g1, cp, tl, PTN_T_C128, ct, mws, mb, s))) return st;
   // T2[l, (s, m'), rb] = sum_{(m, sb)} W[m, sb, s, m'] T1[l, (m, sb), rb]
-  if ((st = mode_mix(D.Dl, D.Ml * D.d, D.d * D.Mr, D.Dr, wt, t1, ct, t2, ct, s))) return st;
+  if ((st = mode_mix(D.Dl, D.Ml * D.d, D.d * D.Mr, D.Dr, wt, t1, ct, t2, ct, s, batch, nW)))
+    return st;
   // out[r, (m', rb)] = sum_{(l, s)} A[(l, s), r] T2[(l, s), (m', rb)]
   ZGemm g3 = {D.Dr, D.Mr * D.Dr, D.Dl * D.d,
               a, 1, D.Dr, 0,
               t2, D.Mr * D.Dr, 1, 0,
-              out, D.Mr * D.Dr, 1, 0, 1, 0, 0, 0};
+              out, D.Mr * D.Dr, 1, 0, batch, nA, n2, nO};
   return zgemm(g3, cp, PTN_T_C128, ct, PTN_T_C128, mws, mb, s);
 }
 
 int bond_apply(int prec, int64_t Dl, int64_t Dr, int64_t M, const void* l, int tl, const void* r,
                int tr, const void* c, int tcin, void* out, int tout, void* ws, size_t ws_bytes,
-               hipStream_t s) {
-  if (Dl <= 0 || Dr <= 0 || M <= 0) return PTN_ERR_ARG;
-  if (ws_bytes < bond_ws_bytes(prec, Dl, Dr, M)) return PTN_ERR_WORKSPACE;
-  const bool sp = is_sparse(prec);
+               hipStream_t s, int64_t batch) {
+  if (Dl <= 0 || Dr <= 0 || M <= 0 || batch < 1) return PTN_ERR_ARG;
+  if (ws_bytes < bond_ws_bytes(prec, Dl, Dr, M, batch)) return PTN_ERR_WORKSPACE;
+  const bool sp = is_sparse(prec) && batch == 1;
   prec = base_prec(prec);
   const int ct = ctype(prec), cp = gemm_compute(prec);
   void* t = ws;
   const size_t mb = mask_bytes2(Dl, M * Dr, Dr, Dl, Dr, Dl * M);
   void* mws = sp ? (char*)ws + align256((size_t)Dl * (size_t)(M * Dr) * csize(prec)) : nullptr;
+  const int64_t nT = Dl * M * Dr, nC = Dl * Dr, nL = Dl * M * Dl, nR = Dr * M * Dr;
   int st;
   // T[l, (m, rb)] = sum_r C[l, r] R[r, (m, rb)]
   ZGemm g1 = {Dl, M * Dr, Dr,
               c, Dr, 1, 0,
               r, M * Dr, 1, 0,
-              t, M * Dr, 1, 0, 1, 0, 0, 0};
+              t, M * Dr, 1, 0, batch, nC, nR, nT};
   if ((st = zgemm(g1, cp, tcin, tr, ct, mws, mb, s))) return st;
   // out[lb, rb] = sum_{(l, m)} L[(l, m), lb] T[(l, m), rb]
   ZGemm g3 = {Dl, Dr, Dl * M,
               l, 1, Dl, 0,
               t, Dr, 1, 0,
-              out, Dr, 1, 0, 1, 0, 0, 0};
+              out, Dr, 1, 0, batch, nL, nT, nC};
   return zgemm(g3, cp, tl, ct, tout, mws, mb, s);
 }
 

@@ -16,20 +16,29 @@ constexpr int QR_SPLITK = 8;    // K chunks of the V^H C products
 //   T     per panel the QR_NB x QR_NB upper-triangular factor of the compact
 //         WY form H_j0 ... H_j0+nb-1 = I - V T V^H (row-major, zero padded)
 // plus scratch for the blocked updates (Vf^H C partials, the reduced product).
-struct QrFact {
-  int64_t m, n, k, npan, ncmax;
-  double2 *W, *Vf, *tau, *rdiag, *rsup, *T, *G, *P, *Z;
+// Per-matrix strides (complex elements) of a batch of factorisations of
+// identical shape; matrix b of a batch uses the arrays offset by b * stride.
+struct QrStrides {
+  int64_t W, V, k, T, G, P, Z;
 };
 
-size_t qr_fact_bytes(int64_t m, int64_t n, int64_t ncmax);
-// carve the buffers of a factorisation out of `ws` (qr_fact_bytes(m, n, ncmax) bytes)
-QrFact qr_fact_layout(int64_t m, int64_t n, int64_t ncmax, void* ws);
-// factor the row-major m x n matrix `a` (leading dimension lda)
-int qr_factor(QrFact& f, const double2* a, int64_t lda, hipStream_t s);
-// C <- Q C for the column-major m x nc matrix C (leading dimension ldc, nc <= ncmax),
-// Q = H_0 H_1 ... H_{k-1}. With `identity_tail` the caller guarantees that C
-// starts as [I_k; 0] (nc == k), so panel j0 only needs columns j0 .. k-1.
+struct QrFact {
+  int64_t m, n, k, npan, ncmax, batch;
+  double2 *W, *Vf, *tau, *rdiag, *rsup, *T, *G, *P, *Z;
+  QrStrides bs;
+};
+
+size_t qr_fact_bytes(int64_t m, int64_t n, int64_t ncmax, int64_t batch = 1);
+// carve the buffers of a factorisation (or of `batch` of them) out of `ws`
+QrFact qr_fact_layout(int64_t m, int64_t n, int64_t ncmax, void* ws, int64_t batch = 1);
+// factor the row-major m x n matrix `a` (leading dimension lda; matrix b of a
+// batch at a + b * sa)
+int qr_factor(QrFact& f, const double2* a, int64_t lda, hipStream_t s, int64_t sa = 0);
+// C <- Q C for the column-major m x nc matrix C (leading dimension ldc, nc <= ncmax;
+// matrix b of a batch at C + b * sc), Q = H_0 H_1 ... H_{k-1}. With
+// `identity_tail` the caller guarantees that C starts as [I_k; 0] (nc == k), so
+// panel j0 only needs columns j0 .. k-1.
 int qr_apply_q(const QrFact& f, double2* C, int64_t ldc, int64_t nc, bool identity_tail,
-               hipStream_t s);
+               hipStream_t s, int64_t sc = 0);
 
 }  // namespace ptn

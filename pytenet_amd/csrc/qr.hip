@@ -94,9 +94,11 @@ __device__ __forceinline__ double2 conj2(double2 a) { return make_double2(a.x, -
 
 // column-major copy: W[j*m + i] = A[i*lda + j]
 __global__ void k_to_colmajor(int64_t m, int64_t n, const double2* __restrict__ a, int64_t lda,
-                              double2* __restrict__ w) {
+                              double2* __restrict__ w, int64_t sa, int64_t sw) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= m * n) return;
+  a += blockIdx.y * sa;
+  w += blockIdx.y * sw;
   int64_t i = t / n, j = t % n;
   w[j * m + i] = a[i * lda + j];
 }
@@ -107,7 +109,11 @@ __global__ void k_to_colmajor(int64_t m, int64_t n, const double2* __restrict__ 
 __global__ __launch_bounds__(256) void k_qr_col(int64_t m, int64_t j, double2* __restrict__ W,
                                                 double2* __restrict__ Vf,
                                                 double2* __restrict__ tau,
-                                                double2* __restrict__ rdiag) {
+                                                double2* __restrict__ rdiag, QrStrides bs) {
+  W += blockIdx.y * bs.W;
+  Vf += blockIdx.y * bs.V;
+  tau += blockIdx.y * bs.k;
+  rdiag += blockIdx.y * bs.k;
   const double2* xj = W + j * m;
   const double2 alpha = xj[j];
   double s = 0;
@@ -192,7 +198,12 @@ __global__ __launch_bounds__(256) void k_qr_col2(int64_t m, int64_t j, double2* 
                                                  double2* __restrict__ Vf,
                                                  double2* __restrict__ tau,
                                                  double2* __restrict__ rdiag,
-                                                 double2* __restrict__ rsup) {
+                                                 double2* __restrict__ rsup, QrStrides bs) {
+  W += blockIdx.y * bs.W;
+  Vf += blockIdx.y * bs.V;
+  tau += blockIdx.y * bs.k;
+  rdiag += blockIdx.y * bs.k;
+  rsup += blockIdx.y * bs.k;
   const double2* xj = W + j * m;
   const double2* xk = W + (j + 1) * m;
   const int b = blockIdx.x;
@@ -279,7 +290,9 @@ __global__ __launch_bounds__(256) void k_qr_col2(int64_t m, int64_t j, double2* 
 
 // R[jj][jj+1] of the two-reflector launches of a panel, written after the panel
 __global__ void k_qr_fixsup(int64_t m, int64_t j0, int64_t pe, const double2* __restrict__ rsup,
-                            double2* __restrict__ W) {
+                            double2* __restrict__ W, QrStrides bs) {
+  rsup += blockIdx.y * bs.k;
+  W += blockIdx.y * bs.W;
   const int64_t jj = j0 + 2 * (int64_t)threadIdx.x;
   if (jj + 1 < pe) W[(jj + 1) * m + jj] = rsup[jj];
 }
@@ -287,7 +300,9 @@ __global__ void k_qr_fixsup(int64_t m, int64_t j0, int64_t pe, const double2* __
 // G[a][b] = v_a^H v_b for the panel's reflectors (rows j0 .. m-1), one workgroup per entry
 __global__ __launch_bounds__(256) void k_qr_vtv(int64_t m, int64_t j0, int nbp,
                                                 const double2* __restrict__ Vf,
-                                                double2* __restrict__ G) {
+                                                double2* __restrict__ G, QrStrides bs) {
+  Vf += blockIdx.y * bs.V;
+  G += blockIdx.y * bs.G;
   const int a = blockIdx.x / nbp, b = blockIdx.x % nbp;
   if (b <= a) return;   // only the strict upper triangle is used
   const double2* va = Vf + (j0 + a) * m;
@@ -308,7 +323,10 @@ __global__ __launch_bounds__(256) void k_qr_vtv(int64_t m, int64_t j0, int nbp,
 // so the column recurrence needs no barriers.
 __global__ __launch_bounds__(64) void k_qr_tmat(int nbp, const double2* __restrict__ G,
                                                 const double2* __restrict__ tau,
-                                                double2* __restrict__ T) {
+                                                double2* __restrict__ T, QrStrides bs) {
+  G += blockIdx.x * bs.G;
+  tau += blockIdx.x * bs.k;
+  T += blockIdx.x * bs.T;
   __shared__ double2 Gs[QR_NB][QR_NB + 1];
   __shared__ double2 taus[QR_NB];
   const int r = threadIdx.x;
@@ -348,7 +366,10 @@ __global__ __launch_bounds__(64) void k_qr_tmat(int nbp, const double2* __restri
 __global__ __launch_bounds__(256) void k_qr_reduce(int nbp, int64_t nc, int nparts,
                                                    const double2* __restrict__ P,
                                                    const double2* __restrict__ T, int adjoint,
-                                                   double2* __restrict__ Z) {
+                                                   double2* __restrict__ Z, QrStrides bs) {
+  P += blockIdx.y * bs.P;
+  T += blockIdx.y * bs.T;
+  Z += blockIdx.y * bs.Z;
   __shared__ double2 Ts[QR_NB][QR_NB + 1];
   __shared__ double2 Ys[8][QR_NB + 1];
   const int t = threadIdx.x;
@@ -387,68 +408,100 @@ __global__ __launch_bounds__(256) void k_qr_reduce(int nbp, int64_t nc, int npar
 
 static inline size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
 
-size_t qr_fact_bytes(int64_t m, int64_t n, int64_t ncmax) {
+static QrStrides qr_strides(int64_t m, int64_t n, int64_t ncmax) {
   const int64_t k = m < n ? m : n;
   const int64_t npan = (k + QR_NB - 1) / QR_NB;
   const int64_t nc = ncmax > n ? ncmax : n;
-  return al((size_t)m * n * 16) + al((size_t)m * k * 16) + 3 * al((size_t)(k + 1) * 16) +
-         al((size_t)(npan + 1) * QR_NB * QR_NB * 16) + al((size_t)QR_NB * QR_NB * 16) +
-         al((size_t)(QR_SPLITK + 1) * QR_NB * nc * 16) + al((size_t)QR_NB * nc * 16);
+  QrStrides b;
+  b.W = m * n;
+  b.V = m * k;
+  b.k = k + 1;
+  b.T = (npan + 1) * QR_NB * QR_NB;
+  b.G = QR_NB * QR_NB;
+  b.P = (QR_SPLITK + 1) * QR_NB * nc;
+  b.Z = QR_NB * nc;
+  return b;
 }
 
-QrFact qr_fact_layout(int64_t m, int64_t n, int64_t ncmax, void* ws) {
+size_t qr_fact_bytes(int64_t m, int64_t n, int64_t ncmax, int64_t batch) {
+  const QrStrides b = qr_strides(m, n, ncmax);
+  const size_t B = (size_t)batch;
+  return al(B * b.W * 16) + al(B * b.V * 16) + 3 * al(B * b.k * 16) + al(B * b.T * 16) +
+         al(B * b.G * 16) + al(B * b.P * 16) + al(B * b.Z * 16);
+}
+
+QrFact qr_fact_layout(int64_t m, int64_t n, int64_t ncmax, void* ws, int64_t batch) {
   QrFact f;
   f.m = m; f.n = n; f.k = m < n ? m : n;
   f.npan = (f.k + QR_NB - 1) / QR_NB;
   f.ncmax = ncmax > n ? ncmax : n;
+  f.batch = batch;
+  f.bs = qr_strides(m, n, ncmax);
+  const size_t B = (size_t)batch;
   char* p = (char*)ws;
-  f.W = (double2*)p; p += al((size_t)m * n * 16);
-  f.Vf = (double2*)p; p += al((size_t)m * f.k * 16);
-  f.tau = (double2*)p; p += al((size_t)(f.k + 1) * 16);
-  f.rdiag = (double2*)p; p += al((size_t)(f.k + 1) * 16);
-  f.rsup = (double2*)p; p += al((size_t)(f.k + 1) * 16);
-  f.T = (double2*)p; p += al((size_t)(f.npan + 1) * QR_NB * QR_NB * 16);
-  f.G = (double2*)p; p += al((size_t)QR_NB * QR_NB * 16);
-  f.P = (double2*)p; p += al((size_t)(QR_SPLITK + 1) * QR_NB * f.ncmax * 16);
+  f.W = (double2*)p; p += al(B * f.bs.W * 16);
+  f.Vf = (double2*)p; p += al(B * f.bs.V * 16);
+  f.tau = (double2*)p; p += al(B * f.bs.k * 16);
+  f.rdiag = (double2*)p; p += al(B * f.bs.k * 16);
+  f.rsup = (double2*)p; p += al(B * f.bs.k * 16);
+  f.T = (double2*)p; p += al(B * f.bs.T * 16);
+  f.G = (double2*)p; p += al(B * f.bs.G * 16);
+  f.P = (double2*)p; p += al(B * f.bs.P * 16);
   f.Z = (double2*)p;
   return f;
 }
 
 // C[j0:m, c0:c0+nc] += V_panel Z with Z = -T^(H) V_panel^H C[j0:m, c0:c0+nc]
-// (C column-major with leading dimension ldc; panel columns j0 .. j0+nbp-1)
+// (C column-major with leading dimension ldc; panel columns j0 .. j0+nbp-1;
+// matrix b of a batch at C + b * sc). A single matrix splits the K = m - j0
+// rows of V^H C into chunks (batched product) for parallelism; a batch of
+// matrices is batched over the matrices instead.
 static int apply_block(const QrFact& f, int64_t j0, int nbp, const double2* Tp, int adjoint,
-                       double2* C, int64_t ldc, int64_t c0, int64_t nc, hipStream_t s) {
+                       double2* C, int64_t ldc, int64_t c0, int64_t nc, hipStream_t s,
+                       int64_t sc) {
   const int64_t m = f.m, mr = m - j0;
   if (nc <= 0 || mr <= 0) return PTN_OK;
   const double2* V = f.Vf + j0 * m + j0;
   double2* Cs = C + c0 * ldc + j0;
   int st;
-  // partials P[s] = V^H C over K chunks of `chunk` rows
-  int64_t chunk = (mr + QR_SPLITK - 1) / QR_SPLITK;
-  chunk = ((chunk + 7) / 8) * 8;
-  const int64_t nfull = mr / chunk, rem = mr - nfull * chunk;
-  if (nfull > 0) {
-    ZGemm g = {nbp, nc, chunk, V, m, 1, 1, Cs, 1, ldc, 0, f.P, nc, 1, 0,
-               nfull, chunk, chunk, (int64_t)QR_NB * nc};
+  int nparts;
+  if (f.batch == 1) {
+    // partials P[s] = V^H C over K chunks of `chunk` rows
+    int64_t chunk = (mr + QR_SPLITK - 1) / QR_SPLITK;
+    chunk = ((chunk + 7) / 8) * 8;
+    const int64_t nfull = mr / chunk, rem = mr - nfull * chunk;
+    if (nfull > 0) {
+      ZGemm g = {nbp, nc, chunk, V, m, 1, 1, Cs, 1, ldc, 0, f.P, nc, 1, 0,
+                 nfull, chunk, chunk, (int64_t)QR_NB * nc};
+      if ((st = zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, s))) return st;
+    }
+    if (rem > 0) {
+      ZGemm g = {nbp, nc, rem, V + nfull * chunk, m, 1, 1, Cs + nfull * chunk, 1, ldc, 0,
+                 f.P + nfull * QR_NB * nc, nc, 1, 0, 1, 0, 0, 0};
+      if ((st = zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, s))) return st;
+    }
+    nparts = (int)(nfull + (rem > 0 ? 1 : 0));
+  } else {
+    ZGemm g = {nbp, nc, mr, V, m, 1, 1, Cs, 1, ldc, 0, f.P, nc, 1, 0,
+               f.batch, f.bs.V, sc, f.bs.P};
     if ((st = zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, s))) return st;
+    nparts = 1;
   }
-  if (rem > 0) {
-    ZGemm g = {nbp, nc, rem, V + nfull * chunk, m, 1, 1, Cs + nfull * chunk, 1, ldc, 0,
-               f.P + nfull * QR_NB * nc, nc, 1, 0, 1, 0, 0, 0};
-    if ((st = zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, s))) return st;
-  }
-  const int nparts = (int)(nfull + (rem > 0 ? 1 : 0));
-  k_qr_reduce<<<(unsigned)((nc + 7) / 8), 256, 0, s>>>(nbp, nc, nparts, f.P, Tp, adjoint, f.Z);
+  k_qr_reduce<<<dim3((unsigned)((nc + 7) / 8), (unsigned)f.batch), 256, 0, s>>>(
+      nbp, nc, nparts, f.P, Tp, adjoint, f.Z, f.bs);
   PTN_LAUNCH_CHECK();
   // C += V Z
-  ZGemm g = {mr, nc, nbp, V, 1, m, 0, f.Z, nc, 1, 0, Cs, 1, ldc, 1, 1, 0, 0, 0};
+  ZGemm g = {mr, nc, nbp, V, 1, m, 0, f.Z, nc, 1, 0, Cs, 1, ldc, 1, f.batch, f.bs.V, f.bs.Z, sc};
   return zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, s);
 }
 
-int qr_factor(QrFact& f, const double2* a, int64_t lda, hipStream_t s) {
+int qr_factor(QrFact& f, const double2* a, int64_t lda, hipStream_t s, int64_t sa) {
   const int64_t m = f.m, n = f.n, k = f.k;
   if (m == 0 || n == 0) return PTN_OK;
-  k_to_colmajor<<<(unsigned)((m * n + 255) / 256), 256, 0, s>>>(m, n, a, lda, f.W);
+  if (f.batch < 1 || f.batch > 65535) return PTN_ERR_ARG;
+  const unsigned B = (unsigned)f.batch;
+  k_to_colmajor<<<dim3((unsigned)((m * n + 255) / 256), B), 256, 0, s>>>(m, n, a, lda, f.W, sa,
+                                                                           f.bs.W);
   PTN_LAUNCH_CHECK();
   for (int64_t p = 0; p < f.npan; ++p) {
     const int64_t j0 = p * QR_NB;
@@ -456,50 +509,59 @@ int qr_factor(QrFact& f, const double2* a, int64_t lda, hipStream_t s) {
     const int nbp = (int)(pe - j0);
     int64_t j = j0;
     for (; j + 1 < pe; j += 2)
-      k_qr_col2<<<(unsigned)(pe - j), 256, 0, s>>>(m, j, f.W, f.Vf, f.tau, f.rdiag, f.rsup);
-    if (j < pe) k_qr_col<<<(unsigned)(pe - j), 256, 0, s>>>(m, j, f.W, f.Vf, f.tau, f.rdiag);
-    if (pe - j0 >= 2) k_qr_fixsup<<<1, QR_NB / 2, 0, s>>>(m, j0, pe, f.rsup, f.W);
+      k_qr_col2<<<dim3((unsigned)(pe - j), B), 256, 0, s>>>(m, j, f.W, f.Vf, f.tau, f.rdiag,
+                                                             f.rsup, f.bs);
+    if (j < pe)
+      k_qr_col<<<dim3((unsigned)(pe - j), B), 256, 0, s>>>(m, j, f.W, f.Vf, f.tau, f.rdiag, f.bs);
+    if (pe - j0 >= 2) k_qr_fixsup<<<dim3(1, B), QR_NB / 2, 0, s>>>(m, j0, pe, f.rsup, f.W, f.bs);
     PTN_LAUNCH_CHECK();
     double2* Tp = f.T + p * QR_NB * QR_NB;
-    k_qr_vtv<<<(unsigned)(nbp * nbp), 256, 0, s>>>(m, j0, nbp, f.Vf, f.G);
-    k_qr_tmat<<<1, 64, 0, s>>>(nbp, f.G, f.tau + j0, Tp);
+    k_qr_vtv<<<dim3((unsigned)(nbp * nbp), B), 256, 0, s>>>(m, j0, nbp, f.Vf, f.G, f.bs);
+    k_qr_tmat<<<B, 64, 0, s>>>(nbp, f.G, f.tau + j0, Tp, f.bs);
     PTN_LAUNCH_CHECK();
     int st;
-    if (pe < n && (st = apply_block(f, j0, nbp, Tp, 1, f.W, m, pe, n - pe, s))) return st;
+    if (pe < n && (st = apply_block(f, j0, nbp, Tp, 1, f.W, m, pe, n - pe, s, f.bs.W))) return st;
   }
   return PTN_OK;
 }
 
 int qr_apply_q(const QrFact& f, double2* C, int64_t ldc, int64_t nc, bool identity_tail,
-               hipStream_t s) {
+               hipStream_t s, int64_t sc) {
   if (nc > f.ncmax) return PTN_ERR_ARG;
   for (int64_t p = f.npan - 1; p >= 0; --p) {
     const int64_t j0 = p * QR_NB;
     const int64_t pe = (j0 + QR_NB < f.k) ? j0 + QR_NB : f.k;
     const int64_t c0 = identity_tail ? j0 : 0;
     int st;
-    if ((st = apply_block(f, j0, (int)(pe - j0), f.T + p * QR_NB * QR_NB, 0, C, ldc, c0,
-                          nc - c0, s)))
+    if ((st = apply_block(f, j0, (int)(pe - j0), f.T + p * QR_NB * QR_NB, 0, C, ldc, c0, nc - c0,
+                          s, sc)))
       return st;
   }
   return PTN_OK;
 }
 
-// column-major m x k identity columns
+// column-major m x k identity columns (one matrix per blockIdx.y, stride m * k)
 __global__ void k_q_init(int64_t m, int64_t k, double2* __restrict__ Qc) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= m * k) return;
+  Qc += blockIdx.y * m * k;
   int64_t c = t / m, i = t % m;
   Qc[t] = make_double2(i == c ? 1.0 : 0.0, 0.0);
 }
 
 // Q (row-major, ldq) from Qc; R (row-major k x n, ldr) from the upper triangle of W
-// with the diagonal from rdiag
+// with the diagonal from rdiag; matrix b of a batch at q + b * sq, r + b * sr
 __global__ void k_qr_out(int64_t m, int64_t n, int64_t k, const double2* __restrict__ W,
                          const double2* __restrict__ rdiag, const double2* __restrict__ Qc,
                          double2* __restrict__ q, int64_t ldq, double2* __restrict__ r,
-                         int64_t ldr) {
+                         int64_t ldr, QrStrides bs, int64_t sq, int64_t sr) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t b = blockIdx.y;
+  W += b * bs.W;
+  rdiag += b * bs.k;
+  Qc += b * m * k;
+  q += b * sq;
+  r += b * sr;
   if (t < m * k) {
     int64_t i = t / k, c = t % k;
     q[i * ldq + c] = Qc[c * m + i];
@@ -510,27 +572,31 @@ __global__ void k_qr_out(int64_t m, int64_t n, int64_t k, const double2* __restr
   }
 }
 
-size_t qr_ws_bytes(int64_t m, int64_t n) {
+size_t qr_ws_bytes(int64_t m, int64_t n, int64_t batch) {
   int64_t k = m < n ? m : n;
-  return al(qr_fact_bytes(m, n, n)) + al((size_t)m * k * 16);
+  return al(qr_fact_bytes(m, n, n, batch)) + al((size_t)batch * m * k * 16);
 }
 
+// `batch` row-major m x n matrices stacked contiguously (a: lda = n; q: m x k,
+// ldq = k; r: k x n, ldr = n) when batch > 1
 int qr_run(int64_t m, int64_t n, const void* a, int64_t lda, void* q, int64_t ldq, void* r,
-           int64_t ldr, void* ws, size_t ws_bytes, hipStream_t s) {
+           int64_t ldr, void* ws, size_t ws_bytes, hipStream_t s, int64_t batch) {
   if (m < 0 || n < 0 || lda < n || ldq < (m < n ? m : n) || ldr < n) return PTN_ERR_ARG;
-  if (ws_bytes < qr_ws_bytes(m, n)) return PTN_ERR_WORKSPACE;
+  if (batch < 1 || batch > 65535) return PTN_ERR_ARG;
+  if (ws_bytes < qr_ws_bytes(m, n, batch)) return PTN_ERR_WORKSPACE;
   const int64_t k = m < n ? m : n;
   if (m == 0 || n == 0) return PTN_OK;
-  QrFact f = qr_fact_layout(m, n, n, ws);
-  double2* Qc = (double2*)((char*)ws + al(qr_fact_bytes(m, n, n)));
+  QrFact f = qr_fact_layout(m, n, n, ws, batch);
+  double2* Qc = (double2*)((char*)ws + al(qr_fact_bytes(m, n, n, batch)));
   int st;
-  if ((st = qr_factor(f, (const double2*)a, lda, s))) return st;
-  k_q_init<<<(unsigned)((m * k + 255) / 256), 256, 0, s>>>(m, k, Qc);
+  if ((st = qr_factor(f, (const double2*)a, lda, s, m * lda))) return st;
+  const unsigned B = (unsigned)batch;
+  k_q_init<<<dim3((unsigned)((m * k + 255) / 256), B), 256, 0, s>>>(m, k, Qc);
   PTN_LAUNCH_CHECK();
-  if ((st = qr_apply_q(f, Qc, m, k, true, s))) return st;
+  if ((st = qr_apply_q(f, Qc, m, k, true, s, m * k))) return st;
   const int64_t tot = (m * k > k * n) ? m * k : k * n;
-  k_qr_out<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(m, n, k, f.W, f.rdiag, Qc, (double2*)q,
-                                                          ldq, (double2*)r, ldr);
+  k_qr_out<<<dim3((unsigned)((tot + 255) / 256), B), 256, 0, s>>>(
+      m, n, k, f.W, f.rdiag, Qc, (double2*)q, ldq, (double2*)r, ldr, f.bs, m * ldq, k * ldr);
   PTN_LAUNCH_CHECK();
   return PTN_OK;
 }
