@@ -69,10 +69,30 @@ class _Bucket:
             backend.report_krylov_status(rows)
 
 
+def _orthonormalize_right(A):
+    """MPS.orthonormalize(mode="right") (mps.py:162-178) of stacked chains:
+    batched QR sweep from the right; returns the per-chain norms (sign >= 0)."""
+    n = len(A)
+    for i in reversed(range(n)):
+        af = _flip(A[i])
+        nb, s0, s1, s2 = af.shape
+        q, r = backend.qr_batched(af.reshape(nb, s0 * s1, s2))
+        A[i] = _flip(q.reshape(nb, s0, s1, q.shape[2]))
+        if i > 0:
+            A[i - 1] = _absorb_right(A[i - 1], r.transpose(1, 2).contiguous())
+    assert tuple(r.shape[1:]) == (1, 1)
+    nrm = r[:, 0, 0].real.cpu().numpy()
+    sign = torch.as_tensor(np.where(nrm < 0, -1.0, 1.0), dtype=torch.complex128,
+                           device=A[0].device)
+    A[0] = A[0] * sign.reshape(-1, 1, 1, 1)
+    return [float(abs(x)) for x in nrm]
+
+
 def _run_bucket(bk, dt, numsteps):
-    """tdvp.py:26-118 on a bucket of stacked chains."""
+    """tdvp.py:26-118 on a bucket of stacked chains; returns the initial norms."""
     n, nb, W, prec = bk.n, bk.nb, bk.W, bk.precision
     A = [torch.stack([p.a[i] for p in bk.psis]) for i in range(n)]
+    nrms = _orthonormalize_right(A)
     one = torch.ones((nb, 1, 1, 1), dtype=torch.complex128, device=A[0].device)
     rblocks = [None] * n
     rblocks[n - 1] = one
@@ -106,6 +126,7 @@ def _run_bucket(bk, dt, numsteps):
         bk.flush()
     for b, psi in enumerate(bk.psis):
         psi.a = [A[i][b].contiguous() for i in range(n)]
+    return nrms
 
 
 def tdvp_singlesite_batched(hamiltonians, psis, dt, numsteps: int, numiter_lanczos: int = 25,
@@ -130,12 +151,11 @@ def tdvp_singlesite_batched(hamiltonians, psis, dt, numsteps: int, numiter_lancz
     host = [not psi.on_device for psi in psis]
     for psi in psis:
         psi.to_device()
-    for k, psi in enumerate(psis):
-        nrms[k] = psi.orthonormalize(mode="right")
     for idx in buckets.values():
         bk = _Bucket([hamiltonians[k] for k in idx], [psis[k] for k in idx], numiter_lanczos,
                      precision)
-        _run_bucket(bk, dt, numsteps)
+        for k, nrm in zip(idx, _run_bucket(bk, dt, numsteps)):
+            nrms[k] = nrm
     for k, psi in enumerate(psis):
         if host[k]:
             psi.to_host()
