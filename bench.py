@@ -296,6 +296,16 @@ def main():
         "energy_rel_err_vs_fp64": rel_e,
         "site_updates_per_s": round(nupd / tmx, 3),
         "svd_2D_x_2D_s": round(svd_s, 4),
+        "svd_roofline": {"kernel": "bond SVD of a 2D x 2D complex matrix (two QR "
+                                   "preconditioners + blocked one-sided Jacobi, FP64)",
+                         "model_flop": 22.0 * 4 * (2 * args.D)**3,
+                         "achieved": round(22.0 * 4 * (2 * args.D)**3 / svd_s / 1e12, 3),
+                         "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                         "frac": round(22.0 * 4 * (2 * args.D)**3 / svd_s / 1e12
+                                       / PEAK_TFLOPS["fp64"], 4),
+                         "note": "model = LAPACK complex SVD count 22 m n^2 x 4 real flop per "
+                                 "complex flop; Jacobi does more work and is bound by its "
+                                 "127 dependent rounds per sweep"},
         "roofline": {"kernel": "two-site effective Hamiltonian (A.R, W mix, L^T.T2)",
                      "bound": "mfma", "achieved": round(heff_tf_mx, 2),
                      "peak": PEAK_TFLOPS[args.mixed], "unit": "TFLOP/s",

@@ -24,3 +24,4 @@ from .dmrg import *
 
 __version__ = "0.1.0"
 from .batched import *
+from .state_io import *

@@ -158,3 +158,29 @@ def test_fp32_contractions_keep_fidelity():
     assert abs(1 - f) < 1e-8
     e64, e32 = ptn.mpo_average(p64, H).real, ptn.mpo_average(p32, H).real
     assert abs(e32 - e64) < 1e-6 * abs(e64)
+
+
+def test_tdvp_checkpoint_resume(tmp_path):
+    """Two real-time steps in one run equal one step, a checkpoint, a resumed run
+    and one more step (the resumed run re-normalises a state of norm one)."""
+    rng = np.random.default_rng(17)
+    L = 8
+    H = ptn.heisenberg_xxz_1d_mpo(L, 1.0, 0.8, -0.1).zero_qnumbers()
+    bonds = [min(8, 2**i, 2**(L - i)) for i in range(L + 1)]
+    psi = ptn.MPS(H.qsite, [np.zeros(b, dtype=int) for b in bonds], fill="random", rng=rng)
+    psi.orthonormalize("left")
+    ref = copy.deepcopy(psi)
+    ptn.tdvp_twosite(H, ref, -0.05j, 2, numiter_lanczos=8, tol_split=0)
+    run = ptn.TwoSiteTDVP(H, psi, -0.05j, 8, 0)
+    run.setup()
+    run.sweep()
+    path = tmp_path / "ckpt.npz"
+    ptn.tdvp_checkpoint(run, path, 1)
+    run2, done = ptn.tdvp_resume(path, H, -0.05j, 8, 0)
+    assert done == 1
+    run2.setup()
+    run2.sweep()
+    run2.finish()
+    got = run2.psi
+    f = abs(ptn.mps_vdot(ref, got))**2 / (ptn.mps_norm(ref)**2 * ptn.mps_norm(got)**2)
+    assert abs(1 - f) < 1e-12

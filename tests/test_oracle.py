@@ -366,3 +366,4 @@ def test_dmrg_matches_exact_diagonalisation():
         en = orc.dmrg_singlesite(H, psi, 4)
     e_exact = np.linalg.eigvalsh(xxz_kron(L, J, D, h))[0]
     assert abs(en[-1] - e_exact) < 1e-12
+
