@@ -192,6 +192,15 @@ int ptn_env_right_batched(int precision, int64_t batch, int64_t Dl, int64_t d, i
 size_t ptn_qr_workspace_batched(int64_t batch, int64_t m, int64_t n);
 int ptn_qr_batched(int64_t batch, int64_t m, int64_t n, const void* a, void* q, void* r,
                    void* ws, size_t ws_bytes, void* stream);
+/* Thin SVDs of `batch` m x n row-major matrices: U (m x k), s (k), Vh (k x n)
+ * per problem, k = min(m, n). The Jacobi sweeps of all problems run together;
+ * info as for ptn_svd, for the whole batch. The per-sector SVDs of
+ * block_sparse_svd (pytenet/block_sparse_util.py:285-307) use it: every
+ * sector is zero-padded to the largest shape of its size class, and the
+ * padding stays exactly zero through the factorisation. */
+size_t ptn_svd_workspace_batched(int64_t batch, int64_t m, int64_t n);
+int ptn_svd_batched(int64_t batch, int64_t m, int64_t n, const void* a, void* u, double* s,
+                    void* vh, int32_t* info, void* ws, size_t ws_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -303,11 +303,7 @@ def svd(a, return_info=False):
     _check(lib.ptn_svd(m, n, ptr(a), a.stride(0), ptr(u), k, ptr(s), ptr(vh), n, ptr(info),
                        ptr(ws), ws.numel(), stream()), "ptn_svd")
     inf = info.cpu().numpy()
-    if inf[2]:
-        raise FloatingPointError("non-finite singular value")
-    if inf[1]:
-        warnings.warn(f"Jacobi SVD of a {m} x {n} matrix stopped after {inf[0]} sweeps "
-                      "before every column pair met the orthogonality tolerance", RuntimeWarning)
+    _svd_status(inf, f"a {m} x {n} matrix")
     if return_info:
         return u, s, vh, inf
     return u, s, vh
@@ -420,9 +416,57 @@ def block_sparse_qr(a, q0, q1):
     return q[:, :k].contiguous(), r[:k, :].contiguous(), qi[:k]
 
 
+def _svd_status(inf, what):
+    if inf[2]:
+        raise FloatingPointError("non-finite singular value")
+    if inf[1]:
+        warnings.warn(f"Jacobi SVD of {what} stopped after {inf[0]} sweeps before every column "
+                      "pair met the orthogonality tolerance", RuntimeWarning)
+
+
+def svd_batched(a, return_info=False):
+    """Thin SVDs of a stack (B, m, n) of complex128 device matrices, all Jacobi
+    sweeps run together: (u (B, m, k), s (B, k) device float64, vh (B, k, n))."""
+    a = as_dev(a).contiguous()
+    nb, m, n = a.shape
+    k = min(m, n)
+    u = torch.empty((nb, m, k), dtype=torch.complex128, device=a.device)
+    vh = torch.empty((nb, k, n), dtype=torch.complex128, device=a.device)
+    s = torch.empty((nb, k), dtype=torch.float64, device=a.device)
+    if k == 0 or nb == 0:
+        return (u, s, vh, np.zeros(4, dtype=np.int32)) if return_info else (u, s, vh)
+    lib = _lib.load()
+    ws = workspace(lib.ptn_svd_workspace_batched(nb, m, n))
+    info = torch.zeros(4, dtype=torch.int32, device=a.device)
+    _check(lib.ptn_svd_batched(nb, m, n, ptr(a), ptr(u), ptr(s), ptr(vh), ptr(info), ptr(ws),
+                               ws.numel(), stream()), "ptn_svd_batched")
+    inf = info.cpu().numpy()
+    _svd_status(inf, f"{nb} matrices of {m} x {n}")
+    if return_info:
+        return u, s, vh, inf
+    return u, s, vh
+
+
+def _svd_classes(shapes):
+    """Group sector shapes for batched SVDs: a wide block is factorised through
+    its adjoint, and blocks whose smaller dimension falls in the same
+    half-octave share one zero-padded batch."""
+    groups = {}
+    for idx, (ms, ns) in enumerate(shapes):
+        small = min(ms, ns)
+        groups.setdefault(int(np.ceil(2 * np.log2(small))) if small > 1 else 0, []).append(idx)
+    return list(groups.values())
+
+
 def block_sparse_svd(a, q0, q1):
-    """Per-sector SVD: (u, s, v, q) with s a host float64 array (sector order,
-    descending within a sector) and u, v device tensors."""
+    """Per-sector SVD (block_sparse_util.py:244-310): (u, s, v, q) with s a host
+    float64 array (sector order, descending within a sector) and u, v device
+    tensors. The sector blocks are factorised in batches: each block is placed,
+    zero-padded, in a common (P, Q) shape (P >= Q, wide blocks as their
+    adjoint). The padding stays exactly zero through the Householder and Jacobi
+    steps (zero columns give identity reflectors and zero Gram entries, so no
+    rotation touches them), hence the first min(m_s, n_s) singular triplets of
+    a padded block are those of the sector block itself."""
     a = as_dev(a)
     assert a.dim() == 2 and len(q0) == a.shape[0] and len(q1) == a.shape[1]
     q0 = np.asarray(q0)
@@ -437,16 +481,36 @@ def block_sparse_svd(a, q0, q1):
             u[0, 0] = 1
         v = torch.zeros((1, a.shape[1]), dtype=torch.complex128, device=a.device)
         return u, np.zeros(1), v, q0[:1]
+    idx = [(torch.as_tensor(rows, device=a.device), torch.as_tensor(cols, device=a.device))
+           for _, rows, cols in secs]
+    shapes = [(len(rows), len(cols)) for _, rows, cols in secs]
+    res = [None] * len(secs)
+    for group in _svd_classes(shapes):
+        P = max(max(shapes[j]) for j in group)
+        Q = max(min(shapes[j]) for j in group)
+        pad = torch.zeros((len(group), P, Q), dtype=torch.complex128, device=a.device)
+        for b, j in enumerate(group):
+            ri, ci = idx[j]
+            blk = a.index_select(0, ri).index_select(1, ci)
+            ms, ns = shapes[j]
+            if ms >= ns:
+                pad[b, :ms, :ns] = blk
+            else:
+                pad[b, :ns, :ms] = blk.mH
+        ub, sb, vb = svd_batched(pad)
+        for b, j in enumerate(group):
+            ms, ns = shapes[j]
+            kk = min(ms, ns)
+            if ms >= ns:
+                res[j] = (ub[b, :ms, :kk], sb[b, :kk], vb[b, :kk, :ns])
+            else:
+                res[j] = (vb[b, :kk, :ms].mH, sb[b, :kk], ub[b, :ns, :kk].mH)
     kmax = min(a.shape)
     u = torch.zeros((a.shape[0], kmax), dtype=torch.complex128, device=a.device)
     v = torch.zeros((kmax, a.shape[1]), dtype=torch.complex128, device=a.device)
     s_parts, qi = [], np.zeros(kmax, dtype=q0.dtype)
     k = 0
-    for qn, rows, cols in secs:
-        ri = torch.as_tensor(rows, device=a.device)
-        ci = torch.as_tensor(cols, device=a.device)
-        blk = a.index_select(0, ri).index_select(1, ci).contiguous()
-        us, ss, vs = svd(blk)
+    for (qn, _, _), (ri, ci), (us, ss, vs) in zip(secs, idx, res):
         kk = len(ss)
         u[ri, k:k + kk] = us
         v[k:k + kk, ci] = vs

@@ -53,7 +53,9 @@ __device__ double wg_sum3(double v, double* red) {
   return s;
 }
 
+// blockIdx.y selects the matrix of a batch (q x q each)
 __global__ void k_eye(int64_t q, double2* __restrict__ V) {
+  V += (int64_t)blockIdx.y * q * q;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < q * q;
        t += (int64_t)gridDim.x * blockDim.x)
     V[t] = make_double2((t / q) == (t % q) ? 1.0 : 0.0, 0.0);
@@ -81,10 +83,12 @@ __device__ __forceinline__ int64_t pair_col(int bp, int bq, int c) {
 // column tiles 2 (w >> 1) and 2 (w >> 1) + 1. Rows are staged through LDS in
 // chunks of GCH.
 static_assert(JP == 32, "the Gram and apply kernels are laid out for 32-column block pairs");
+// blockIdx.z selects the problem of a batch (X at X + z sx).
 __global__ __launch_bounds__(256) void k_jac_gram(int64_t p, int nblk, int r,
                                                   const double2* __restrict__ X,
-                                                  double2* __restrict__ Gp, int nseg) {
+                                                  double2* __restrict__ Gp, int nseg, int64_t sx) {
   __shared__ double2 Y[GCH][JP + 1];
+  X += (int64_t)blockIdx.z * sx;
   int bp, bq;
   round_pair(nblk, r, blockIdx.y, bp, bq);
   const int seg = blockIdx.x;
@@ -134,7 +138,7 @@ __global__ __launch_bounds__(256) void k_jac_gram(int64_t p, int nblk, int r,
   // result element v of tile h: complex row i0 + kk + 4v, real column l16 of the
   // tile (complex column 8 (jt0 + h) + l16 / 2, real part for even l16); the
   // partner lane l16 ^ 1 holds the other part
-  double2* out = Gp + ((int64_t)blockIdx.y * nseg + seg) * JP * JP;
+  double2* out = Gp + (((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * nseg + seg) * JP * JP;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     double mine[4], other[4];
@@ -174,7 +178,9 @@ __global__ __launch_bounds__(256) void k_jac_solve(int nblk, const double2* __re
   __shared__ double rc[JB], rsr[JB], rsi[JB];
   __shared__ int rot_any, rot_big;
   const int t = threadIdx.x;
-  const double2* gsrc = Gp + (int64_t)blockIdx.x * nseg * JP * JP;
+  // pair blockIdx.x of problem blockIdx.y
+  const int64_t pidx = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+  const double2* gsrc = Gp + pidx * nseg * JP * JP;
   {
     // sum of the partial Gram matrices in a fixed order; the four elements of a
     // thread and several partials are loaded before they are added
@@ -258,35 +264,38 @@ __global__ __launch_bounds__(256) void k_jac_solve(int nblk, const double2* __re
     }
     __syncthreads();
   }
-  double2* jout = Jg + (int64_t)blockIdx.x * JP * JP;
+  double2* jout = Jg + pidx * JP * JP;
   for (int e = t; e < JP * JP; e += 256) jout[e] = J[e / JP][e % JP];
   if (t == 0) {
-    rotflag[blockIdx.x] = rot_any;
+    rotflag[pidx] = rot_any;
     if (rot_big) atomicAdd(counters, 1);
   }
 }
 
-// Y <- Y J (pass blockIdx.z = 0, rows of X) or V <- V J (pass 1) for AROWS rows
+// Y <- Y J (pass blockIdx.z & 1 = 0, rows of X) or V <- V J (pass 1) for AROWS rows
 // per workgroup, 16 per wave, on FP64 MFMA: the rows' 32 complex entries are
 // the A operand (real k = 2 column + part, the interleaved complex layout) and
 // J's real embedding [[Re, Im], [-Im, Re]] the B operand, read from LDS.
 // Every wave loads all A operands of its rows before it stores any result.
+// Problem blockIdx.z >> 1 of a batch: X (p x qp) and V (qp x qp) stacked.
 __global__ __launch_bounds__(256) void k_jac_apply(int64_t p, int64_t qp, int nblk, int r,
                                                    double2* __restrict__ X,
                                                    double2* __restrict__ V,
                                                    const double2* __restrict__ Jg,
                                                    const int* __restrict__ rotflag) {
   __shared__ double2 J[JP][JP + 1];
-  if (!rotflag[blockIdx.y]) return;
-  const int pass = blockIdx.z;
-  double2* M = pass ? V : X;
+  const int64_t prob = blockIdx.z >> 1;
+  const int64_t pidx = prob * gridDim.y + blockIdx.y;
+  if (!rotflag[pidx]) return;
+  const int pass = blockIdx.z & 1;
+  double2* M = pass ? V + prob * qp * qp : X + prob * p * qp;
   const int64_t ld = pass ? qp : p;
   const int64_t rowblk = (int64_t)blockIdx.x * AROWS;
   if (rowblk >= ld) return;
   int bp, bq;
   round_pair(nblk, r, blockIdx.y, bp, bq);
   const int t = threadIdx.x;
-  const double2* jsrc = Jg + (int64_t)blockIdx.y * JP * JP;
+  const double2* jsrc = Jg + pidx * JP * JP;
   for (int e = t; e < JP * JP; e += 256) J[e / JP][e % JP] = jsrc[e];
   __syncthreads();
   const int lane = t & 63, wave = t >> 6;
@@ -334,11 +343,13 @@ __global__ __launch_bounds__(256) void k_jac_apply(int64_t p, int64_t qp, int nb
   }
 }
 
-// column norms of X
+// column norms of X (problem blockIdx.y of a batch at X + y sx, nrm + y sn)
 __global__ __launch_bounds__(256) void k_colnorms(int64_t pr, const double2* __restrict__ X,
-                                                  double* __restrict__ nrm) {
+                                                  double* __restrict__ nrm, int64_t sx,
+                                                  int64_t sn) {
   __shared__ double red[256];
-  const double2* x = X + (int64_t)blockIdx.x * pr;
+  const double2* x = X + blockIdx.y * sx + (int64_t)blockIdx.x * pr;
+  nrm += blockIdx.y * sn;
   double s = 0;
   for (int64_t i = threadIdx.x; i < pr; i += 256) {
     double2 u = x[i];
@@ -351,10 +362,14 @@ __global__ __launch_bounds__(256) void k_colnorms(int64_t pr, const double2* __r
 // descending order of the q column norms (stable: ties keep column order),
 // one workgroup; rank of column j = #{i : s_i > s_j} + #{i < j : s_i == s_j}.
 // A non-finite norm is ordered as -1 (norms are >= 0) so that the ranks stay a
-// permutation; the flag reports it.
+// permutation; the flag reports it. One workgroup per problem of a batch
+// (nrm and perm with stride sn, s_out with stride ss).
 __global__ __launch_bounds__(256) void k_rank(int64_t q, const double* __restrict__ nrm,
                                               int64_t* __restrict__ perm, double* __restrict__ s_out,
-                                              int* __restrict__ nonfinite) {
+                                              int* __restrict__ nonfinite, int64_t sn, int64_t ss) {
+  nrm += blockIdx.x * sn;
+  perm += blockIdx.x * sn;
+  s_out += blockIdx.x * ss;
   for (int64_t j = threadIdx.x; j < q; j += 256) {
     const double sj = nrm[j];
     if (!isfinite(sj)) atomicOr(nonfinite, 1);
@@ -373,10 +388,19 @@ __global__ __launch_bounds__(256) void k_rank(int64_t q, const double* __restric
 // (column-major q x q) of B = U_B diag(sigma) V_B^H:
 //   m >= n (B = A):    U[i][c] = UB[i][pc],  Vh[c][j] = conj(VB[j][pc])
 //   m <  n (B = A^H):  U[i][c] = VB[i][pc],  Vh[c][j] = conj(UB[j][pc])
+// Problem blockIdx.y of a batch: UB, VB, perm (stride sn) stacked in the
+// workspace, u and vh with strides su and sv.
 __global__ void k_svd_out(int64_t m, int64_t n, int64_t k, int adjoint, int64_t p, int64_t q,
                           const double2* __restrict__ UB, const double2* __restrict__ VB,
                           const int64_t* __restrict__ perm, double2* __restrict__ u, int64_t ldu,
-                          double2* __restrict__ vh, int64_t ldvh) {
+                          double2* __restrict__ vh, int64_t ldvh, int64_t sn, int64_t su,
+                          int64_t sv) {
+  const int64_t b = blockIdx.y;
+  UB += b * p * q;
+  VB += b * q * q;
+  perm += b * sn;
+  u += b * su;
+  vh += b * sv;
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < m * k) {
     const int64_t i = t / k, c = t % k, pc = perm[c];
@@ -389,9 +413,12 @@ __global__ void k_svd_out(int64_t m, int64_t n, int64_t k, int adjoint, int64_t 
   }
 }
 
-// B = A^H (row-major n x m) from A (m x n, lda)
+// B = A^H (row-major n x m) from A (m x n, lda); problem blockIdx.y of a batch
+// at a + y sa, b + y m n
 __global__ void k_adjoint(int64_t m, int64_t n, const double2* __restrict__ a, int64_t lda,
-                          double2* __restrict__ b) {
+                          double2* __restrict__ b, int64_t sa) {
+  a += blockIdx.y * sa;
+  b += blockIdx.y * m * n;
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= m * n) return;
   const int64_t i = t / n, j = t % n;
@@ -402,10 +429,14 @@ __global__ void k_adjoint(int64_t m, int64_t n, const double2* __restrict__ a, i
 // R^H of a factorisation (strict upper triangle of R in the column-major p x q
 // W, diagonal in rdiag): element (i, c) of R^H is conj(R[c][i]). Written either
 // column-major with leading dimension ld and zero padding columns up to ncols
-// (colmajor = 1), or row-major q x q (colmajor = 0).
+// (colmajor = 1), or row-major q x q (colmajor = 0). Problem blockIdx.y of a
+// batch: W, rdiag with strides sw, sr; out with stride ld ncols.
 __global__ void k_rh(int64_t p, int64_t q, int64_t ld, int64_t ncols, int colmajor,
                      const double2* __restrict__ W, const double2* __restrict__ rdiag,
-                     double2* __restrict__ out) {
+                     double2* __restrict__ out, int64_t sw, int64_t sr) {
+  W += blockIdx.y * sw;
+  rdiag += blockIdx.y * sr;
+  out += blockIdx.y * ld * ncols;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ld * ncols;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t c = colmajor ? t / ld : t % ld, i = colmajor ? t % ld : t / ld;
@@ -416,9 +447,14 @@ __global__ void k_rh(int64_t p, int64_t q, int64_t ld, int64_t ncols, int colmaj
 }
 
 // UB (column-major p x q) = [X / sigma; 0] for X column-major q x q (sigma = its
-// column norms; a zero column stays zero)
+// column norms; a zero column stays zero). Problem blockIdx.y of a batch: X
+// with stride sx, nrm with stride sn, UB with stride p q.
 __global__ void k_embed_u(int64_t p, int64_t q, const double2* __restrict__ X,
-                          const double* __restrict__ nrm, double2* __restrict__ UB) {
+                          const double* __restrict__ nrm, double2* __restrict__ UB, int64_t sx,
+                          int64_t sn) {
+  X += blockIdx.y * sx;
+  nrm += blockIdx.y * sn;
+  UB += blockIdx.y * p * q;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < p * q;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t c = t / p, i = t % p;
@@ -430,8 +466,11 @@ __global__ void k_embed_u(int64_t p, int64_t q, const double2* __restrict__ X,
 }
 
 // VB (column-major q x q) = V(0:q, 0:q) for V column-major with leading dimension qp
+// (problem blockIdx.y of a batch: V with stride qp qp, VB with stride q q)
 __global__ void k_copy_v(int64_t q, int64_t qp, const double2* __restrict__ V,
                          double2* __restrict__ VB) {
+  V += blockIdx.y * qp * qp;
+  VB += blockIdx.y * q * q;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < q * q;
        t += (int64_t)gridDim.x * blockDim.x)
     VB[t] = V[(t / q) * qp + t % q];
@@ -460,51 +499,69 @@ static int* pinned_counter() {
 }
 
 struct SvdLayout {
-  int64_t p, q, qp, nseg;
+  int64_t p, q, qp, nseg, batch;
   int nblk;
   size_t bt, fact, fact2, rh, x, v, ub, vb, gp, jg, rflag, nrm, perm, cnt, total;
 };
 
-static SvdLayout svd_layout(int64_t m, int64_t n) {
+// every per-problem region holds `batch` consecutive copies
+static SvdLayout svd_layout(int64_t m, int64_t n, int64_t batch) {
   SvdLayout L;
+  L.batch = batch;
   L.q = m < n ? m : n;
   L.p = m < n ? n : m;
   L.qp = ((L.q + JP - 1) / JP) * JP;
   L.nblk = (int)(L.qp / JB);
   L.nseg = (L.q + GSEG - 1) / GSEG;   // Jacobi runs on X = R2^H with q rows
   const int npairs = L.nblk / 2;
+  const size_t B = (size_t)batch;
   size_t off = 0;
-  L.bt = off; off += (m < n) ? al((size_t)L.p * L.q * 16) : 0;
-  L.fact = off; off += al(qr_fact_bytes(L.p, L.q, L.q));
-  L.fact2 = off; off += al(qr_fact_bytes(L.q, L.q, L.q));
-  L.rh = off; off += al((size_t)L.q * L.q * 16);
-  L.x = off; off += al((size_t)L.q * L.qp * 16);
-  L.v = off; off += al((size_t)L.qp * L.qp * 16);
-  L.ub = off; off += al((size_t)L.p * L.q * 16);
-  L.vb = off; off += al((size_t)L.q * L.q * 16);
-  L.gp = off; off += al((size_t)npairs * L.nseg * JP * JP * 16);
-  L.jg = off; off += al((size_t)npairs * JP * JP * 16);
-  L.rflag = off; off += al((size_t)npairs * 4);
-  L.nrm = off; off += al((size_t)L.qp * 8);
-  L.perm = off; off += al((size_t)L.qp * 8);
+  L.bt = off; off += (m < n) ? al(B * L.p * L.q * 16) : 0;
+  L.fact = off; off += al(qr_fact_bytes(L.p, L.q, L.q, batch));
+  L.fact2 = off; off += al(qr_fact_bytes(L.q, L.q, L.q, batch));
+  L.rh = off; off += al(B * L.q * L.q * 16);
+  L.x = off; off += al(B * L.q * L.qp * 16);
+  L.v = off; off += al(B * L.qp * L.qp * 16);
+  L.ub = off; off += al(B * L.p * L.q * 16);
+  L.vb = off; off += al(B * L.q * L.q * 16);
+  L.gp = off; off += al(B * npairs * L.nseg * JP * JP * 16);
+  L.jg = off; off += al(B * npairs * JP * JP * 16);
+  L.rflag = off; off += al(B * npairs * 4);
+  L.nrm = off; off += al(B * L.qp * 8);
+  L.perm = off; off += al(B * L.qp * 8);
   L.cnt = off; off += al(16);
   L.total = off;
   return L;
 }
 
-size_t svd_ws_bytes(int64_t m, int64_t n) { return svd_layout(m, n).total; }
+size_t svd_ws_bytes(int64_t m, int64_t n, int64_t batch) { return svd_layout(m, n, batch).total; }
 
+// grid width of the grid-stride helper kernels for `total` elements per problem
+static inline unsigned gs_blocks(int64_t total) {
+  const int64_t b = (total + 255) / 256;
+  return (unsigned)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+}
+
+// Thin SVD of `batch` m x n matrices of identical shape (problem b at a + b sa,
+// u + b su, s + b ss, vh + b svh). The Jacobi sweeps run on all problems at
+// once and stop together: a sweep in which no problem needed a rotation above
+// `big` is followed by one final sweep. info (if given) receives the number of
+// sweeps, whether the sweep limit was hit, and whether any singular value was
+// not finite.
 int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t ldu, double* s,
-            void* vh, int64_t ldvh, int32_t* info, void* ws, size_t ws_bytes, hipStream_t st) {
+            void* vh, int64_t ldvh, int32_t* info, void* ws, size_t ws_bytes, hipStream_t st,
+            int64_t batch, int64_t sa, int64_t su, int64_t ss, int64_t svh) {
   if (m <= 0 || n <= 0 || lda < n) return PTN_ERR_ARG;
   const int64_t k = m < n ? m : n;
   if (ldu < k || ldvh < n) return PTN_ERR_ARG;
-  const SvdLayout L = svd_layout(m, n);
+  if (batch < 1 || batch > 32767) return PTN_ERR_ARG;
+  const SvdLayout L = svd_layout(m, n, batch);
   if (ws_bytes < L.total) return PTN_ERR_WORKSPACE;
   const int npairs = L.nblk / 2;
   const int64_t aseg = (L.qp + AROWS - 1) / AROWS;   // qp >= q rows
-  if (npairs > 65535 || aseg > 65535) return PTN_ERR_ARG;
+  if (npairs > 65535 || aseg > 65535 || L.q > 65535) return PTN_ERR_ARG;
   const int adjoint = m < n;
+  const unsigned NB = (unsigned)batch;
   char* base = (char*)ws;
   double2* X = (double2*)(base + L.x);
   double2* V = (double2*)(base + L.v);
@@ -516,28 +573,32 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
   double* nrm = (double*)(base + L.nrm);
   int64_t* perm = (int64_t*)(base + L.perm);
   int* counters = (int*)(base + L.cnt);   // [0] rotations in the current sweep, [1] non-finite
+  const int64_t sx = L.q * L.qp;          // X: q x qp column-major per problem
   int stt;
 
   // 1. B = Q R
   const void* B = a;
-  int64_t ldb = lda;
+  int64_t ldb = lda, sb = sa;
   if (adjoint) {
-    k_adjoint<<<(unsigned)((m * n + 255) / 256), 256, 0, st>>>(m, n, (const double2*)a, lda,
-                                                                (double2*)(base + L.bt));
+    k_adjoint<<<dim3((unsigned)((m * n + 255) / 256), NB), 256, 0, st>>>(
+        m, n, (const double2*)a, lda, (double2*)(base + L.bt), sa);
     PTN_LAUNCH_CHECK();
     B = base + L.bt;
     ldb = m;
+    sb = m * n;
   }
-  QrFact f = qr_fact_layout(L.p, L.q, L.q, base + L.fact);
-  if ((stt = qr_factor(f, (const double2*)B, ldb, st))) return stt;
+  QrFact f = qr_fact_layout(L.p, L.q, L.q, base + L.fact, batch);
+  if ((stt = qr_factor(f, (const double2*)B, ldb, st, sb))) return stt;
   // 2. R^H = Q2 R2, and one-sided Jacobi on X = R2^H
-  QrFact f2 = qr_fact_layout(L.q, L.q, L.q, base + L.fact2);
+  QrFact f2 = qr_fact_layout(L.q, L.q, L.q, base + L.fact2, batch);
   double2* RH = (double2*)(base + L.rh);
-  k_rh<<<1024, 256, 0, st>>>(L.p, L.q, L.q, L.q, 0, f.W, f.rdiag, RH);
+  k_rh<<<dim3(gs_blocks(L.q * L.q), NB), 256, 0, st>>>(L.p, L.q, L.q, L.q, 0, f.W, f.rdiag, RH,
+                                                       f.bs.W, f.bs.k);
   PTN_LAUNCH_CHECK();
-  if ((stt = qr_factor(f2, RH, L.q, st))) return stt;
-  k_rh<<<1024, 256, 0, st>>>(L.q, L.q, L.q, L.qp, 1, f2.W, f2.rdiag, X);
-  k_eye<<<1024, 256, 0, st>>>(L.qp, V);
+  if ((stt = qr_factor(f2, RH, L.q, st, L.q * L.q))) return stt;
+  k_rh<<<dim3(gs_blocks(L.q * L.qp), NB), 256, 0, st>>>(L.q, L.q, L.q, L.qp, 1, f2.W, f2.rdiag, X,
+                                                        f2.bs.W, f2.bs.k);
+  k_eye<<<dim3(gs_blocks(L.qp * L.qp), NB), 256, 0, st>>>(L.qp, V);
   PTN_LAUNCH_CHECK();
   int* host_rot = pinned_counter();
   if (!host_rot) return PTN_ERR_HIP;
@@ -552,11 +613,11 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
   while (!converged && sweeps < max_sweeps) {
     k_set_ints<<<1, 64, 0, st>>>(counters, 1, 0);
     for (int r = 0; r < L.nblk - 1; ++r) {
-      k_jac_gram<<<dim3((unsigned)L.nseg, (unsigned)npairs), 256, 0, st>>>(L.q, L.nblk, r, X, Gp,
-                                                                           (int)L.nseg);
-      k_jac_solve<<<(unsigned)npairs, 256, 0, st>>>(L.nblk, Gp, (int)L.nseg, tol, big, Jg, rflag,
-                                                     counters);
-      k_jac_apply<<<dim3((unsigned)aseg, (unsigned)npairs, 2), 256, 0, st>>>(
+      k_jac_gram<<<dim3((unsigned)L.nseg, (unsigned)npairs, NB), 256, 0, st>>>(
+          L.q, L.nblk, r, X, Gp, (int)L.nseg, sx);
+      k_jac_solve<<<dim3((unsigned)npairs, NB), 256, 0, st>>>(L.nblk, Gp, (int)L.nseg, tol, big,
+                                                              Jg, rflag, counters);
+      k_jac_apply<<<dim3((unsigned)aseg, (unsigned)npairs, 2 * NB), 256, 0, st>>>(
           L.q, L.qp, L.nblk, r, X, V, Jg, rflag);
     }
     PTN_LAUNCH_CHECK();
@@ -567,17 +628,17 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
     else if (*host_rot == 0) final_sweep = 1;
   }
   // 3. B = Q1 R2^H Q2^H = (Q1 [U_X; 0]) diag(sigma) (Q2 V)^H with U_X = X / sigma
-  k_colnorms<<<(unsigned)L.q, 256, 0, st>>>(L.q, X, nrm);
-  k_embed_u<<<1024, 256, 0, st>>>(L.p, L.q, X, nrm, UB);
-  k_copy_v<<<1024, 256, 0, st>>>(L.q, L.qp, V, VB);
+  k_colnorms<<<dim3((unsigned)L.q, NB), 256, 0, st>>>(L.q, X, nrm, sx, L.qp);
+  k_embed_u<<<dim3(gs_blocks(L.p * L.q), NB), 256, 0, st>>>(L.p, L.q, X, nrm, UB, sx, L.qp);
+  k_copy_v<<<dim3(gs_blocks(L.q * L.q), NB), 256, 0, st>>>(L.q, L.qp, V, VB);
   PTN_LAUNCH_CHECK();
-  if ((stt = qr_apply_q(f, UB, L.p, L.q, false, st))) return stt;
-  if ((stt = qr_apply_q(f2, VB, L.q, L.q, false, st))) return stt;
-  k_rank<<<1, 256, 0, st>>>(L.q, nrm, perm, s, counters + 1);
+  if ((stt = qr_apply_q(f, UB, L.p, L.q, false, st, L.p * L.q))) return stt;
+  if ((stt = qr_apply_q(f2, VB, L.q, L.q, false, st, L.q * L.q))) return stt;
+  k_rank<<<NB, 256, 0, st>>>(L.q, nrm, perm, s, counters + 1, L.qp, ss);
   const int64_t tot = (m * k > k * n) ? m * k : k * n;
-  k_svd_out<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(m, n, k, adjoint, L.p, L.q, UB, VB,
-                                                            perm, (double2*)u, ldu, (double2*)vh,
-                                                            ldvh);
+  k_svd_out<<<dim3((unsigned)((tot + 255) / 256), NB), 256, 0, st>>>(
+      m, n, k, adjoint, L.p, L.q, UB, VB, perm, (double2*)u, ldu, (double2*)vh, ldvh, L.qp, su,
+      svh);
   if (info) k_svd_info<<<1, 64, 0, st>>>(info, sweeps, converged ? 0 : 1, counters);
   PTN_LAUNCH_CHECK();
   return PTN_OK;

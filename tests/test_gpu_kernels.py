@@ -243,6 +243,71 @@ def test_block_sparse_factorisations_match_reference_fixture():
     np.testing.assert_allclose((uh * s) @ vh, z["svd_a"], atol=1e-13)
 
 
+@pytest.mark.parametrize("m,n", [(40, 24), (24, 40), (70, 70), (33, 1)])
+def test_svd_batched_matches_lapack(m, n):
+    rng = np.random.default_rng(m * 7 + n)
+    k = min(m, n)
+    mats, svals = [], []
+    for b in range(5):
+        u0, _, v0 = np.linalg.svd(orc.crandn((m, n), rng), full_matrices=False)
+        s0 = np.logspace(0, -(3 + 2 * b), k)
+        mats.append((u0 * s0) @ v0)
+        svals.append(s0)
+    u, s, vh = backend.svd_batched(np.stack(mats))
+    u, s, vh = backend.to_host(u), s.cpu().numpy(), backend.to_host(vh)
+    for b in range(5):
+        np.testing.assert_allclose(s[b], svals[b], rtol=1e-9, atol=1e-14)
+        np.testing.assert_allclose(u[b].conj().T @ u[b], np.identity(k), atol=1e-12)
+        np.testing.assert_allclose(vh[b] @ vh[b].conj().T, np.identity(k), atol=1e-12)
+        np.testing.assert_allclose((u[b] * s[b]) @ vh[b], mats[b], atol=1e-13)
+
+
+def test_svd_batched_zero_padding_stays_exact():
+    """A block zero-padded into a larger shape: the first min(m_s, n_s) triplets
+    are the block's own, with exactly zero entries in the padding (also for a
+    rank-deficient and an all-zero block)."""
+    rng = np.random.default_rng(12)
+    P, Q = 50, 30
+    shapes = [(50, 30), (20, 12), (7, 7), (45, 3), (20, 12), (9, 5)]
+    blocks = [orc.crandn(sh, rng) for sh in shapes]
+    blocks[4] = orc.crandn((20, 2), rng) @ orc.crandn((2, 12), rng)   # rank 2
+    blocks[5] = np.zeros((9, 5), dtype=complex)
+    pad = np.zeros((len(shapes), P, Q), dtype=complex)
+    for b, (ms, ns) in enumerate(shapes):
+        pad[b, :ms, :ns] = blocks[b]
+    u, s, vh = backend.svd_batched(pad)
+    u, s, vh = backend.to_host(u), s.cpu().numpy(), backend.to_host(vh)
+    for b, (ms, ns) in enumerate(shapes):
+        kk = min(ms, ns)
+        assert np.all(u[b, ms:, :kk] == 0) and np.all(vh[b, :kk, ns:] == 0)
+        s_ref = np.linalg.svd(blocks[b], compute_uv=False)
+        np.testing.assert_allclose(s[b, :kk], s_ref, rtol=1e-11, atol=1e-13)
+        rec = (u[b, :ms, :kk] * s[b, :kk]) @ vh[b, :kk, :ns]
+        np.testing.assert_allclose(rec, blocks[b], atol=1e-13)
+        nz = s[b, :kk] > 1e-12
+        uu, vv = u[b, :ms, :kk][:, nz], vh[b, :kk, :ns][nz]
+        np.testing.assert_allclose(uu.conj().T @ uu, np.identity(nz.sum()), atol=1e-12)
+        np.testing.assert_allclose(vv @ vv.conj().T, np.identity(nz.sum()), atol=1e-12)
+
+
+def test_block_sparse_svd_many_sectors_matches_oracle():
+    """Sectors of many sizes, tall and wide (several batched size classes)."""
+    rng = np.random.default_rng(21)
+    q0 = rng.binomial(12, 0.5, size=300) - 6
+    q1 = rng.binomial(10, 0.45, size=240) - 5
+    a = orc.crandn((300, 240), rng) * (q0[:, None] == q1[None, :])
+    u, s, v, qs = backend.block_sparse_svd(a, q0, q1)
+    u0, s0, v0, qs0 = orc.block_sparse_svd(a, q0, q1)
+    assert np.array_equal(qs, qs0)
+    np.testing.assert_allclose(s, s0, rtol=1e-11, atol=1e-13)
+    uh, vh = backend.to_host(u), backend.to_host(v)
+    mask = q0[:, None] == qs[None, :]
+    assert np.all(uh[~mask] == 0) and np.all(vh[(q1[None, :] != qs[:, None])] == 0)
+    np.testing.assert_allclose(uh.conj().T @ uh, np.identity(len(s)), atol=1e-12)
+    np.testing.assert_allclose(vh @ vh.conj().T, np.identity(len(s)), atol=1e-12)
+    np.testing.assert_allclose((uh * s) @ vh, a, atol=1e-12)
+
+
 def test_qsparse_kernels():
     rng = np.random.default_rng(6)
     q0, q1, q2 = rng.integers(-2, 3, 5), np.array([1, -1]), rng.integers(-2, 3, 7)
