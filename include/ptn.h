@@ -201,6 +201,16 @@ int ptn_qr_batched(int64_t batch, int64_t m, int64_t n, const void* a, void* q, 
 size_t ptn_svd_workspace_batched(int64_t batch, int64_t m, int64_t n);
 int ptn_svd_batched(int64_t batch, int64_t m, int64_t n, const void* a, void* u, double* s,
                     void* vh, int32_t* info, void* ws, size_t ws_bytes, void* stream);
+/* As ptn_svd_batched, with FP32 preconditioning sweeps (matrices with at least
+ * 128 columns after the adjoint): the Jacobi iteration starts on a complex64
+ * copy on FP32 MFMA, its accumulated rotations are made unitary in FP64
+ * (polar factor by Newton-Schulz) and the FP64 sweeps finish from there under
+ * the same stopping rule, so the output has the FP64 path's accuracy. Used by
+ * the mixed-precision sweeps in place of np.linalg.svd at
+ * pytenet/block_sparse_util.py:294. info[3] = number of FP32 sweeps. */
+size_t ptn_svd_workspace_precond32(int64_t batch, int64_t m, int64_t n);
+int ptn_svd_precond32(int64_t batch, int64_t m, int64_t n, const void* a, void* u, double* s,
+                      void* vh, int32_t* info, void* ws, size_t ws_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -285,10 +285,12 @@ def qr(a):
     return q, r
 
 
-def svd(a, return_info=False):
+def svd(a, return_info=False, precond32=False):
     """Thin SVD of a 2-D complex128 device tensor: (u, s, vh), s descending (device float64).
     Synchronises the stream (the Jacobi sweeps are checked for convergence on the host).
-    With return_info, also returns the status words (sweeps, not converged, non-finite)."""
+    With return_info, also returns the status words (FP64 sweeps, not converged,
+    non-finite, FP32 sweeps). With precond32, the Jacobi iteration starts with
+    FP32 sweeps and finishes in FP64 (same output accuracy)."""
     a = as_dev(a)
     m, n = a.shape
     k = min(m, n)
@@ -296,12 +298,18 @@ def svd(a, return_info=False):
     vh = torch.empty((k, n), dtype=torch.complex128, device=a.device)
     s = torch.empty(k, dtype=torch.float64, device=a.device)
     if k == 0:
-        return u, s, vh
+        return (u, s, vh, np.zeros(4, dtype=np.int32)) if return_info else (u, s, vh)
     lib = _lib.load()
-    ws = workspace(lib.ptn_svd_workspace(m, n))
     info = torch.zeros(4, dtype=torch.int32, device=a.device)
-    _check(lib.ptn_svd(m, n, ptr(a), a.stride(0), ptr(u), k, ptr(s), ptr(vh), n, ptr(info),
-                       ptr(ws), ws.numel(), stream()), "ptn_svd")
+    if precond32:
+        a = a.contiguous()
+        ws = workspace(lib.ptn_svd_workspace_precond32(1, m, n))
+        _check(lib.ptn_svd_precond32(1, m, n, ptr(a), ptr(u), ptr(s), ptr(vh), ptr(info), ptr(ws),
+                                     ws.numel(), stream()), "ptn_svd_precond32")
+    else:
+        ws = workspace(lib.ptn_svd_workspace(m, n))
+        _check(lib.ptn_svd(m, n, ptr(a), a.stride(0), ptr(u), k, ptr(s), ptr(vh), n, ptr(info),
+                           ptr(ws), ws.numel(), stream()), "ptn_svd")
     inf = info.cpu().numpy()
     _svd_status(inf, f"a {m} x {n} matrix")
     if return_info:
@@ -424,7 +432,7 @@ def _svd_status(inf, what):
                       "pair met the orthogonality tolerance", RuntimeWarning)
 
 
-def svd_batched(a, return_info=False):
+def svd_batched(a, return_info=False, precond32=False):
     """Thin SVDs of a stack (B, m, n) of complex128 device matrices, all Jacobi
     sweeps run together: (u (B, m, k), s (B, k) device float64, vh (B, k, n))."""
     a = as_dev(a).contiguous()
@@ -436,10 +444,15 @@ def svd_batched(a, return_info=False):
     if k == 0 or nb == 0:
         return (u, s, vh, np.zeros(4, dtype=np.int32)) if return_info else (u, s, vh)
     lib = _lib.load()
-    ws = workspace(lib.ptn_svd_workspace_batched(nb, m, n))
     info = torch.zeros(4, dtype=torch.int32, device=a.device)
-    _check(lib.ptn_svd_batched(nb, m, n, ptr(a), ptr(u), ptr(s), ptr(vh), ptr(info), ptr(ws),
-                               ws.numel(), stream()), "ptn_svd_batched")
+    if precond32:
+        ws = workspace(lib.ptn_svd_workspace_precond32(nb, m, n))
+        fn, name = lib.ptn_svd_precond32, "ptn_svd_precond32"
+    else:
+        ws = workspace(lib.ptn_svd_workspace_batched(nb, m, n))
+        fn, name = lib.ptn_svd_batched, "ptn_svd_batched"
+    _check(fn(nb, m, n, ptr(a), ptr(u), ptr(s), ptr(vh), ptr(info), ptr(ws), ws.numel(),
+              stream()), name)
     inf = info.cpu().numpy()
     _svd_status(inf, f"{nb} matrices of {m} x {n}")
     if return_info:
@@ -458,7 +471,7 @@ def _svd_classes(shapes):
     return list(groups.values())
 
 
-def block_sparse_svd(a, q0, q1):
+def block_sparse_svd(a, q0, q1, precond32=False):
     """Per-sector SVD (block_sparse_util.py:244-310): (u, s, v, q) with s a host
     float64 array (sector order, descending within a sector) and u, v device
     tensors. The sector blocks are factorised in batches: each block is placed,
@@ -472,7 +485,7 @@ def block_sparse_svd(a, q0, q1):
     q0 = np.asarray(q0)
     q1 = np.asarray(q1)
     if not np.any(q0) and not np.any(q1):
-        u, s, vh = svd(a)
+        u, s, vh = svd(a, precond32=precond32)
         return u, s.cpu().numpy(), vh, np.zeros(len(s), dtype=q0.dtype)
     secs = _sectors(q0, q1)
     if not secs:
@@ -497,7 +510,7 @@ def block_sparse_svd(a, q0, q1):
                 pad[b, :ms, :ns] = blk
             else:
                 pad[b, :ns, :ms] = blk.mH
-        ub, sb, vb = svd_batched(pad)
+        ub, sb, vb = svd_batched(pad, precond32=precond32)
         for b, j in enumerate(group):
             ms, ns = shapes[j]
             kk = min(ms, ns)

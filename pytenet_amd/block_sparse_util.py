@@ -111,9 +111,10 @@ def block_sparse_qr(a, q0, q1):
     return backend.block_sparse_qr(a, np.asarray(q0), np.asarray(q1))
 
 
-def block_sparse_svd(a, q0, q1):
+def block_sparse_svd(a, q0, q1, precond32=False):
     """Block-wise SVD (block_sparse_util.py:244-319): per common sector, singular
     values in descending order, sectors in ascending quantum number.
-    `a` is a device tensor (complex128)."""
+    `a` is a device tensor (complex128). precond32 (extension) starts the Jacobi
+    iteration with FP32 sweeps; the result keeps FP64 accuracy."""
     from . import backend
-    return backend.block_sparse_svd(a, np.asarray(q0), np.asarray(q1))
+    return backend.block_sparse_svd(a, np.asarray(q0), np.asarray(q1), precond32)

@@ -37,10 +37,10 @@ def cap_bond_indices(s, idx, max_bond):
     return np.sort(idx[order[:max_bond]])
 
 
-def split_block_sparse_matrix_svd(a, q0, q1, tol, max_bond=None):
+def split_block_sparse_matrix_svd(a, q0, q1, tol, max_bond=None, svd_precond32=False):
     """Block-sparse SVD followed by truncation (bond_ops.py:41-54);
     returns (u, s, v, q) with u, v device tensors and s, q host arrays."""
-    u, s, v, q = block_sparse_svd(a, q0, q1)
+    u, s, v, q = block_sparse_svd(a, q0, q1, svd_precond32)
     idx = cap_bond_indices(s, retained_bond_indices(s, tol), max_bond)
     import torch
     it = torch.as_tensor(idx, device=u.device)

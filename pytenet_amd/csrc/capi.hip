@@ -13,10 +13,10 @@ int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, do
 size_t qr_ws_bytes(int64_t m, int64_t n, int64_t batch);
 int qr_run(int64_t m, int64_t n, const void* a, int64_t lda, void* q, int64_t ldq, void* r,
            int64_t ldr, void* ws, size_t ws_bytes, hipStream_t s, int64_t batch);
-size_t svd_ws_bytes(int64_t m, int64_t n, int64_t batch);
+size_t svd_ws_bytes(int64_t m, int64_t n, int64_t batch, bool precond32);
 int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t ldu, double* s,
             void* vh, int64_t ldvh, int32_t* info, void* ws, size_t ws_bytes, hipStream_t st,
-            int64_t batch, int64_t sa, int64_t su, int64_t ss, int64_t svh);
+            int64_t batch, int64_t sa, int64_t su, int64_t ss, int64_t svh, bool precond32);
 
 static thread_local hipError_t g_last_hip = hipSuccess;
 void ptn_set_last_hip_error(hipError_t e) { g_last_hip = e; }
@@ -258,23 +258,34 @@ int ptn_qr(int64_t m, int64_t n, const void* a, int64_t lda, void* q, int64_t ld
   return qr_run(m, n, a, lda, q, ldq, r, ldr, ws, ws_bytes, (hipStream_t)stream, 1);
 }
 
-size_t ptn_svd_workspace(int64_t m, int64_t n) { return svd_ws_bytes(m, n, 1); }
+size_t ptn_svd_workspace(int64_t m, int64_t n) { return svd_ws_bytes(m, n, 1, false); }
 
 int ptn_svd(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t ldu, double* s,
             void* vh, int64_t ldvh, int32_t* info, void* ws, size_t ws_bytes, void* stream) {
   return svd_run(m, n, a, lda, u, ldu, s, vh, ldvh, info, ws, ws_bytes, (hipStream_t)stream, 1,
-                 0, 0, 0, 0);
+                 0, 0, 0, 0, false);
 }
 
 size_t ptn_svd_workspace_batched(int64_t batch, int64_t m, int64_t n) {
-  return svd_ws_bytes(m, n, batch);
+  return svd_ws_bytes(m, n, batch, false);
 }
 
 int ptn_svd_batched(int64_t batch, int64_t m, int64_t n, const void* a, void* u, double* s,
                     void* vh, int32_t* info, void* ws, size_t ws_bytes, void* stream) {
   const int64_t k = m < n ? m : n;
   return svd_run(m, n, a, n, u, k, s, vh, n, info, ws, ws_bytes, (hipStream_t)stream, batch,
-                 m * n, m * k, k, k * n);
+                 m * n, m * k, k, k * n, false);
+}
+
+size_t ptn_svd_workspace_precond32(int64_t batch, int64_t m, int64_t n) {
+  return svd_ws_bytes(m, n, batch, true);
+}
+
+int ptn_svd_precond32(int64_t batch, int64_t m, int64_t n, const void* a, void* u, double* s,
+                      void* vh, int32_t* info, void* ws, size_t ws_bytes, void* stream) {
+  const int64_t k = m < n ? m : n;
+  return svd_run(m, n, a, n, u, k, s, vh, n, info, ws, ws_bytes, (hipStream_t)stream, batch,
+                 m * n, m * k, k, k * n, true);
 }
 
 int ptn_zgemm_batched(int64_t batch, int64_t M, int64_t N, int64_t K, const void* A, int64_t a_m,

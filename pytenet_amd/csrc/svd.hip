@@ -74,20 +74,40 @@ __device__ __forceinline__ int64_t pair_col(int bp, int bq, int c) {
   return (c < JB) ? (int64_t)bp * JB + c : (int64_t)bq * JB + (c - JB);
 }
 
+// element types of the two Jacobi precisions: complex storage, MFMA accumulator
+template <typename T> struct JacT;
+template <> struct JacT<double> { using c2 = double2; using acc = d4_t; };
+template <> struct JacT<float> { using c2 = float2; using acc = f4_t; };
+__device__ __forceinline__ d4_t jmfma(double a, double b, d4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4_t jmfma(float a, float b, f4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+// row of accumulator element v in lane group kk = lane / 16 of a 16 x 16 MFMA
+// result: kk + 4 v for v_mfma_f64_16x16x4f64, 4 kk + v for v_mfma_f32_16x16x4f32
+template <typename T> __device__ __forceinline__ int acc_row(int kk, int v) {
+  return sizeof(T) == 8 ? kk + 4 * v : 4 * kk + v;
+}
+
 // partial Gram matrices Gp[pair][seg][i][j] = sum_{rows in seg} conj(Y[row][i]) Y[row][j]
-// on FP64 MFMA (v_mfma_f64_16x16x4f64) through the real embedding of the
-// complex product with A = Y^H (real k = 2 row + part):
+// on MFMA (v_mfma_f64_16x16x4f64, or v_mfma_f32_16x16x4f32 for the FP32
+// preconditioning sweeps) through the real embedding of the complex product
+// with A = Y^H (real k = 2 row + part):
 //   A_real[i][2r] = Re Y[r][i], A_real[i][2r+1] = -Im Y[r][i];
 //   B_real[2r + q][2j + p] = (Re, Im; -Im, Re)[q][p] of Y[r][j].
 // The 32 x 64 real result is 8 tiles of 16 x 16; wave w owns row tile w & 1 and
 // column tiles 2 (w >> 1) and 2 (w >> 1) + 1. Rows are staged through LDS in
-// chunks of GCH.
+// chunks of GCH. blockIdx.z selects the problem of a batch (X at X + z sx).
 static_assert(JP == 32, "the Gram and apply kernels are laid out for 32-column block pairs");
-// blockIdx.z selects the problem of a batch (X at X + z sx).
+template <typename T>
 __global__ __launch_bounds__(256) void k_jac_gram(int64_t p, int nblk, int r,
-                                                  const double2* __restrict__ X,
-                                                  double2* __restrict__ Gp, int nseg, int64_t sx) {
-  __shared__ double2 Y[GCH][JP + 1];
+                                                  const typename JacT<T>::c2* __restrict__ X,
+                                                  typename JacT<T>::c2* __restrict__ Gp, int nseg,
+                                                  int64_t sx) {
+  using C2 = typename JacT<T>::c2;
+  using Acc = typename JacT<T>::acc;
+  __shared__ C2 Y[GCH][JP + 1];
   X += (int64_t)blockIdx.z * sx;
   int bp, bq;
   round_pair(nblk, r, blockIdx.y, bp, bq);
@@ -99,17 +119,17 @@ __global__ __launch_bounds__(256) void k_jac_gram(int64_t p, int nblk, int r,
   const int l16 = lane & 15, kk = lane >> 4;
   const int i0 = 16 * (wave & 1), jt0 = 2 * (wave >> 1);
   const int q = kk & 1, pb = l16 & 1;
-  d4_t acc[2] = {d4_t{0, 0, 0, 0}, d4_t{0, 0, 0, 0}};
+  Acc acc[2] = {Acc{0, 0, 0, 0}, Acc{0, 0, 0, 0}};
   // the next chunk is loaded into registers while the current one is multiplied
   constexpr int NL = GCH * JP / 256;
-  double2 pre[NL];
+  C2 pre[NL];
   auto load_chunk = [&](int64_t c0) {
 #pragma unroll
     for (int u = 0; u < NL; ++u) {
       const int e = t + 256 * u;
       const int col = e / GCH, row = e % GCH;
       const int64_t grow = c0 + row;
-      pre[u] = (grow < r1) ? X[pair_col(bp, bq, col) * p + grow] : make_double2(0.0, 0.0);
+      pre[u] = (grow < r1) ? X[pair_col(bp, bq, col) * p + grow] : C2{0, 0};
     }
   };
   load_chunk(r0);
@@ -125,23 +145,23 @@ __global__ __launch_bounds__(256) void k_jac_gram(int64_t p, int nblk, int r,
 #pragma unroll 4
     for (int s = 0; s < GCH / 2; ++s) {
       const int row = 2 * s + (kk >> 1);
-      const double2 ya = Y[row][i0 + l16];
-      const double a = q ? -ya.y : ya.x;
+      const C2 ya = Y[row][i0 + l16];
+      const T a = q ? -ya.y : ya.x;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const double2 yb = Y[row][8 * (jt0 + h) + (l16 >> 1)];
-        const double b = (q == pb) ? yb.x : (q ? -yb.y : yb.y);
-        acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[h], 0, 0, 0);
+        const C2 yb = Y[row][8 * (jt0 + h) + (l16 >> 1)];
+        const T b = (q == pb) ? yb.x : (q ? -yb.y : yb.y);
+        acc[h] = jmfma(a, b, acc[h]);
       }
     }
   }
-  // result element v of tile h: complex row i0 + kk + 4v, real column l16 of the
-  // tile (complex column 8 (jt0 + h) + l16 / 2, real part for even l16); the
-  // partner lane l16 ^ 1 holds the other part
-  double2* out = Gp + (((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * nseg + seg) * JP * JP;
+  // result element v of tile h: complex row i0 + acc_row(kk, v), real column l16
+  // of the tile (complex column 8 (jt0 + h) + l16 / 2, real part for even l16);
+  // the partner lane l16 ^ 1 holds the other part
+  C2* out = Gp + (((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * nseg + seg) * JP * JP;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
-    double mine[4], other[4];
+    T mine[4], other[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       mine[v] = acc[h][v];
@@ -151,12 +171,79 @@ __global__ __launch_bounds__(256) void k_jac_gram(int64_t p, int nblk, int r,
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       // even lanes store elements 0, 1 and odd lanes elements 2, 3 (constant indices)
-      const int i = i0 + kk + 4 * (2 * pb + u);
-      const double re = pb ? other[2 + u] : mine[u];
-      const double im = pb ? mine[2 + u] : other[u];
-      out[i * JP + j] = make_double2(re, im);
+      const int i = i0 + acc_row<T>(kk, 2 * pb + u);
+      const T re = pb ? other[2 + u] : mine[u];
+      const T im = pb ? mine[2 + u] : other[u];
+      out[i * JP + j] = C2{re, im};
     }
   }
+}
+
+// rotation parameters of the index pair (i, j) of a Hermitian Gram matrix with
+// diagonal entries a, b and off-diagonal entry c = G[i][j]: identity unless
+// |c| > tol sqrt(a b) (compared in squares, the Gram entries already hold
+// squared column scales); one reciprocal square root gives 1/|c|. Branch-free
+// (a skipped rotation computes on a harmless stand-in and selects the identity).
+__device__ __forceinline__ void jac_rotation(double a, double b, double2 c, double tol,
+                                             double big, double& cs, double& sr, double& si,
+                                             bool& rot, bool& rbig) {
+  const double tol2 = tol * tol, big2 = big * big;
+  const double ac2 = c.x * c.x + c.y * c.y;
+  const double ab = fmax(a, 0.0) * fmax(b, 0.0);
+  rot = ac2 > 0.0 && ac2 > tol2 * ab;
+  rbig = rot && ac2 > big2 * ab;
+  const double iac = rsqrt(rot ? ac2 : 1.0);
+  const double zeta = (b - a) * (0.5 * iac);
+  const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  const double c0 = rsqrt(1.0 + tt * tt);
+  const double sn = c0 * tt;
+  cs = rot ? c0 : 1.0;
+  sr = rot ? sn * (c.x * iac) : 0.0;   // s = sn e^{i phi}
+  si = rot ? sn * (c.y * iac) : 0.0;
+}
+// FP32 preconditioning sweeps: the same rotation from the hardware reciprocal,
+// reciprocal square root and square root (about 1 ulp each; the sweeps only
+// need rotations that are unitary to FP32 rounding). |c| is formed from c
+// scaled by mc = max(|Re c|, |Im c|), so no square underflows into the
+// denormals that the hardware reciprocal square root flushes to zero: Gram
+// entries down to 1e-36 (columns scaled to unit maximum norm, so column norms
+// down to about 1e-18) are rotated, smaller couplings are left to the FP64
+// sweeps. A zeta whose square overflows gives t = 0 (no rotation; |c| is then
+// negligible against |a - b|).
+__device__ __forceinline__ void jac_rotation(float a, float b, float2 c, float tol, float big,
+                                             float& cs, float& sr, float& si, bool& rot,
+                                             bool& rbig) {
+  const float mc0 = fmaxf(fabsf(c.x), fabsf(c.y));
+  const bool nz = mc0 > 1e-36f;
+  const float imc = __builtin_amdgcn_rcpf(nz ? mc0 : 1.0f);
+  const float cx = c.x * imc, cy = c.y * imc;   // max(|cx|, |cy|) = 1 when nz
+  const float r2 = nz ? cx * cx + cy * cy : 1.0f;
+  const float ir = __builtin_amdgcn_rsqf(r2);
+  const float ac = mc0 * r2 * ir;   // |c|
+  const float sab = __builtin_amdgcn_sqrtf(fmaxf(a, 0.0f)) * __builtin_amdgcn_sqrtf(fmaxf(b, 0.0f));
+  rot = nz && ac > tol * sab;
+  rbig = rot && ac > big * sab;
+  const float iac = imc * ir;
+  const float zeta = (b - a) * (0.5f * iac);
+  const float tt = copysignf(1.0f, zeta) *
+                   __builtin_amdgcn_rcpf(fabsf(zeta) + __builtin_amdgcn_sqrtf(1.0f + zeta * zeta));
+  const float c0 = __builtin_amdgcn_rsqf(1.0f + tt * tt);
+  const float sn = c0 * tt;
+  cs = rot ? c0 : 1.0f;
+  sr = rot ? sn * (cx * ir) : 0.0f;   // s = sn e^{i phi}, e^{i phi} = c / |c|
+  si = rot ? sn * (cy * ir) : 0.0f;
+}
+
+// players of pair k at step 0 of the round-robin over 32 (round_pair with
+// r = 0); from one step to the next both advance by one modulo 31, except the
+// fixed player 31 of pair 0
+__device__ __forceinline__ void rr_start(int k, int& p, int& q) {
+  p = k;
+  q = (k == 0) ? 31 : 31 - k;
+}
+__device__ __forceinline__ void rr_next(int k, int& p, int& q) {
+  p = (p == 30) ? 0 : p + 1;
+  if (k != 0) q = (q == 30) ? 0 : q + 1;
 }
 
 // one cyclic Jacobi sweep on the pair's Gram matrix (sum of the partials,
@@ -165,34 +252,38 @@ __global__ __launch_bounds__(256) void k_jac_gram(int64_t p, int nblk, int r,
 // disjoint index pairs (i_k, j_k) at once: with R_k = [[cs, s], [-conj(s), cs]]
 // acting on columns (i_k, j_k), G <- R^H G R is done as independent 2 x 2
 // block updates (rows of pair k, columns of pair l; one thread per (k, l))
-// and J <- J R as independent column-pair updates, so a step needs one
-// barrier after the rotation parameters and one after the updates.
-static_assert(JB * JB == 256, "the solve maps one thread to each (pair, pair) block of G");
-__global__ __launch_bounds__(256) void k_jac_solve(int nblk, const double2* __restrict__ Gp,
-                                                   int nseg, double tol, double big,
-                                                   double2* __restrict__ Jg,
+// and J <- J R as independent column-pair updates. Lane l of a wave computes
+// the rotation of pair l % JB (the column pair of its block) and takes that of
+// its row pair from lane k of the wave by a shuffle; G is read from one buffer
+// and the rotated G written to the other, so a step needs a single barrier.
+// `counters` counts the pairs with a rotation above `big`.
+static_assert(JB * JB == 256 && JB == 16, "the solve maps one thread to each (pair, pair) block of G");
+template <typename T>
+__global__ __launch_bounds__(256) void k_jac_solve(int nblk, const typename JacT<T>::c2* __restrict__ Gp,
+                                                   int nseg, T tol, T big,
+                                                   typename JacT<T>::c2* __restrict__ Jg,
                                                    int* __restrict__ rotflag,
                                                    int* __restrict__ counters) {
-  __shared__ double2 G[JP][JP + 1];
-  __shared__ double2 J[JP][JP + 1];
-  __shared__ double rc[JB], rsr[JB], rsi[JB];
+  using C2 = typename JacT<T>::c2;
+  __shared__ C2 G[2][JP][JP + 1];
+  __shared__ C2 J[JP][JP + 1];
   __shared__ int rot_any, rot_big;
   const int t = threadIdx.x;
   // pair blockIdx.x of problem blockIdx.y
   const int64_t pidx = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
-  const double2* gsrc = Gp + pidx * nseg * JP * JP;
+  const C2* gsrc = Gp + pidx * nseg * JP * JP;
   {
     // sum of the partial Gram matrices in a fixed order; the four elements of a
     // thread and several partials are loaded before they are added
     constexpr int NE = JP * JP / 256;
-    double sr[NE], si[NE];
+    T sr[NE], si[NE];
 #pragma unroll
     for (int u = 0; u < NE; ++u) { sr[u] = 0; si[u] = 0; }
 #pragma unroll 4
     for (int s = 0; s < nseg; ++s) {
 #pragma unroll
       for (int u = 0; u < NE; ++u) {
-        const double2 v = gsrc[(int64_t)s * JP * JP + t + 256 * u];
+        const C2 v = gsrc[(int64_t)s * JP * JP + t + 256 * u];
         sr[u] += v.x;
         si[u] += v.y;
       }
@@ -200,71 +291,60 @@ __global__ __launch_bounds__(256) void k_jac_solve(int nblk, const double2* __re
 #pragma unroll
     for (int u = 0; u < NE; ++u) {
       const int e = t + 256 * u;
-      G[e / JP][e % JP] = make_double2(sr[u], si[u]);
-      J[e / JP][e % JP] = make_double2((e / JP) == (e % JP) ? 1.0 : 0.0, 0.0);
+      G[0][e / JP][e % JP] = C2{sr[u], si[u]};
+      J[e / JP][e % JP] = C2{(e / JP) == (e % JP) ? T(1) : T(0), T(0)};
     }
   }
   if (t == 0) { rot_any = 0; rot_big = 0; }
   __syncthreads();
   const int bk = t / JB, bl = t % JB;     // the (row pair, column pair) block of this thread
+  int pk, qk, pl, ql;
+  rr_start(bk, pk, qk);
+  rr_start(bl, pl, ql);
+  bool any = false, anybig = false;
   for (int step = 0; step < JP - 1; ++step) {
-    if (t < JB) {
-      int i, j;
-      round_pair(JP, step, t, i, j);
-      const double a = G[i][i].x, b = G[j][j].x;
-      const double2 c = G[i][j];
-      // |c| > tol sqrt(a b) compared in squares (Gram entries already hold
-      // squared column scales); one reciprocal square root gives 1/|c|
-      const double ac2 = c.x * c.x + c.y * c.y;
-      const double ab = fmax(a, 0.0) * fmax(b, 0.0);
-      double cs = 1.0, sr = 0.0, si = 0.0;
-      if (ac2 > 0.0 && ac2 > (tol * tol) * ab) {
-        const double iac = rsqrt(ac2);
-        const double zeta = (b - a) * (0.5 * iac);
-        const double tt = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        cs = rsqrt(1.0 + tt * tt);
-        const double sn = cs * tt;
-        sr = sn * (c.x * iac);   // s = sn e^{i phi}
-        si = sn * (c.y * iac);
-        rot_any = 1;
-        if (ac2 > (big * big) * ab) rot_big = 1;
-      }
-      rc[t] = cs; rsr[t] = sr; rsi[t] = si;
-    }
-    __syncthreads();
+    const int cur = step & 1;
+    const int ik = min(pk, qk), jk = max(pk, qk), il = min(pl, ql), jl = max(pl, ql);
+    const C2 b00 = G[cur][ik][il], b01 = G[cur][ik][jl], b10 = G[cur][jk][il],
+             b11 = G[cur][jk][jl];
+    const T gaa = G[cur][il][il].x, gbb = G[cur][jl][jl].x;
+    const C2 gab = G[cur][il][jl];
+    const C2 x0 = J[2 * bk][il], y0 = J[2 * bk][jl];
+    const C2 x1 = J[2 * bk + 1][il], y1 = J[2 * bk + 1][jl];
+    T cl, slr, sli;
+    bool rl, bgl;
+    jac_rotation(gaa, gbb, gab, tol, big, cl, slr, sli, rl, bgl);
+    any |= rl;
+    anybig |= bgl;
+    // the row pair's rotation, computed by lane bk (< 16) of this wave
+    const T ck = __shfl(cl, bk), skr = __shfl(slr, bk), ski = __shfl(sli, bk);
     {
-      int ik, jk, il, jl;
-      round_pair(JP, step, bk, ik, jk);
-      round_pair(JP, step, bl, il, jl);
-      const double ck = rc[bk], skr = rsr[bk], ski = rsi[bk];
-      const double cl = rc[bl], slr = rsr[bl], sli = rsi[bl];
-      const double2 b00 = G[ik][il], b01 = G[ik][jl], b10 = G[jk][il], b11 = G[jk][jl];
       // columns: x' = cl x - conj(sl) y, y' = sl x + cl y (x = column il, y = column jl)
-      double2 c00, c01, c10, c11;
-      c00 = make_double2(cl * b00.x - (slr * b01.x + sli * b01.y), cl * b00.y - (slr * b01.y - sli * b01.x));
-      c10 = make_double2(cl * b10.x - (slr * b11.x + sli * b11.y), cl * b10.y - (slr * b11.y - sli * b11.x));
-      c01 = make_double2((slr * b00.x - sli * b00.y) + cl * b01.x, (slr * b00.y + sli * b00.x) + cl * b01.y);
-      c11 = make_double2((slr * b10.x - sli * b10.y) + cl * b11.x, (slr * b10.y + sli * b10.x) + cl * b11.y);
+      C2 c00, c01, c10, c11;
+      c00 = C2{cl * b00.x - (slr * b01.x + sli * b01.y), cl * b00.y - (slr * b01.y - sli * b01.x)};
+      c10 = C2{cl * b10.x - (slr * b11.x + sli * b11.y), cl * b10.y - (slr * b11.y - sli * b11.x)};
+      c01 = C2{(slr * b00.x - sli * b00.y) + cl * b01.x, (slr * b00.y + sli * b00.x) + cl * b01.y};
+      c11 = C2{(slr * b10.x - sli * b10.y) + cl * b11.x, (slr * b10.y + sli * b10.x) + cl * b11.y};
       // rows: row_i' = ck row_i - sk row_j, row_j' = conj(sk) row_i + ck row_j
-      G[ik][il] = make_double2(ck * c00.x - (skr * c10.x - ski * c10.y), ck * c00.y - (skr * c10.y + ski * c10.x));
-      G[ik][jl] = make_double2(ck * c01.x - (skr * c11.x - ski * c11.y), ck * c01.y - (skr * c11.y + ski * c11.x));
-      G[jk][il] = make_double2((skr * c00.x + ski * c00.y) + ck * c10.x, (skr * c00.y - ski * c00.x) + ck * c10.y);
-      G[jk][jl] = make_double2((skr * c01.x + ski * c01.y) + ck * c11.x, (skr * c01.y - ski * c01.x) + ck * c11.y);
+      C2 (*Gn)[JP + 1] = G[cur ^ 1];
+      Gn[ik][il] = C2{ck * c00.x - (skr * c10.x - ski * c10.y), ck * c00.y - (skr * c10.y + ski * c10.x)};
+      Gn[ik][jl] = C2{ck * c01.x - (skr * c11.x - ski * c11.y), ck * c01.y - (skr * c11.y + ski * c11.x)};
+      Gn[jk][il] = C2{(skr * c00.x + ski * c00.y) + ck * c10.x, (skr * c00.y - ski * c00.x) + ck * c10.y};
+      Gn[jk][jl] = C2{(skr * c01.x + ski * c01.y) + ck * c11.x, (skr * c01.y - ski * c01.x) + ck * c11.y};
     }
     // J <- J R: column pair bl for rows 2*bk and 2*bk + 1
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int row = 2 * bk + h;
-      int i, j;
-      round_pair(JP, step, bl, i, j);
-      const double cs = rc[bl], sr = rsr[bl], si = rsi[bl];
-      const double2 x = J[row][i], y = J[row][j];
-      J[row][i] = make_double2(cs * x.x - (sr * y.x + si * y.y), cs * x.y - (sr * y.y - si * y.x));
-      J[row][j] = make_double2((sr * x.x - si * x.y) + cs * y.x, (sr * x.y + si * x.x) + cs * y.y);
-    }
+    J[2 * bk][il] = C2{cl * x0.x - (slr * y0.x + sli * y0.y), cl * x0.y - (slr * y0.y - sli * y0.x)};
+    J[2 * bk][jl] = C2{(slr * x0.x - sli * x0.y) + cl * y0.x, (slr * x0.y + sli * x0.x) + cl * y0.y};
+    J[2 * bk + 1][il] = C2{cl * x1.x - (slr * y1.x + sli * y1.y), cl * x1.y - (slr * y1.y - sli * y1.x)};
+    J[2 * bk + 1][jl] = C2{(slr * x1.x - sli * x1.y) + cl * y1.x, (slr * x1.y + sli * x1.x) + cl * y1.y};
+    rr_next(bk, pk, qk);
+    rr_next(bl, pl, ql);
     __syncthreads();
   }
-  double2* jout = Jg + pidx * JP * JP;
+  if (any) rot_any = 1;
+  if (anybig) rot_big = 1;
+  __syncthreads();
+  C2* jout = Jg + pidx * JP * JP;
   for (int e = t; e < JP * JP; e += 256) jout[e] = J[e / JP][e % JP];
   if (t == 0) {
     rotflag[pidx] = rot_any;
@@ -273,29 +353,32 @@ __global__ __launch_bounds__(256) void k_jac_solve(int nblk, const double2* __re
 }
 
 // Y <- Y J (pass blockIdx.z & 1 = 0, rows of X) or V <- V J (pass 1) for AROWS rows
-// per workgroup, 16 per wave, on FP64 MFMA: the rows' 32 complex entries are
+// per workgroup, 16 per wave, on MFMA: the rows' 32 complex entries are
 // the A operand (real k = 2 column + part, the interleaved complex layout) and
 // J's real embedding [[Re, Im], [-Im, Re]] the B operand, read from LDS.
 // Every wave loads all A operands of its rows before it stores any result.
 // Problem blockIdx.z >> 1 of a batch: X (p x qp) and V (qp x qp) stacked.
+template <typename T>
 __global__ __launch_bounds__(256) void k_jac_apply(int64_t p, int64_t qp, int nblk, int r,
-                                                   double2* __restrict__ X,
-                                                   double2* __restrict__ V,
-                                                   const double2* __restrict__ Jg,
+                                                   typename JacT<T>::c2* __restrict__ X,
+                                                   typename JacT<T>::c2* __restrict__ V,
+                                                   const typename JacT<T>::c2* __restrict__ Jg,
                                                    const int* __restrict__ rotflag) {
-  __shared__ double2 J[JP][JP + 1];
+  using C2 = typename JacT<T>::c2;
+  using Acc = typename JacT<T>::acc;
+  __shared__ C2 J[JP][JP + 1];
   const int64_t prob = blockIdx.z >> 1;
   const int64_t pidx = prob * gridDim.y + blockIdx.y;
   if (!rotflag[pidx]) return;
   const int pass = blockIdx.z & 1;
-  double2* M = pass ? V + prob * qp * qp : X + prob * p * qp;
+  C2* M = pass ? V + prob * qp * qp : X + prob * p * qp;
   const int64_t ld = pass ? qp : p;
   const int64_t rowblk = (int64_t)blockIdx.x * AROWS;
   if (rowblk >= ld) return;
   int bp, bq;
   round_pair(nblk, r, blockIdx.y, bp, bq);
   const int t = threadIdx.x;
-  const double2* jsrc = Jg + pidx * JP * JP;
+  const C2* jsrc = Jg + pidx * JP * JP;
   for (int e = t; e < JP * JP; e += 256) J[e / JP][e % JP] = jsrc[e];
   __syncthreads();
   const int lane = t & 63, wave = t >> 6;
@@ -304,41 +387,41 @@ __global__ __launch_bounds__(256) void k_jac_apply(int64_t p, int64_t qp, int nb
   const int64_t row0 = rowblk + 16 * wave;
   const int64_t arow = row0 + l16;
   const bool aok = arow < ld;
-  double a[16];
+  T a[16];
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
     const int kc = 2 * s + (kk >> 1);
-    const double2 y = aok ? M[pair_col(bp, bq, kc) * ld + arow] : make_double2(0.0, 0.0);
+    const C2 y = aok ? M[pair_col(bp, bq, kc) * ld + arow] : C2{0, 0};
     a[s] = q ? y.y : y.x;
   }
-  d4_t acc[4];
+  Acc acc[4];
 #pragma unroll
-  for (int tile = 0; tile < 4; ++tile) acc[tile] = d4_t{0, 0, 0, 0};
+  for (int tile = 0; tile < 4; ++tile) acc[tile] = Acc{0, 0, 0, 0};
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
     const int kc = 2 * s + (kk >> 1);
 #pragma unroll
     for (int tile = 0; tile < 4; ++tile) {
-      const double2 jv = J[kc][8 * tile + (l16 >> 1)];
-      const double b = (q == pb) ? jv.x : (q ? -jv.y : jv.y);
-      acc[tile] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b, acc[tile], 0, 0, 0);
+      const C2 jv = J[kc][8 * tile + (l16 >> 1)];
+      const T b = (q == pb) ? jv.x : (q ? -jv.y : jv.y);
+      acc[tile] = jmfma(a[s], b, acc[tile]);
     }
   }
 #pragma unroll
   for (int tile = 0; tile < 4; ++tile) {
-    double mine[4], other[4];
+    T mine[4], other[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       mine[v] = acc[tile][v];
       other[v] = __shfl_xor(mine[v], 1);
     }
-    double2* col = M + pair_col(bp, bq, 8 * tile + (l16 >> 1)) * ld;
+    C2* col = M + pair_col(bp, bq, 8 * tile + (l16 >> 1)) * ld;
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      const int64_t row = row0 + kk + 4 * (2 * pb + u);
-      const double re = pb ? other[2 + u] : mine[u];
-      const double im = pb ? mine[2 + u] : other[u];
-      if (row < ld) col[row] = make_double2(re, im);
+      const int64_t row = row0 + acc_row<T>(kk, 2 * pb + u);
+      const T re = pb ? other[2 + u] : mine[u];
+      const T im = pb ? mine[2 + u] : other[u];
+      if (row < ld) col[row] = C2{re, im};
     }
   }
 }
@@ -480,12 +563,69 @@ __global__ void k_set_ints(int* p, int n, int v) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = v;
 }
 
-__global__ void k_svd_info(int32_t* info, int sweeps, int notconv, const int* counters) {
+__global__ void k_svd_info(int32_t* info, int sweeps, int notconv, const int* counters,
+                           int sweeps32) {
   if (threadIdx.x == 0) {
     info[0] = sweeps;
     info[1] = notconv;
     info[2] = counters[1];
-    info[3] = 0;
+    info[3] = sweeps32;
+  }
+}
+
+// scale[b] = largest column norm of problem b (1 when zero or not finite), one
+// workgroup per problem
+__global__ __launch_bounds__(256) void k_maxnorm(int64_t q, const double* __restrict__ nrm,
+                                                 int64_t sn, double* __restrict__ scale) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  nrm += blockIdx.x * sn;
+  double m = 0.0;
+  for (int64_t i = t; i < q; i += 256) m = fmax(m, nrm[i]);
+  red[t] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] = fmax(red[t], red[t + w]);
+    __syncthreads();
+  }
+  if (t == 0) scale[blockIdx.x] = (red[0] > 0.0 && isfinite(red[0])) ? red[0] : 1.0;
+}
+
+// X32 = X / scale[b] rounded to complex64 (n elements per problem, problem blockIdx.y)
+__global__ void k_narrow_scaled(int64_t n, const double2* __restrict__ X,
+                                const double* __restrict__ scale, float2* __restrict__ X32) {
+  X += blockIdx.y * n;
+  X32 += blockIdx.y * n;
+  const double inv = 1.0 / scale[blockIdx.y];
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const double2 x = X[t];
+    X32[t] = make_float2((float)(x.x * inv), (float)(x.y * inv));
+  }
+}
+
+// complex64 identity, q x q per problem (problem blockIdx.y)
+__global__ void k_eye32(int64_t q, float2* __restrict__ V) {
+  V += (int64_t)blockIdx.y * q * q;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < q * q;
+       t += (int64_t)gridDim.x * blockDim.x)
+    V[t] = make_float2((t / q) == (t % q) ? 1.0f : 0.0f, 0.0f);
+}
+
+__global__ void k_widen(int64_t n, const float2* __restrict__ x, double2* __restrict__ y) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    y[t] = make_double2((double)x[t].x, (double)x[t].y);
+}
+
+// Newton-Schulz step of the unitary polar factor: out = 1.5 Q - 0.5 Q (Q^H Q)
+// with QG = Q (Q^H Q) (n elements)
+__global__ void k_ns_update(int64_t n, const double2* __restrict__ Q, const double2* __restrict__ QG,
+                            double2* __restrict__ out) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const double2 a = Q[t], b = QG[t];
+    out[t] = make_double2(1.5 * a.x - 0.5 * b.x, 1.5 * a.y - 0.5 * b.y);
   }
 }
 
@@ -501,11 +641,16 @@ static int* pinned_counter() {
 struct SvdLayout {
   int64_t p, q, qp, nseg, batch;
   int nblk;
-  size_t bt, fact, fact2, rh, x, v, ub, vb, gp, jg, rflag, nrm, perm, cnt, total;
+  bool pre32;
+  size_t bt, fact, fact2, rh, x, v, ub, vb, gp, jg, rflag, nrm, perm, cnt, x32, v32, t1, t2, t3,
+      scale, total;
 };
 
+// FP32 preconditioning sweeps are used from this many columns up
+constexpr int64_t PRE32_MIN_Q = 4 * JP;
+
 // every per-problem region holds `batch` consecutive copies
-static SvdLayout svd_layout(int64_t m, int64_t n, int64_t batch) {
+static SvdLayout svd_layout(int64_t m, int64_t n, int64_t batch, bool precond32) {
   SvdLayout L;
   L.batch = batch;
   L.q = m < n ? m : n;
@@ -530,11 +675,23 @@ static SvdLayout svd_layout(int64_t m, int64_t n, int64_t batch) {
   L.nrm = off; off += al(B * L.qp * 8);
   L.perm = off; off += al(B * L.qp * 8);
   L.cnt = off; off += al(16);
+  L.pre32 = precond32 && L.q >= PRE32_MIN_Q;
+  L.x32 = L.v32 = L.t1 = L.t2 = L.t3 = L.scale = off;
+  if (L.pre32) {
+    L.x32 = off; off += al(B * L.q * L.qp * 8);
+    L.v32 = off; off += al(B * L.qp * L.qp * 8);
+    L.t1 = off; off += al(B * L.qp * L.qp * 16);
+    L.t2 = off; off += al(B * L.qp * L.qp * 16);
+    L.t3 = off; off += al(B * L.qp * L.qp * 16);
+    L.scale = off; off += al(B * 8);
+  }
   L.total = off;
   return L;
 }
 
-size_t svd_ws_bytes(int64_t m, int64_t n, int64_t batch) { return svd_layout(m, n, batch).total; }
+size_t svd_ws_bytes(int64_t m, int64_t n, int64_t batch, bool precond32) {
+  return svd_layout(m, n, batch, precond32).total;
+}
 
 // grid width of the grid-stride helper kernels for `total` elements per problem
 static inline unsigned gs_blocks(int64_t total) {
@@ -548,14 +705,23 @@ static inline unsigned gs_blocks(int64_t total) {
 // `big` is followed by one final sweep. info (if given) receives the number of
 // sweeps, whether the sweep limit was hit, and whether any singular value was
 // not finite.
+// With `precond32` (and at least PRE32_MIN_Q columns) the Jacobi phase starts
+// with FP32 sweeps: X32 = X / max column norm in complex64 is orthogonalised
+// on FP32 MFMA until no pair needs a rotation above 1e-3 relative, with the
+// rotations accumulated in V32. V32 is unitary only to FP32 rounding, so it is
+// replaced by its unitary polar factor Q (two Newton-Schulz steps in FP64,
+// Q <- 1.5 Q - 0.5 Q (Q^H Q), each squaring the distance from unitarity), and
+// the FP64 sweeps continue from X Q with V = Q. The FP64 sweeps and their
+// stopping rule are those of the plain path, so the result has the same FP64
+// accuracy; the FP32 sweeps only remove most of the rotation work.
 int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t ldu, double* s,
             void* vh, int64_t ldvh, int32_t* info, void* ws, size_t ws_bytes, hipStream_t st,
-            int64_t batch, int64_t sa, int64_t su, int64_t ss, int64_t svh) {
+            int64_t batch, int64_t sa, int64_t su, int64_t ss, int64_t svh, bool precond32) {
   if (m <= 0 || n <= 0 || lda < n) return PTN_ERR_ARG;
   const int64_t k = m < n ? m : n;
   if (ldu < k || ldvh < n) return PTN_ERR_ARG;
   if (batch < 1 || batch > 32767) return PTN_ERR_ARG;
-  const SvdLayout L = svd_layout(m, n, batch);
+  const SvdLayout L = svd_layout(m, n, batch, precond32);
   if (ws_bytes < L.total) return PTN_ERR_WORKSPACE;
   const int npairs = L.nblk / 2;
   const int64_t aseg = (L.qp + AROWS - 1) / AROWS;   // qp >= q rows
@@ -598,10 +764,68 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
   if ((stt = qr_factor(f2, RH, L.q, st, L.q * L.q))) return stt;
   k_rh<<<dim3(gs_blocks(L.q * L.qp), NB), 256, 0, st>>>(L.q, L.q, L.q, L.qp, 1, f2.W, f2.rdiag, X,
                                                         f2.bs.W, f2.bs.k);
-  k_eye<<<dim3(gs_blocks(L.qp * L.qp), NB), 256, 0, st>>>(L.qp, V);
   PTN_LAUNCH_CHECK();
   int* host_rot = pinned_counter();
   if (!host_rot) return PTN_ERR_HIP;
+  int sweeps32 = 0;
+  k_set_ints<<<1, 64, 0, st>>>(counters, 2, 0);
+  if (L.pre32) {
+    float2* X32 = (float2*)(base + L.x32);
+    float2* V32 = (float2*)(base + L.v32);
+    float2* Gp32 = (float2*)Gp;
+    float2* Jg32 = (float2*)Jg;
+    double* scale = (double*)(base + L.scale);
+    k_colnorms<<<dim3((unsigned)L.q, NB), 256, 0, st>>>(L.q, X, nrm, sx, L.qp);
+    k_maxnorm<<<NB, 256, 0, st>>>(L.q, nrm, L.qp, scale);
+    k_narrow_scaled<<<dim3(gs_blocks(sx), NB), 256, 0, st>>>(sx, X, scale, X32);
+    k_eye32<<<dim3(gs_blocks(L.qp * L.qp), NB), 256, 0, st>>>(L.qp, V32);
+    PTN_LAUNCH_CHECK();
+    const float tol32 = 32.0f * FLT_EPSILON, big32 = 1e-3f;
+    const int max_sweeps32 = 20;
+    while (sweeps32 < max_sweeps32) {
+      k_set_ints<<<1, 64, 0, st>>>(counters, 1, 0);
+      for (int r = 0; r < L.nblk - 1; ++r) {
+        k_jac_gram<float><<<dim3((unsigned)L.nseg, (unsigned)npairs, NB), 256, 0, st>>>(
+            L.q, L.nblk, r, X32, Gp32, (int)L.nseg, sx);
+        k_jac_solve<float><<<dim3((unsigned)npairs, NB), 256, 0, st>>>(
+            L.nblk, Gp32, (int)L.nseg, tol32, big32, Jg32, rflag, counters);
+        k_jac_apply<float><<<dim3((unsigned)aseg, (unsigned)npairs, 2 * NB), 256, 0, st>>>(
+            L.q, L.qp, L.nblk, r, X32, V32, Jg32, rflag);
+      }
+      PTN_LAUNCH_CHECK();
+      PTN_HIP_CHECK(hipMemcpyAsync(host_rot, counters, sizeof(int), hipMemcpyDeviceToHost, st));
+      PTN_HIP_CHECK(hipStreamSynchronize(st));
+      ++sweeps32;
+      if (*host_rot == 0) break;
+    }
+    // unitary polar factor of V32 (qp x qp column-major per problem) into V
+    const int64_t nv = L.qp * L.qp, nvb = nv * batch;
+    double2* Q = (double2*)(base + L.t1);
+    double2* QhQ = (double2*)(base + L.t2);
+    double2* QQ = (double2*)(base + L.t3);
+    k_widen<<<gs_blocks(nvb), 256, 0, st>>>(nvb, V32, Q);
+    PTN_LAUNCH_CHECK();
+    for (int it = 0; it < 2; ++it) {
+      ZGemm g1 = {L.qp, L.qp, L.qp, Q, L.qp, 1, 1, Q, 1, L.qp, 0, QhQ, 1, L.qp, 0,
+                  batch, nv, nv, nv};
+      if ((stt = zgemm(g1, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, st)))
+        return stt;
+      ZGemm g2 = {L.qp, L.qp, L.qp, Q, 1, L.qp, 0, QhQ, 1, L.qp, 0, QQ, 1, L.qp, 0,
+                  batch, nv, nv, nv};
+      if ((stt = zgemm(g2, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, st)))
+        return stt;
+      k_ns_update<<<gs_blocks(nvb), 256, 0, st>>>(nvb, Q, QQ, it == 1 ? V : Q);
+      PTN_LAUNCH_CHECK();
+    }
+    // X <- X V (q x qp column-major per problem), into t3
+    ZGemm g3 = {L.q, L.qp, L.qp, X, 1, L.q, 0, V, 1, L.qp, 0, QQ, 1, L.q, 0,
+                batch, sx, nv, sx};
+    if ((stt = zgemm(g3, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, st))) return stt;
+    X = QQ;
+  } else {
+    k_eye<<<dim3(gs_blocks(L.qp * L.qp), NB), 256, 0, st>>>(L.qp, V);
+    PTN_LAUNCH_CHECK();
+  }
   const double tol = DBL_EPSILON * sqrt((double)L.q);
   // a sweep without rotations above this relative size is followed by one
   // final sweep (quadratic convergence takes it to tol) instead of repeating
@@ -609,15 +833,14 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
   const double big = 1e-7;
   const int max_sweeps = 60;
   int sweeps = 0, converged = (L.q < 2), final_sweep = 0;
-  k_set_ints<<<1, 64, 0, st>>>(counters, 2, 0);
   while (!converged && sweeps < max_sweeps) {
     k_set_ints<<<1, 64, 0, st>>>(counters, 1, 0);
     for (int r = 0; r < L.nblk - 1; ++r) {
-      k_jac_gram<<<dim3((unsigned)L.nseg, (unsigned)npairs, NB), 256, 0, st>>>(
+      k_jac_gram<double><<<dim3((unsigned)L.nseg, (unsigned)npairs, NB), 256, 0, st>>>(
           L.q, L.nblk, r, X, Gp, (int)L.nseg, sx);
-      k_jac_solve<<<dim3((unsigned)npairs, NB), 256, 0, st>>>(L.nblk, Gp, (int)L.nseg, tol, big,
-                                                              Jg, rflag, counters);
-      k_jac_apply<<<dim3((unsigned)aseg, (unsigned)npairs, 2 * NB), 256, 0, st>>>(
+      k_jac_solve<double><<<dim3((unsigned)npairs, NB), 256, 0, st>>>(L.nblk, Gp, (int)L.nseg, tol,
+                                                                      big, Jg, rflag, counters);
+      k_jac_apply<double><<<dim3((unsigned)aseg, (unsigned)npairs, 2 * NB), 256, 0, st>>>(
           L.q, L.qp, L.nblk, r, X, V, Jg, rflag);
     }
     PTN_LAUNCH_CHECK();
@@ -639,7 +862,7 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
   k_svd_out<<<dim3((unsigned)((tot + 255) / 256), NB), 256, 0, st>>>(
       m, n, k, adjoint, L.p, L.q, UB, VB, perm, (double2*)u, ldu, (double2*)vh, ldvh, L.qp, su,
       svh);
-  if (info) k_svd_info<<<1, 64, 0, st>>>(info, sweeps, converged ? 0 : 1, counters);
+  if (info) k_svd_info<<<1, 64, 0, st>>>(info, sweeps, converged ? 0 : 1, counters, sweeps32);
   PTN_LAUNCH_CHECK();
   return PTN_OK;
 }

@@ -61,8 +61,12 @@ def dmrg_singlesite(hamiltonian: MPO, psi: MPS, numsweeps: int, numiter_lanczos:
 
 
 def dmrg_twosite(hamiltonian: MPO, psi: MPS, numsweeps: int, numiter_lanczos: int = 25,
-                 tol_split: float = 0, precision: str = "fp64", max_bond=None):
-    """Two-site DMRG (dmrg.py:96-178): returns the energy after each sweep."""
+                 tol_split: float = 0, precision: str = "fp64", max_bond=None,
+                 svd_precond32=None):
+    """Two-site DMRG (dmrg.py:96-178): returns the energy after each sweep.
+    svd_precond32: split SVD with FP32 Jacobi preconditioning sweeps (FP64
+    output accuracy); None = on when precision is not "fp64"."""
+    pre32 = (precision != "fp64") if svd_precond32 is None else bool(svd_precond32)
     nsites = hamiltonian.nsites
     assert nsites == psi.nsites
     ctx = SweepContext(hamiltonian, psi, numiter_lanczos, precision)
@@ -80,14 +84,14 @@ def dmrg_twosite(hamiltonian: MPO, psi: MPS, numsweeps: int, numiter_lanczos: in
             en, a2 = ctx.site_eigh(a2, lblocks[i], h2[i], rblocks[i + 1])
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "right", tol=tol_split,
-                max_bond=max_bond)
+                max_bond=max_bond, svd_precond32=pre32)
             lblocks[i + 1] = ctx.env_left(psi.a[i], W[i], lblocks[i])
         for i in reversed(range(nsites - 1)):
             a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
             en, a2 = ctx.site_eigh(a2, lblocks[i], h2[i], rblocks[i + 1])
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=tol_split,
-                max_bond=max_bond)
+                max_bond=max_bond, svd_precond32=pre32)
             rblocks[i] = ctx.env_right(psi.a[i + 1], W[i + 1], rblocks[i + 1])
         psi.a[0], _, psi.qbonds[0] = mps_local_orthonormalize_right_qr(
             psi.a[0], ctx.one(), psi.qsite, psi.qbonds[:2])

@@ -239,11 +239,13 @@ def mps_merge_tensor_pair(a0, a1):
     return _like(backend.merge_pair(a0, a1), host)
 
 
-def mps_split_tensor_svd(a, qsite0, qsite1, qbonds_outer, svd_distr: str, tol=0, max_bond=None):
+def mps_split_tensor_svd(a, qsite0, qsite1, qbonds_outer, svd_distr: str, tol=0, max_bond=None,
+                         svd_precond32=False):
     """Split (b0, d0*d1, b2) into (b0, d0, k) and (k, d1, b2) by a block-sparse
     SVD with discarded-weight truncation; sigma multiplied into the left or right
-    factor, or its square root into both (mps.py:538-568). `max_bond` caps the new
-    bond (extension; None reproduces the reference)."""
+    factor, or its square root into both (mps.py:538-568). Extensions: `max_bond`
+    caps the new bond (None reproduces the reference); `svd_precond32` starts the
+    Jacobi SVD with FP32 sweeps (FP64 output accuracy)."""
     host = _is_host(a)
     a = backend.as_dev(a)
     assert a.dim() == 3
@@ -256,7 +258,7 @@ def mps_split_tensor_svd(a, qsite0, qsite1, qbonds_outer, svd_distr: str, tol=0,
     b0, b2 = a.shape[0], a.shape[2]
     q0 = qnumber_flatten([qbonds_outer[0], qsite0])
     q1 = qnumber_flatten([-qsite1, qbonds_outer[1]])
-    u, s, v, q = block_sparse_svd(a.reshape(b0 * d0, d1 * b2), q0, q1)
+    u, s, v, q = block_sparse_svd(a.reshape(b0 * d0, d1 * b2), q0, q1, svd_precond32)
     idx = cap_bond_indices(s, retained_bond_indices(s, tol), max_bond)
     k = len(idx)
     if k != len(s):

@@ -4,10 +4,13 @@ environment blocks and all local Krylov solves resident on the GPU.
 
 Same signatures as the reference plus keyword extensions with
 reference-preserving defaults:
-  precision: "fp64" (default; the parity mode) or "fp32" for the dense
-             contractions (Krylov scalars, QR and SVD stay FP64);
+  precision: "fp64" (default; the parity mode), "fp32" or "bf16x3" for the
+             dense contractions (Krylov scalars, QR and SVD stay FP64);
   max_bond:  bond-dimension cap for the two-site split (None = no cap, as
-             in the reference).
+             in the reference);
+  svd_precond32: two-site split SVD with FP32 Jacobi preconditioning sweeps
+             finished in FP64 (same output accuracy); None = on when
+             precision is not "fp64".
 
 Reference:
     J. Haegeman, C. Lubich, I. Oseledets, B. Vandereycken, F. Verstraete
@@ -74,12 +77,13 @@ class TwoSiteTDVP:
     sweep() (one symmetric time step), so that sweeps can be timed on their own."""
 
     def __init__(self, hamiltonian: MPO, psi: MPS, dt, numiter_lanczos: int = 25, tol_split=0,
-                 precision: str = "fp64", max_bond=None):
+                 precision: str = "fp64", max_bond=None, svd_precond32=None):
         self.nsites = hamiltonian.nsites
         assert self.nsites == psi.nsites
         assert self.nsites >= 2
         self.H, self.psi, self.dt = hamiltonian, psi, dt
         self.tol, self.max_bond = tol_split, max_bond
+        self.pre32 = (precision != "fp64") if svd_precond32 is None else bool(svd_precond32)
         self.ctx = SweepContext(hamiltonian, psi, numiter_lanczos, precision)
 
     def setup(self):
@@ -99,7 +103,7 @@ class TwoSiteTDVP:
             a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -0.5 * dt)
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, (psi.qbonds[i], psi.qbonds[i + 2]), "right", tol=self.tol,
-                max_bond=self.max_bond)
+                max_bond=self.max_bond, svd_precond32=self.pre32)
             lblocks[i + 1] = ctx.env_left(psi.a[i], W[i], lblocks[i])
             psi.a[i + 1] = ctx.site_expm(psi.a[i + 1], lblocks[i + 1], W[i + 1], rblocks[i + 1],
                                          0.5 * dt)
@@ -108,7 +112,7 @@ class TwoSiteTDVP:
         a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -dt)
         psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
             a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=self.tol,
-            max_bond=self.max_bond)
+            max_bond=self.max_bond, svd_precond32=self.pre32)
         rblocks[i] = ctx.env_right(psi.a[i + 1], W[i + 1], rblocks[i + 1])
         for i in reversed(range(n - 2)):
             psi.a[i + 1] = ctx.site_expm(psi.a[i + 1], lblocks[i + 1], W[i + 1], rblocks[i + 1],
@@ -117,7 +121,7 @@ class TwoSiteTDVP:
             a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -0.5 * dt)
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=self.tol,
-                max_bond=self.max_bond)
+                max_bond=self.max_bond, svd_precond32=self.pre32)
             rblocks[i] = ctx.env_right(psi.a[i + 1], W[i + 1], rblocks[i + 1])
         ctx.flush()
 
@@ -126,10 +130,11 @@ class TwoSiteTDVP:
 
 
 def tdvp_twosite(hamiltonian: MPO, psi: MPS, dt, numsteps: int, numiter_lanczos: int = 25,
-                 tol_split=0, precision: str = "fp64", max_bond=None):
+                 tol_split=0, precision: str = "fp64", max_bond=None, svd_precond32=None):
     """Symmetric two-site TDVP (tdvp.py:121-220). `psi` is overwritten with the
     evolved state; returns the norm of the initial psi."""
-    run = TwoSiteTDVP(hamiltonian, psi, dt, numiter_lanczos, tol_split, precision, max_bond)
+    run = TwoSiteTDVP(hamiltonian, psi, dt, numiter_lanczos, tol_split, precision, max_bond,
+                      svd_precond32)
     nrm = run.setup()
     for _ in range(numsteps):
         run.sweep()

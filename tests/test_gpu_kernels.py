@@ -262,20 +262,45 @@ def test_svd_batched_matches_lapack(m, n):
         np.testing.assert_allclose((u[b] * s[b]) @ vh[b], mats[b], atol=1e-13)
 
 
-def test_svd_batched_zero_padding_stays_exact():
+@pytest.mark.parametrize("m,n", [(130, 128), (300, 200), (200, 300), (256, 256)])
+def test_svd_precond32_matches_lapack(m, n):
+    """FP32 preconditioning sweeps, FP64 finish: the FP64 path's tolerances on a
+    spectrum graded to 1e-9 and on an MPS-like exponentially decaying one."""
+    rng = np.random.default_rng(m * 3 + n)
+    k = min(m, n)
+    u0, _, v0 = np.linalg.svd(orc.crandn((m, n), rng), full_matrices=False)
+    for s0 in (np.logspace(0, -9, k), np.exp(-np.arange(k) / (k / 10))):
+        a = (u0 * s0) @ v0
+        u, s, vh, inf = backend.svd(dev(a), return_info=True, precond32=True)
+        assert inf[3] > 0 and inf[1] == 0
+        s = s.cpu().numpy()
+        np.testing.assert_allclose(s, s0, rtol=1e-9, atol=1e-14)
+        uh, vhh = backend.to_host(u), backend.to_host(vh)
+        np.testing.assert_allclose(uh.conj().T @ uh, np.identity(k), atol=1e-12)
+        np.testing.assert_allclose(vhh @ vhh.conj().T, np.identity(k), atol=1e-12)
+        np.testing.assert_allclose((uh * s) @ vhh, a, atol=1e-13)
+
+
+@pytest.mark.parametrize("precond32", [False, True])
+def test_svd_batched_zero_padding_stays_exact(precond32):
     """A block zero-padded into a larger shape: the first min(m_s, n_s) triplets
     are the block's own, with exactly zero entries in the padding (also for a
     rank-deficient and an all-zero block)."""
     rng = np.random.default_rng(12)
-    P, Q = 50, 30
-    shapes = [(50, 30), (20, 12), (7, 7), (45, 3), (20, 12), (9, 5)]
+    if precond32:
+        P, Q = 200, 160
+        shapes = [(200, 160), (150, 140), (90, 60), (190, 30), (100, 80), (9, 5)]
+    else:
+        P, Q = 50, 30
+        shapes = [(50, 30), (20, 12), (7, 7), (45, 3), (20, 12), (9, 5)]
     blocks = [orc.crandn(sh, rng) for sh in shapes]
-    blocks[4] = orc.crandn((20, 2), rng) @ orc.crandn((2, 12), rng)   # rank 2
+    blocks[4] = orc.crandn((shapes[4][0], 2), rng) @ orc.crandn((2, shapes[4][1]), rng)   # rank 2
     blocks[5] = np.zeros((9, 5), dtype=complex)
     pad = np.zeros((len(shapes), P, Q), dtype=complex)
     for b, (ms, ns) in enumerate(shapes):
         pad[b, :ms, :ns] = blocks[b]
-    u, s, vh = backend.svd_batched(pad)
+    u, s, vh, inf = backend.svd_batched(pad, return_info=True, precond32=precond32)
+    assert (inf[3] > 0) == precond32
     u, s, vh = backend.to_host(u), s.cpu().numpy(), backend.to_host(vh)
     for b, (ms, ns) in enumerate(shapes):
         kk = min(ms, ns)
