@@ -276,6 +276,7 @@ __global__ __launch_bounds__(256) void k_qr_col2(int64_t m, int64_t j, double2* 
   const double2 zk = (j + 1 < m) ? z(j + 1) : make_double2(0.0, 0.0);
   const double2 yz = cmul2(conj2(h2.sc), make_double2(d3[1], d3[2]));
   const double2 f2 = cmul2(conj2(h2.tau), make_double2(zk.x + yz.x, zk.y + yz.y));
+  __syncthreads();   // every thread has read row j+1 of x_c (zk) before it is overwritten
   // every row is read into registers before the row is written (same thread)
   for (int64_t i = j + t; i < m; i += 256) {
     double2 o = z(i);
@@ -285,6 +286,144 @@ __global__ __launch_bounds__(256) void k_qr_col2(int64_t m, int64_t j, double2* 
       o.y -= g.y;
     }
     xc[i] = o;
+  }
+}
+
+// k_qr_col2 with the rows j .. m-1 of the three columns held in registers
+// (row j + t + 256 u in thread t, u < R; requires m - j <= 256 R): one load of
+// each column instead of one per pass. Every workgroup of a launch runs the
+// same reductions in the same order, so the reflectors agree bitwise across
+// workgroups (the grouping of rows per thread differs from k_qr_col2, so the
+// two kernels agree to rounding).
+template <int R>
+__global__ __launch_bounds__(256) void k_qr_col2r(int64_t m, int64_t j, double2* __restrict__ W,
+                                                  double2* __restrict__ Vf,
+                                                  double2* __restrict__ tau,
+                                                  double2* __restrict__ rdiag,
+                                                  double2* __restrict__ rsup, QrStrides bs) {
+  W += blockIdx.y * bs.W;
+  Vf += blockIdx.y * bs.V;
+  tau += blockIdx.y * bs.k;
+  rdiag += blockIdx.y * bs.k;
+  rsup += blockIdx.y * bs.k;
+  const double2* xj = W + j * m;
+  const double2* xk = W + (j + 1) * m;
+  const int b = blockIdx.x;
+  const int t = threadIdx.x;
+  double2* xc = W + (j + b) * m;
+  const bool own = b >= 2;
+  double2 rj[R], rk[R], rc[R];
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int64_t i = j + t + 256 * u;
+    const bool in = i < m;
+    rj[u] = in ? xj[i] : make_double2(0.0, 0.0);
+    rk[u] = (in && b >= 1) ? xk[i] : make_double2(0.0, 0.0);
+    rc[u] = (in && own) ? xc[i] : make_double2(0.0, 0.0);
+  }
+  const double2 aj = xj[j];
+  double n1[1] = {0};
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int64_t i = j + t + 256 * u;
+    if (i > j && i < m) n1[0] += rj[u].x * rj[u].x + rj[u].y * rj[u].y;
+  }
+  block_sum<1>(n1);
+  const Refl h1 = make_refl(aj, sqrt(n1[0]));
+  auto v1r = [&](int u, int64_t i) -> double2 {
+    return i == j ? make_double2(1.0, 0.0) : cmul2(rj[u], h1.sc);
+  };
+  if (b == 0) {
+    double2* vj = Vf + j * m;
+    for (int64_t i = t; i < j; i += 256) vj[i] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const int64_t i = j + t + 256 * u;
+      if (i < m) vj[i] = v1r(u, i);
+    }
+    if (t == 0) { tau[j] = h1.tau; rdiag[j] = h1.rd; }
+    return;
+  }
+  double d2[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int64_t i = j + t + 256 * u;
+    if (i < m) {
+      const double2 v = v1r(u, i), x = rk[u];
+      d2[0] += v.x * x.x + v.y * x.y;
+      d2[1] += v.x * x.y - v.y * x.x;
+      if (own) {
+        const double2 z = rc[u];
+        d2[2] += v.x * z.x + v.y * z.y;
+        d2[3] += v.x * z.y - v.y * z.x;
+      }
+    }
+  }
+  block_sum<4>(d2);
+  const double2 f1 = cmul2(conj2(h1.tau), make_double2(d2[0], d2[1]));
+  const double2 fc = cmul2(conj2(h1.tau), make_double2(d2[2], d2[3]));
+  double2 y[R], z[R];
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int64_t i = j + t + 256 * u;
+    const double2 v = v1r(u, i), g = cmul2(f1, v), gc = cmul2(fc, v);
+    y[u] = make_double2(rk[u].x - g.x, rk[u].y - g.y);
+    z[u] = make_double2(rc[u].x - gc.x, rc[u].y - gc.y);
+  }
+  double d3[3] = {0, 0, 0};
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int64_t i = j + t + 256 * u;
+    if (i >= j + 2 && i < m) {
+      d3[0] += y[u].x * y[u].x + y[u].y * y[u].y;
+      if (own) {
+        d3[1] += y[u].x * z[u].x + y[u].y * z[u].y;
+        d3[2] += y[u].x * z[u].y - y[u].y * z[u].x;
+      }
+    }
+  }
+  block_sum<3>(d3);
+  // rows j and j+1 of y and z from memory (every thread needs them)
+  auto yg = [&](int64_t i) -> double2 {
+    const double2 v = i == j ? make_double2(1.0, 0.0) : cmul2(xj[i], h1.sc), g = cmul2(f1, v),
+                  x = xk[i];
+    return make_double2(x.x - g.x, x.y - g.y);
+  };
+  const double2 alpha2 = yg(j + 1);
+  const Refl h2 = make_refl(alpha2, sqrt(d3[0]));
+  if (b == 1) {
+    double2* vk = Vf + (j + 1) * m;
+    for (int64_t i = t; i <= j; i += 256) vk[i] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const int64_t i = j + t + 256 * u;
+      if (i > j && i < m) vk[i] = i == j + 1 ? make_double2(1.0, 0.0) : cmul2(y[u], h2.sc);
+    }
+    if (t == 0) { tau[j + 1] = h2.tau; rdiag[j + 1] = h2.rd; rsup[j] = yg(j); }
+    return;
+  }
+  // w2 = v2^H z = z_{j+1} + conj(sc2) (y^H z)_{> j+1};  x_c <- z - conj(tau2) w2 v2
+  double2 zk = make_double2(0.0, 0.0);
+  if (j + 1 < m) {
+    const double2 v = cmul2(xj[j + 1], h1.sc), g = cmul2(fc, v), x = xc[j + 1];
+    zk = make_double2(x.x - g.x, x.y - g.y);
+  }
+  const double2 yz = cmul2(conj2(h2.sc), make_double2(d3[1], d3[2]));
+  const double2 f2 = cmul2(conj2(h2.tau), make_double2(zk.x + yz.x, zk.y + yz.y));
+  __syncthreads();   // every thread has read row j+1 of x_c (zk) before it is overwritten
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int64_t i = j + t + 256 * u;
+    if (i < m) {
+      double2 o = z[u];
+      if (i >= j + 1) {
+        const double2 v2 = i == j + 1 ? make_double2(1.0, 0.0) : cmul2(y[u], h2.sc);
+        const double2 g = cmul2(f2, v2);
+        o.x -= g.x;
+        o.y -= g.y;
+      }
+      xc[i] = o;
+    }
   }
 }
 
@@ -508,9 +647,18 @@ int qr_factor(QrFact& f, const double2* a, int64_t lda, hipStream_t s, int64_t s
     const int64_t pe = (j0 + QR_NB < k) ? j0 + QR_NB : k;
     const int nbp = (int)(pe - j0);
     int64_t j = j0;
-    for (; j + 1 < pe; j += 2)
-      k_qr_col2<<<dim3((unsigned)(pe - j), B), 256, 0, s>>>(m, j, f.W, f.Vf, f.tau, f.rdiag,
-                                                             f.rsup, f.bs);
+    for (; j + 1 < pe; j += 2) {
+      const dim3 g((unsigned)(pe - j), B);
+      const int64_t rows = m - j;
+      if (rows <= 256 * 4)
+        k_qr_col2r<4><<<g, 256, 0, s>>>(m, j, f.W, f.Vf, f.tau, f.rdiag, f.rsup, f.bs);
+      else if (rows <= 256 * 8)
+        k_qr_col2r<8><<<g, 256, 0, s>>>(m, j, f.W, f.Vf, f.tau, f.rdiag, f.rsup, f.bs);
+      else if (rows <= 256 * 16)
+        k_qr_col2r<16><<<g, 256, 0, s>>>(m, j, f.W, f.Vf, f.tau, f.rdiag, f.rsup, f.bs);
+      else
+        k_qr_col2<<<g, 256, 0, s>>>(m, j, f.W, f.Vf, f.tau, f.rdiag, f.rsup, f.bs);
+    }
     if (j < pe)
       k_qr_col<<<dim3((unsigned)(pe - j), B), 256, 0, s>>>(m, j, f.W, f.Vf, f.tau, f.rdiag, f.bs);
     if (pe - j0 >= 2) k_qr_fixsup<<<dim3(1, B), QR_NB / 2, 0, s>>>(m, j0, pe, f.rsup, f.W, f.bs);
