@@ -442,26 +442,36 @@ __global__ __launch_bounds__(256) void k_colnorms(int64_t pr, const double2* __r
   if (threadIdx.x == 0) nrm[blockIdx.x] = sqrt(s);
 }
 
-// descending order of the q column norms (stable: ties keep column order),
-// one workgroup; rank of column j = #{i : s_i > s_j} + #{i < j : s_i == s_j}.
-// A non-finite norm is ordered as -1 (norms are >= 0) so that the ranks stay a
-// permutation; the flag reports it. One workgroup per problem of a batch
-// (nrm and perm with stride sn, s_out with stride ss).
+// descending order of the q column norms (stable: ties keep column order);
+// rank of column j = #{i : s_i > s_j} + #{i < j : s_i == s_j}, one thread per
+// column j (blockIdx.x selects a chunk of 256 columns, blockIdx.y the problem;
+// nrm and perm with stride sn, s_out with stride ss). A non-finite norm is
+// ordered as -1 (norms are >= 0) so that the ranks stay a permutation; the
+// flag reports it.
 __global__ __launch_bounds__(256) void k_rank(int64_t q, const double* __restrict__ nrm,
                                               int64_t* __restrict__ perm, double* __restrict__ s_out,
                                               int* __restrict__ nonfinite, int64_t sn, int64_t ss) {
-  nrm += blockIdx.x * sn;
-  perm += blockIdx.x * sn;
-  s_out += blockIdx.x * ss;
-  for (int64_t j = threadIdx.x; j < q; j += 256) {
-    const double sj = nrm[j];
-    if (!isfinite(sj)) atomicOr(nonfinite, 1);
-    const double kj = isfinite(sj) ? sj : -1.0;
-    int64_t rank = 0;
-    for (int64_t i = 0; i < q; ++i) {
-      const double ki = isfinite(nrm[i]) ? nrm[i] : -1.0;
-      rank += (ki > kj) || (ki == kj && i < j);
+  __shared__ double key[256];
+  nrm += blockIdx.y * sn;
+  perm += blockIdx.y * sn;
+  s_out += blockIdx.y * ss;
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const double sj = j < q ? nrm[j] : 0.0;
+  if (j < q && !isfinite(sj)) atomicOr(nonfinite, 1);
+  const double kj = isfinite(sj) ? sj : -1.0;
+  int64_t rank = 0;
+  for (int64_t c0 = 0; c0 < q; c0 += 256) {
+    __syncthreads();
+    const int64_t ic = c0 + threadIdx.x;
+    key[threadIdx.x] = ic < q ? (isfinite(nrm[ic]) ? nrm[ic] : -1.0) : 0.0;
+    __syncthreads();
+    const int64_t n = (q - c0 < 256) ? q - c0 : 256;
+    for (int64_t u = 0; u < n; ++u) {
+      const double ki = key[u];
+      rank += (ki > kj) || (ki == kj && c0 + u < j);
     }
+  }
+  if (j < q) {
     perm[rank] = j;
     s_out[rank] = sj;
   }
@@ -857,7 +867,8 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
   PTN_LAUNCH_CHECK();
   if ((stt = qr_apply_q(f, UB, L.p, L.q, false, st, L.p * L.q))) return stt;
   if ((stt = qr_apply_q(f2, VB, L.q, L.q, false, st, L.q * L.q))) return stt;
-  k_rank<<<NB, 256, 0, st>>>(L.q, nrm, perm, s, counters + 1, L.qp, ss);
+  k_rank<<<dim3((unsigned)((L.q + 255) / 256), NB), 256, 0, st>>>(L.q, nrm, perm, s, counters + 1,
+                                                                   L.qp, ss);
   const int64_t tot = (m * k > k * n) ? m * k : k * n;
   k_svd_out<<<dim3((unsigned)((tot + 255) / 256), NB), 256, 0, st>>>(
       m, n, k, adjoint, L.p, L.q, UB, VB, perm, (double2*)u, ldu, (double2*)vh, ldvh, L.qp, su,
