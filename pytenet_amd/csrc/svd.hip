@@ -38,7 +38,7 @@ constexpr int JB = 16;          // columns per block
 constexpr int JP = 2 * JB;      // columns per block pair
 constexpr int GSEG = 128;       // rows per Gram workgroup (one partial Gram matrix each)
 constexpr int GCH = 64;         // rows per LDS chunk in the Gram kernel
-constexpr int AROWS = 64;       // rows per apply workgroup (16 per wave)
+constexpr int AROWS = 64;       // rows per apply workgroup (a multiple of 64; 16-row tiles per wave)
 
 __device__ double wg_sum3(double v, double* red) {
   const int t = threadIdx.x;
@@ -384,44 +384,49 @@ __global__ __launch_bounds__(256) void k_jac_apply(int64_t p, int64_t qp, int nb
   const int lane = t & 63, wave = t >> 6;
   const int l16 = lane & 15, kk = lane >> 4;
   const int q = kk & 1, pb = l16 & 1;
-  const int64_t row0 = rowblk + 16 * wave;
-  const int64_t arow = row0 + l16;
-  const bool aok = arow < ld;
-  T a[16];
+  constexpr int RT = AROWS / 64;   // 16-row tiles per wave
+#pragma unroll 1
+  for (int rt = 0; rt < RT; ++rt) {
+    const int64_t row0 = rowblk + 16 * (RT * wave + rt);
+    if (row0 >= ld) break;
+    const int64_t arow = row0 + l16;
+    const bool aok = arow < ld;
+    T a[16];
 #pragma unroll
-  for (int s = 0; s < 16; ++s) {
-    const int kc = 2 * s + (kk >> 1);
-    const C2 y = aok ? M[pair_col(bp, bq, kc) * ld + arow] : C2{0, 0};
-    a[s] = q ? y.y : y.x;
-  }
-  Acc acc[4];
+    for (int s = 0; s < 16; ++s) {
+      const int kc = 2 * s + (kk >> 1);
+      const C2 y = aok ? M[pair_col(bp, bq, kc) * ld + arow] : C2{0, 0};
+      a[s] = q ? y.y : y.x;
+    }
+    Acc acc[4];
 #pragma unroll
-  for (int tile = 0; tile < 4; ++tile) acc[tile] = Acc{0, 0, 0, 0};
+    for (int tile = 0; tile < 4; ++tile) acc[tile] = Acc{0, 0, 0, 0};
 #pragma unroll
-  for (int s = 0; s < 16; ++s) {
-    const int kc = 2 * s + (kk >> 1);
+    for (int s = 0; s < 16; ++s) {
+      const int kc = 2 * s + (kk >> 1);
+#pragma unroll
+      for (int tile = 0; tile < 4; ++tile) {
+        const C2 jv = J[kc][8 * tile + (l16 >> 1)];
+        const T b = (q == pb) ? jv.x : (q ? -jv.y : jv.y);
+        acc[tile] = jmfma(a[s], b, acc[tile]);
+      }
+    }
 #pragma unroll
     for (int tile = 0; tile < 4; ++tile) {
-      const C2 jv = J[kc][8 * tile + (l16 >> 1)];
-      const T b = (q == pb) ? jv.x : (q ? -jv.y : jv.y);
-      acc[tile] = jmfma(a[s], b, acc[tile]);
-    }
-  }
+      T mine[4], other[4];
 #pragma unroll
-  for (int tile = 0; tile < 4; ++tile) {
-    T mine[4], other[4];
+      for (int v = 0; v < 4; ++v) {
+        mine[v] = acc[tile][v];
+        other[v] = __shfl_xor(mine[v], 1);
+      }
+      C2* col = M + pair_col(bp, bq, 8 * tile + (l16 >> 1)) * ld;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      mine[v] = acc[tile][v];
-      other[v] = __shfl_xor(mine[v], 1);
-    }
-    C2* col = M + pair_col(bp, bq, 8 * tile + (l16 >> 1)) * ld;
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int64_t row = row0 + acc_row<T>(kk, 2 * pb + u);
-      const T re = pb ? other[2 + u] : mine[u];
-      const T im = pb ? mine[2 + u] : other[u];
-      if (row < ld) col[row] = C2{re, im};
+      for (int u = 0; u < 2; ++u) {
+        const int64_t row = row0 + acc_row<T>(kk, 2 * pb + u);
+        const T re = pb ? other[2 + u] : mine[u];
+        const T im = pb ? mine[2 + u] : other[u];
+        if (row < ld) col[row] = C2{re, im};
+      }
     }
   }
 }
