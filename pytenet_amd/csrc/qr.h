@@ -15,6 +15,10 @@ constexpr int QR_SPLITK = 8;    // K chunks of the V^H C products
 //   rsup  k superdiagonal entries R[j][j+1] produced by two-reflector launches
 //   T     per panel the QR_NB x QR_NB upper-triangular factor of the compact
 //         WY form H_j0 ... H_j0+nb-1 = I - V T V^H (row-major, zero padded)
+//   scl   per matrix the exact power of two the input was multiplied by
+//         before the factorisation (1 unless its largest entry lies outside
+//         [2^-256, 2^256], where sums of squares would underflow or overflow);
+//         W, rdiag and rsup hold R * scl, the reflectors are scale-free
 // plus scratch for the blocked updates (Vf^H C partials, the reduced product).
 // Per-matrix strides (complex elements) of a batch of factorisations of
 // identical shape; matrix b of a batch uses the arrays offset by b * stride.
@@ -25,6 +29,8 @@ struct QrStrides {
 struct QrFact {
   int64_t m, n, k, npan, ncmax, batch;
   double2 *W, *Vf, *tau, *rdiag, *rsup, *T, *G, *P, *Z;
+  double* scl;
+  unsigned long long* mxb;   // largest |Re|, |Im| per matrix (bits of a non-negative double)
   QrStrides bs;
 };
 

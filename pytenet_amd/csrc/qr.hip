@@ -92,15 +92,53 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
 }
 __device__ __forceinline__ double2 conj2(double2 a) { return make_double2(a.x, -a.y); }
 
-// column-major copy: W[j*m + i] = A[i*lda + j]
+__global__ void k_zero_u64(int64_t n, unsigned long long* __restrict__ p) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0ull;
+}
+
+// mxb[b] = largest |Re|, |Im| of matrix b (m x n, lda, at a + b sa): the bit
+// patterns of non-negative doubles order like the doubles, so an integer
+// atomic max gives the float max (NaN entries are skipped by fmax)
+__global__ __launch_bounds__(256) void k_maxabs(int64_t m, int64_t n, const double2* __restrict__ a,
+                                                int64_t lda, int64_t sa,
+                                                unsigned long long* __restrict__ mxb) {
+  a += blockIdx.y * sa;
+  double v = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < m * n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const double2 x = a[(t / n) * lda + t % n];
+    v = fmax(v, fmax(fabs(x.x), fabs(x.y)));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0 && v > 0.0)
+    atomicMax(mxb + blockIdx.y, (unsigned long long)__double_as_longlong(v));
+}
+
+// the exact power of two that brings the largest entry into [0.5, 1) when it
+// lies outside [2^-256, 2^256], else 1 (no change in the usual range)
+__device__ __forceinline__ double qr_scale_of(unsigned long long bits) {
+  const double mx = __longlong_as_double((long long)bits);
+  if (!(mx > 0.0) || !isfinite(mx) || (mx >= 0x1p-256 && mx <= 0x1p256)) return 1.0;
+  int e;
+  frexp(mx, &e);
+  return ldexp(1.0, -e);
+}
+
+// column-major copy times the matrix's scale: W[j*m + i] = scl A[i*lda + j]
 __global__ void k_to_colmajor(int64_t m, int64_t n, const double2* __restrict__ a, int64_t lda,
-                              double2* __restrict__ w, int64_t sa, int64_t sw) {
+                              double2* __restrict__ w, int64_t sa, int64_t sw,
+                              const unsigned long long* __restrict__ mxb, double* __restrict__ scl) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double sc = qr_scale_of(mxb[blockIdx.y]);
+  if (t == 0) scl[blockIdx.y] = sc;
   if (t >= m * n) return;
   a += blockIdx.y * sa;
   w += blockIdx.y * sw;
   int64_t i = t / n, j = t % n;
-  w[j * m + i] = a[i * lda + j];
+  const double2 x = a[i * lda + j];
+  w[j * m + i] = make_double2(x.x * sc, x.y * sc);
 }
 
 // reflector j, applied to column c = j + blockIdx.x (< pe) of the panel.
@@ -566,7 +604,7 @@ size_t qr_fact_bytes(int64_t m, int64_t n, int64_t ncmax, int64_t batch) {
   const QrStrides b = qr_strides(m, n, ncmax);
   const size_t B = (size_t)batch;
   return al(B * b.W * 16) + al(B * b.V * 16) + 3 * al(B * b.k * 16) + al(B * b.T * 16) +
-         al(B * b.G * 16) + al(B * b.P * 16) + al(B * b.Z * 16);
+         al(B * b.G * 16) + al(B * b.P * 16) + al(B * b.Z * 16) + 2 * al(B * 8);
 }
 
 QrFact qr_fact_layout(int64_t m, int64_t n, int64_t ncmax, void* ws, int64_t batch) {
@@ -586,7 +624,9 @@ QrFact qr_fact_layout(int64_t m, int64_t n, int64_t ncmax, void* ws, int64_t bat
   f.T = (double2*)p; p += al(B * f.bs.T * 16);
   f.G = (double2*)p; p += al(B * f.bs.G * 16);
   f.P = (double2*)p; p += al(B * f.bs.P * 16);
-  f.Z = (double2*)p;
+  f.Z = (double2*)p; p += al(B * f.bs.Z * 16);
+  f.scl = (double*)p; p += al(B * 8);
+  f.mxb = (unsigned long long*)p;
   return f;
 }
 
@@ -639,8 +679,14 @@ int qr_factor(QrFact& f, const double2* a, int64_t lda, hipStream_t s, int64_t s
   if (m == 0 || n == 0) return PTN_OK;
   if (f.batch < 1 || f.batch > 65535) return PTN_ERR_ARG;
   const unsigned B = (unsigned)f.batch;
+  k_zero_u64<<<(unsigned)((f.batch + 255) / 256), 256, 0, s>>>(f.batch, f.mxb);
+  {
+    int64_t nb = (m * n + 255) / 256;
+    if (nb > 1024) nb = 1024;
+    k_maxabs<<<dim3((unsigned)nb, B), 256, 0, s>>>(m, n, a, lda, sa, f.mxb);
+  }
   k_to_colmajor<<<dim3((unsigned)((m * n + 255) / 256), B), 256, 0, s>>>(m, n, a, lda, f.W, sa,
-                                                                           f.bs.W);
+                                                                           f.bs.W, f.mxb, f.scl);
   PTN_LAUNCH_CHECK();
   for (int64_t p = 0; p < f.npan; ++p) {
     const int64_t j0 = p * QR_NB;
@@ -702,9 +748,11 @@ __global__ void k_q_init(int64_t m, int64_t k, double2* __restrict__ Qc) {
 __global__ void k_qr_out(int64_t m, int64_t n, int64_t k, const double2* __restrict__ W,
                          const double2* __restrict__ rdiag, const double2* __restrict__ Qc,
                          double2* __restrict__ q, int64_t ldq, double2* __restrict__ r,
-                         int64_t ldr, QrStrides bs, int64_t sq, int64_t sr) {
+                         int64_t ldr, QrStrides bs, int64_t sq, int64_t sr,
+                         const double* __restrict__ scl) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t b = blockIdx.y;
+  const double inv = 1.0 / scl[b];   // exact: scl is a power of two
   W += b * bs.W;
   rdiag += b * bs.k;
   Qc += b * m * k;
@@ -716,7 +764,8 @@ __global__ void k_qr_out(int64_t m, int64_t n, int64_t k, const double2* __restr
   }
   if (t < k * n) {
     int64_t i = t / n, c = t % n;
-    r[i * ldr + c] = (i < c) ? W[c * m + i] : (i == c ? rdiag[i] : make_double2(0.0, 0.0));
+    const double2 v = (i < c) ? W[c * m + i] : (i == c ? rdiag[i] : make_double2(0.0, 0.0));
+    r[i * ldr + c] = make_double2(v.x * inv, v.y * inv);
   }
 }
 
@@ -744,7 +793,8 @@ int qr_run(int64_t m, int64_t n, const void* a, int64_t lda, void* q, int64_t ld
   if ((st = qr_apply_q(f, Qc, m, k, true, s, m * k))) return st;
   const int64_t tot = (m * k > k * n) ? m * k : k * n;
   k_qr_out<<<dim3((unsigned)((tot + 255) / 256), B), 256, 0, s>>>(
-      m, n, k, f.W, f.rdiag, Qc, (double2*)q, ldq, (double2*)r, ldr, f.bs, m * ldq, k * ldr);
+      m, n, k, f.W, f.rdiag, Qc, (double2*)q, ldq, (double2*)r, ldr, f.bs, m * ldq, k * ldr,
+      f.scl);
   PTN_LAUNCH_CHECK();
   return PTN_OK;
 }

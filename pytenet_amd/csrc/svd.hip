@@ -452,10 +452,11 @@ __global__ __launch_bounds__(256) void k_colnorms(int64_t pr, const double2* __r
 // column j (blockIdx.x selects a chunk of 256 columns, blockIdx.y the problem;
 // nrm and perm with stride sn, s_out with stride ss). A non-finite norm is
 // ordered as -1 (norms are >= 0) so that the ranks stay a permutation; the
-// flag reports it.
+// flag reports it. The norms are those of the input times scl[problem].
 __global__ __launch_bounds__(256) void k_rank(int64_t q, const double* __restrict__ nrm,
                                               int64_t* __restrict__ perm, double* __restrict__ s_out,
-                                              int* __restrict__ nonfinite, int64_t sn, int64_t ss) {
+                                              int* __restrict__ nonfinite, int64_t sn, int64_t ss,
+                                              const double* __restrict__ scl) {
   __shared__ double key[256];
   nrm += blockIdx.y * sn;
   perm += blockIdx.y * sn;
@@ -478,7 +479,7 @@ __global__ __launch_bounds__(256) void k_rank(int64_t q, const double* __restric
   }
   if (j < q) {
     perm[rank] = j;
-    s_out[rank] = sj;
+    s_out[rank] = sj / scl[blockIdx.y];   // undo the input scaling (a power of two)
   }
 }
 
@@ -873,7 +874,7 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
   if ((stt = qr_apply_q(f, UB, L.p, L.q, false, st, L.p * L.q))) return stt;
   if ((stt = qr_apply_q(f2, VB, L.q, L.q, false, st, L.q * L.q))) return stt;
   k_rank<<<dim3((unsigned)((L.q + 255) / 256), NB), 256, 0, st>>>(L.q, nrm, perm, s, counters + 1,
-                                                                   L.qp, ss);
+                                                                   L.qp, ss, f.scl);
   const int64_t tot = (m * k > k * n) ? m * k : k * n;
   k_svd_out<<<dim3((unsigned)((tot + 255) / 256), NB), 256, 0, st>>>(
       m, n, k, adjoint, L.p, L.q, UB, VB, perm, (double2*)u, ldu, (double2*)vh, ldvh, L.qp, su,

@@ -11,6 +11,7 @@ import pytest
 import torch
 
 import oracle as orc
+import pytenet_amd as ptn
 from conftest import load_fixture
 from pytenet_amd import backend, _lib
 from pytenet_amd.hamiltonian import heisenberg_xxz_1d_mpo
@@ -198,6 +199,46 @@ def test_qr_matches_lapack(m, n):
     qr_, rr_ = np.linalg.qr(a)
     assert relerr(q, qr_) < 1e-12
     assert relerr(r, rr_) < 1e-12
+
+
+@pytest.mark.parametrize("scale", [1e-250, 1e250])
+def test_qr_and_svd_of_extreme_scales_match_lapack(scale):
+    """Entries whose squares under- or overflow: the factorisations scale the
+    input by a power of two first, as LAPACK's scaled norms would."""
+    rng = np.random.default_rng(3)
+    a = orc.crandn((40, 24), rng) * scale
+    q, r = backend.qr(dev(a))
+    q0, r0 = np.linalg.qr(a)
+    assert relerr(q, q0) < 1e-12 and relerr(r, r0) < 1e-12
+    qb, rb = backend.qr_batched(np.stack([a, a / scale]))
+    assert relerr(qb[0], q0) < 1e-12 and relerr(rb[0], r0) < 1e-12
+    assert relerr(qb[1], q0) < 1e-12 and relerr(rb[1], r0 / scale) < 1e-12
+    for precond32 in (False, True):
+        b = orc.crandn((160, 150), rng) * scale
+        u, s, vh = backend.svd(dev(b), precond32=precond32)
+        s = s.cpu().numpy()
+        s0 = np.linalg.svd(b, compute_uv=False)
+        np.testing.assert_allclose(s, s0, rtol=1e-11)
+        uh, vhh = backend.to_host(u), backend.to_host(vh)
+        np.testing.assert_allclose(uh.conj().T @ uh, np.identity(150), atol=1e-12)
+        assert np.abs((uh * s) @ vhh - b).max() < 1e-12 * s0[0]
+
+
+def test_orthonormalize_of_a_tiny_state_matches_oracle():
+    """A chain whose norm (about 1e-288) squares to below the double range."""
+    rng = np.random.default_rng(8)
+    L, D = 24, 6
+    qs = np.array([0, 0])
+    qb = [np.zeros(min(D, 2**i, 2**(L - i)), dtype=int) for i in range(L + 1)]
+    ref = orc.MPS(qs, qb, fill="random", rng=rng)
+    ref.a = [x * 1e-12 for x in ref.a]
+    psi = ptn.MPS(qs, qb, fill=0.0)
+    psi.a = [x.copy() for x in ref.a]
+    nrm = psi.orthonormalize("right")
+    nrm0 = ref.orthonormalize(mode="right")
+    assert 0 < nrm0 < 1e-250 and abs(nrm - nrm0) < 1e-12 * nrm0
+    for x, y in zip(psi.a, ref.a):
+        assert np.abs(backend.to_host(x) - y).max() < 1e-12
 
 
 def test_qr_rank_deficient_matches_lapack():
