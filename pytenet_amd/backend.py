@@ -460,14 +460,19 @@ def svd_batched(a, return_info=False, precond32=False):
     return u, s, vh
 
 
-def _svd_classes(shapes):
+def _svd_classes(shapes, merge_below=512):
     """Group sector shapes for batched SVDs: a wide block is factorised through
-    its adjoint, and blocks whose smaller dimension falls in the same
-    half-octave share one zero-padded batch."""
+    its adjoint. Blocks whose smaller dimension is at most `merge_below` share
+    one zero-padded batch (their Jacobi rounds and QR panels are bound by
+    launch latency, so one batch at the largest shape costs about as much per
+    round as the smallest would, and the number of dependent rounds is that of
+    the largest block alone); larger blocks are grouped by half-octave of the
+    smaller dimension."""
     groups = {}
     for idx, (ms, ns) in enumerate(shapes):
         small = min(ms, ns)
-        groups.setdefault(int(np.ceil(2 * np.log2(small))) if small > 1 else 0, []).append(idx)
+        key = -1 if small <= merge_below else int(np.ceil(2 * np.log2(small)))
+        groups.setdefault(key, []).append(idx)
     return list(groups.values())
 
 
