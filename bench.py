@@ -218,6 +218,161 @@ def profile_summary(dirs):
                     print(f"{len(v):6d} {c:>12} {sum(v) / len(v):14.1f}  {short(k)}")
 
 
+def _sectored_bonds(L, Dmax, rng):
+    """Sorted U(1) bond quantum numbers: bond i draws min(Dmax, 2^min(i, L-i))
+    labels from the binomial Sz distribution of min(i, L-i) spins."""
+    from math import comb
+    qbonds = []
+    for i in range(L + 1):
+        m = min(i, L - i)
+        qs = np.arange(-m, m + 1, 2)
+        w = np.array([comb(m, (q + m) // 2) for q in qs], float)
+        qbonds.append(np.sort(rng.choice(qs, size=min(Dmax, 2**m), p=w / w.sum())))
+    return qbonds
+
+
+def workload_dmrg1(args):
+    """Single-site DMRG, XXZ L=64 (J=1, Delta=1, h=0, zero quantum numbers),
+    seeded random MPS with bonds min(D, 2^i, 2^(L-i)), numiter 10, FP64."""
+    import torch
+    import pytenet_amd as ptn
+    L, D, sweeps = args.L or 64, args.D or 512, args.sweeps or 10
+    H = ptn.heisenberg_xxz_1d_mpo(L, 1.0, 1.0, 0.0).zero_qnumbers()
+    bonds = [min(D, 2**min(i, 40), 2**min(L - i, 40)) for i in range(L + 1)]
+    psi = ptn.MPS(H.qsite, [np.zeros(b, dtype=int) for b in bonds], fill="random",
+                  rng=np.random.default_rng(args.seed))
+    psi.to_device()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    en = ptn.dmrg_singlesite(H, psi, sweeps, numiter_lanczos=args.numiter or 10)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"workload": f"single-site DMRG, XXZ L={L} Delta=1, D={D} (tapered), "
+                        f"{sweeps} sweeps, FP64", "value": round(dt / sweeps, 4),
+            "unit": "s/sweep", "total_s": round(dt, 2),
+            "site_updates_per_s": round(2 * (L - 1) * sweeps / dt, 2),
+            "energies": [float(e) for e in np.asarray(en).real]}
+
+
+def workload_dmrg2_u1(args):
+    """Two-site DMRG with U(1) sectors, XXZ L=128 (J=1, Delta=1, h=0), sorted
+    seeded bond labels (about 30 sectors at the centre, bond cap D=2048),
+    tol_split 0 (bonds stay at the cap), numiter 25, FP64; one progress line
+    per 16 bond splits."""
+    import torch
+    import pytenet_amd as ptn
+    from pytenet_amd import dmrg as dm
+    L, D, sweeps = args.L or 128, args.D or 2048, args.sweeps or 1
+    rng = np.random.default_rng(4)
+    H = ptn.heisenberg_xxz_1d_mpo(L, 1.0, 1.0, 0.0)
+    qbonds = _sectored_bonds(L, D, rng)
+    u, cnt = np.unique(qbonds[L // 2], return_counts=True)
+    psi = ptn.MPS(H.qsite, qbonds, fill="random", rng=rng)
+    psi.to_device()
+    split = dm.mps_split_tensor_svd
+    t0 = [0.0]
+    nsplit = [0]
+
+    def split_progress(*a, **k):
+        r = split(*a, **k)
+        nsplit[0] += 1
+        if nsplit[0] % 16 == 0:
+            print(f"[bench] split {nsplit[0]} done, {time.perf_counter() - t0[0]:.1f} s",
+                  file=sys.stderr, flush=True)
+        return r
+
+    dm.mps_split_tensor_svd = split_progress
+    try:
+        torch.cuda.synchronize()
+        t0[0] = time.perf_counter()
+        en = ptn.dmrg_twosite(H, psi, sweeps, numiter_lanczos=args.numiter or 25, tol_split=0,
+                              max_bond=D)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0[0]
+    finally:
+        dm.mps_split_tensor_svd = split
+    return {"workload": f"two-site DMRG with U(1) sectors, XXZ L={L} Delta=1, D_max={D}, "
+                        f"{sweeps} sweep(s) incl. setup, FP64, tile-skipping products",
+            "value": round(dt / sweeps, 2), "unit": "s/sweep", "total_s": round(dt, 2),
+            "sectors_centre": int(len(u)), "largest_sector": int(cnt.max()),
+            "energies": [float(e) for e in np.asarray(en).real],
+            "bond_max": int(max(psi.bond_dims)),
+            "gpu_mem_peak_gib": round(torch.cuda.max_memory_allocated() / 2**30, 1)}
+
+
+def workload_chains(args):
+    """256 transverse-field Ising chains (L=32, J=1, h=0, g uniform in
+    [0.5, 1.5], seeded), 64 each at D = 32, 64, 128, 256 (seeded random MPS),
+    imaginary-time single-site TDVP dt=0.05, 20 steps, numiter 10, evolved
+    together by tdvp_singlesite_batched in FP32 and in FP64; the oracle is
+    timed on one chain per D for one step."""
+    import copy as cp
+    import torch
+    import pytenet_amd as ptn
+    import oracle as orc
+    nch, L, steps, nit = 256, args.L or 32, args.steps_chains, args.numiter or 10
+    Ds = [32, 64, 128, 256]
+    rng = np.random.default_rng(args.seed)
+    Hs, psis = [], []
+    for k in range(nch):
+        g = rng.uniform(0.5, 1.5)
+        H = ptn.ising_1d_mpo(L, 1.0, 0.0, g)
+        D = Ds[k % len(Ds)]
+        bonds = [min(D, 2**min(i, 40), 2**min(L - i, 40)) for i in range(L + 1)]
+        psi = ptn.MPS(H.qsite, [np.zeros(b, dtype=int) for b in bonds], fill="random", rng=rng)
+        psi.to_device()
+        Hs.append(H)
+        psis.append(psi)
+    out = {"workload": f"{nch} Ising chains L={L}, D in {Ds}, single-site TDVP dt=0.05, "
+                       f"{steps} steps, numiter {nit}, batched", "unit": "site updates/s"}
+    upd = nch * steps * 2 * (L - 1)
+    ens = {}
+    for prec in ("fp32", "fp64"):
+        ps = [cp.deepcopy(p) for p in psis]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ptn.tdvp_singlesite_batched(Hs, ps, 0.05, steps, numiter_lanczos=nit, precision=prec)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        ens[prec] = np.array([ptn.mpo_average(p, H).real / ptn.mps_norm(p)**2
+                              for p, H in zip(ps, Hs)])
+        out[f"{prec}_s"] = round(dt, 2)
+        out[f"{prec}_site_updates_per_s"] = round(upd / dt, 1)
+        print(f"[bench] chains {prec} done", file=sys.stderr, flush=True)
+    err = np.abs(ens["fp32"] - ens["fp64"]) / np.abs(ens["fp64"])
+    out["value"] = out["fp32_site_updates_per_s"]
+    out["energy_rel_err_fp32_vs_fp64_max"] = float(err.max())
+    out["energy_rel_err_fp32_vs_fp64_median"] = float(np.median(err))
+    t_cpu = 0.0
+    for k, D in enumerate(Ds):
+        p = orc.MPS(psis[k].qsite, psis[k].qbonds)
+        p.a = [a.cpu().numpy() for a in psis[k].a]
+        t0 = time.perf_counter()
+        orc.tdvp_singlesite(orc.MPO(Hs[k].qsite, Hs[k].qbonds, Hs[k].a), p, 0.05, 1,
+                            numiter_lanczos=nit)
+        t_cpu += time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": round(len(Ds) * 2 * (L - 1) / t_cpu, 2),
+                           "unit": "site updates/s", "kind": "port",
+                           "sample": "the oracle, one chain per D, one step"}
+    return out
+
+
+def workload_svd(args):
+    """The bond SVD of the two-site split at D: a seeded random 2D x 2D complex
+    matrix, plain FP64 Jacobi and FP32-preconditioned Jacobi (two timed calls
+    each after one untimed)."""
+    D = args.D or 1024
+    t64, sw64, _ = svd_time(D, False)
+    t32, sw64p, sw32p = svd_time(D, True)
+    return {"workload": f"SVD of a {2 * D} x {2 * D} complex matrix (seeded random)",
+            "unit": "s", "value": round(t32, 4), "fp64_s": round(t64, 4), "fp64_sweeps": sw64,
+            "precond32_s": round(t32, 4), "precond32_sweeps_fp32_fp64": [sw32p, sw64p]}
+
+
+WORKLOADS = {"dmrg1": workload_dmrg1, "dmrg2-u1": workload_dmrg2_u1, "chains": workload_chains,
+             "svd": workload_svd}
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--profile-summary":
         profile_summary(sys.argv[2:])
@@ -226,9 +381,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--L", type=int, default=100)
-    ap.add_argument("--D", type=int, default=1024)
-    ap.add_argument("--numiter", type=int, default=8)
+    ap.add_argument("--L", type=int, default=None)
+    ap.add_argument("--D", type=int, default=None)
+    ap.add_argument("--numiter", type=int, default=None)
+    ap.add_argument("--workload", default="tdvp2", choices=["tdvp2"] + sorted(WORKLOADS),
+                    help="tdvp2 (default): the headline line; the others time the further "
+                         "workloads (single-site DMRG, two-site DMRG with U(1) sectors, "
+                         "many independent chains, the bond SVD) and print one JSON line each")
+    ap.add_argument("--sweeps", type=int, default=None)
+    ap.add_argument("--steps-chains", type=int, default=20)
     ap.add_argument("--mixed", default="fp32")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu", action="store_true")
@@ -246,6 +407,12 @@ def main():
     import pytenet_amd as ptn
 
     torch.cuda.set_device(0)
+    if args.workload != "tdvp2":
+        print(json.dumps(WORKLOADS[args.workload](args)), flush=True)
+        return
+    args.L = args.L or 100
+    args.D = args.D or 1024
+    args.numiter = args.numiter or 8
     if args.roofline_only:
         # a plain device copy of known size: calibrates the HBM byte counters of a profile run
         src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
