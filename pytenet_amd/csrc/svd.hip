@@ -19,8 +19,9 @@
 //   k_jac_solve: one cyclic sweep of two-sided Jacobi on G in LDS (parallel
 //                ordering, JB disjoint rotations per step), accumulating the unitary J;
 //   k_jac_apply: Y <- Y J and V <- V J (FP64 MFMA).
-// Sweeps repeat until no pair needs a rotation, i.e. every column pair
-// satisfies |x_i^H x_j| <= tol |x_i| |x_j|. Then sigma_j = |x_j| and
+// Sweeps repeat until a sweep in which no column pair needed a rotation by
+// more than 1e-8 relative (|x_i^H x_j| <= 1e-8 |x_i| |x_j| when the pair was
+// processed); quadratic convergence leaves couplings of order 1e-16 after it. Then sigma_j = |x_j| and
 // U_X = X / sigma. One-sided Jacobi computes small singular values to high
 // relative accuracy, which the discarded-weight truncation relies on.
 //
@@ -717,8 +718,8 @@ static inline unsigned gs_blocks(int64_t total) {
 
 // Thin SVD of `batch` m x n matrices of identical shape (problem b at a + b sa,
 // u + b su, s + b ss, vh + b svh). The Jacobi sweeps run on all problems at
-// once and stop together: a sweep in which no problem needed a rotation above
-// `big` is followed by one final sweep. info (if given) receives the number of
+// once and stop together, after the first sweep in which no problem needed a
+// rotation above `big` (relative). info (if given) receives the number of
 // sweeps, whether the sweep limit was hit, and whether any singular value was
 // not finite.
 // With `precond32` (and at least PRE32_MIN_Q columns) the Jacobi phase starts
@@ -843,12 +844,13 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
     PTN_LAUNCH_CHECK();
   }
   const double tol = DBL_EPSILON * sqrt((double)L.q);
-  // a sweep without rotations above this relative size is followed by one
-  // final sweep (quadratic convergence takes it to tol) instead of repeating
-  // until rounding noise stops triggering tiny rotations
-  const double big = 1e-7;
+  // the iteration stops after a sweep in which no pair needed a rotation above
+  // this relative size: convergence is quadratic, so the couplings left after
+  // such a sweep are of order big^2 (n big^2 at most), and a further sweep
+  // would only rotate by rounding-level angles
+  const double big = 1e-8;
   const int max_sweeps = 60;
-  int sweeps = 0, converged = (L.q < 2), final_sweep = 0;
+  int sweeps = 0, converged = (L.q < 2);
   while (!converged && sweeps < max_sweeps) {
     k_set_ints<<<1, 64, 0, st>>>(counters, 1, 0);
     for (int r = 0; r < L.nblk - 1; ++r) {
@@ -863,8 +865,7 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
     PTN_HIP_CHECK(hipMemcpyAsync(host_rot, counters, sizeof(int), hipMemcpyDeviceToHost, st));
     PTN_HIP_CHECK(hipStreamSynchronize(st));
     ++sweeps;
-    if (final_sweep) converged = 1;
-    else if (*host_rot == 0) final_sweep = 1;
+    if (*host_rot == 0) converged = 1;
   }
   // 3. B = Q1 R2^H Q2^H = (Q1 [U_X; 0]) diag(sigma) (Q2 V)^H with U_X = X / sigma
   k_colnorms<<<dim3((unsigned)L.q, NB), 256, 0, st>>>(L.q, X, nrm, sx, L.qp);
