@@ -142,6 +142,41 @@ def svd_time(D, precond32, reps=2):
     return (time.perf_counter() - t0) / reps, int(inf[0]), int(inf[3])
 
 
+def jacobi_apply_roofline(D):
+    """Live HBM roofline of the SVD's dominant kernel, k_jac_apply<double>: one
+    FP64 SVD of a seeded random 2D x 2D matrix under torch.profiler (device
+    times of the library's kernels on the stream they run on); the first sweep
+    (2D/16 - 1 launches) rotates every block pair, so each of its launches
+    reads and writes all of X (2D x 2D) and V (2D x 2D): 4 x (2D)^2 x 16 bytes."""
+    import torch
+    from torch.profiler import profile, ProfilerActivity
+    from pytenet_amd import backend
+    dev = backend.device()
+    g = torch.Generator(device=dev).manual_seed(1)
+    a = torch.randn(2 * D, 2 * D, dtype=torch.complex128, device=dev, generator=g)
+    backend.svd(a)
+    torch.cuda.synchronize()
+    with profile(activities=[ProfilerActivity.CUDA]) as prof:
+        backend.svd(a)
+        torch.cuda.synchronize()
+    ev = [e for e in prof.events() if "k_jac_apply<double>" in e.name]
+    ev.sort(key=lambda e: e.time_range.start)
+    rounds = 2 * D // 16 - 1
+    first = ev[:rounds]
+    if len(first) < rounds:
+        return None
+    dur = [getattr(e, "device_time_total", 0.0) or e.cuda_time_total for e in first]
+    avg_us = float(np.mean(dur))
+    nbytes = 4.0 * (2 * D)**2 * 16
+    ach = nbytes / (avg_us * 1e-6) / 1e9
+    return {"kernel": "k_jac_apply<double>: Jacobi rotation of the X and V column pairs, "
+                      "first sweep of a %d x %d SVD" % (2 * D, 2 * D),
+            "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": nbytes, "avg_us": round(avg_us, 2), "launches": len(first),
+            "source": "torch.profiler device times of the first-sweep launches"}
+
+
 def cpu_baseline(D, L, numiter, budget_s=30.0):
     """Oracle (NumPy/LAPACK restatement of the reference) on a bounded sample:
     one two-site local update at a saturated bond (merge, Krylov exponential
@@ -366,7 +401,8 @@ def workload_svd(args):
     t32, sw64p, sw32p = svd_time(D, True)
     return {"workload": f"SVD of a {2 * D} x {2 * D} complex matrix (seeded random)",
             "unit": "s", "value": round(t32, 4), "fp64_s": round(t64, 4), "fp64_sweeps": sw64,
-            "precond32_s": round(t32, 4), "precond32_sweeps_fp32_fp64": [sw32p, sw64p]}
+            "precond32_s": round(t32, 4), "precond32_sweeps_fp32_fp64": [sw32p, sw64p],
+            "apply_roofline": jacobi_apply_roofline(D)}
 
 
 WORKLOADS = {"dmrg1": workload_dmrg1, "dmrg2-u1": workload_dmrg2_u1, "chains": workload_chains,
@@ -446,6 +482,7 @@ def main():
 
     heff_ms_mx, heff_tf_mx = heff_roofline(args.D, args.mixed)
     heff_ms_64, heff_tf_64 = heff_roofline(args.D, "fp64")
+    apply_rl = jacobi_apply_roofline(args.D)
     svd_s, svd_sw, _ = svd_time(args.D, False)
     svdp_s, svdp_sw64, svdp_sw32 = svd_time(args.D, True)
 
@@ -487,6 +524,7 @@ def main():
                          "note": "model = LAPACK complex SVD count 22 m n^2 x 4 real flop per "
                                  "complex flop; Jacobi does more work and is bound by its "
                                  "127 dependent rounds per sweep"},
+        "svd_apply_roofline": apply_rl,
         "roofline": {"kernel": "two-site effective Hamiltonian (A.R, W mix, L^T.T2)",
                      "bound": "mfma", "achieved": round(heff_tf_mx, 2),
                      "peak": PEAK_TFLOPS[args.mixed], "unit": "TFLOP/s",
