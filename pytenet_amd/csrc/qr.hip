@@ -540,6 +540,7 @@ __global__ __launch_bounds__(64) void k_qr_tmat(int nbp, const double2* __restri
 // Y = sum of the nparts partial products P[s] (QR_NB x nc, rows < nbp used);
 // Z = -T^H Y (adjoint = 1) or Z = -T Y (adjoint = 0). A workgroup handles 8
 // columns; thread (r, cl) sums row r of column cl, then forms row r of Z.
+static_assert(QR_NB * 8 == 256, "k_qr_reduce maps its 256 threads onto QR_NB rows x 8 columns");
 __global__ __launch_bounds__(256) void k_qr_reduce(int nbp, int64_t nc, int nparts,
                                                    const double2* __restrict__ P,
                                                    const double2* __restrict__ T, int adjoint,
