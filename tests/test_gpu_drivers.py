@@ -160,6 +160,31 @@ def test_fp32_contractions_keep_fidelity():
     assert abs(e32 - e64) < 1e-6 * abs(e64)
 
 
+def test_tdvp_twosite_precond32_svd_matches_fp64_svd():
+    """Splits of 128 columns (D=64) use the FP32-preconditioned Jacobi SVD when
+    asked: with FP64 contractions the state equals the plain-SVD run to
+    rounding, and the default mixed mode keeps the fidelity bound."""
+    rng = np.random.default_rng(43)
+    L = 12
+    H = ptn.heisenberg_xxz_1d_mpo(L, 1.0, 0.8, -0.1).zero_qnumbers()
+    bonds = [min(64, 2**i, 2**(L - i)) for i in range(L + 1)]
+    psi = ptn.MPS(H.qsite, [np.zeros(b, dtype=int) for b in bonds], fill="random", rng=rng)
+    psi.orthonormalize("left")
+    runs = {}
+    for key, prec, pre in (("plain", "fp64", False), ("pre", "fp64", True), ("mixed", "fp32", None)):
+        p = copy.deepcopy(psi)
+        ptn.tdvp_twosite(H, p, -0.05j, 2, numiter_lanczos=8, tol_split=0, precision=prec,
+                         max_bond=64, svd_precond32=pre)
+        runs[key] = p
+    def infid(a, b):
+        return abs(1 - abs(ptn.mps_vdot(a, b))**2 / (ptn.mps_norm(a)**2 * ptn.mps_norm(b)**2))
+    assert max(runs["pre"].bond_dims) == 64
+    assert infid(runs["plain"], runs["pre"]) < 1e-12
+    e0, e1 = ptn.mpo_average(runs["plain"], H).real, ptn.mpo_average(runs["pre"], H).real
+    assert abs(e1 - e0) < 1e-11 * abs(e0)
+    assert infid(runs["plain"], runs["mixed"]) < 1e-8
+
+
 def test_tdvp_checkpoint_resume(tmp_path):
     """Two real-time steps in one run equal one step, a checkpoint, a resumed run
     and one more step (the resumed run re-normalises a state of norm one)."""
