@@ -99,7 +99,8 @@ __global__ void k_zero_u64(int64_t n, unsigned long long* __restrict__ p) {
 
 // mxb[b] = largest |Re|, |Im| of matrix b (m x n, lda, at a + b sa): the bit
 // patterns of non-negative doubles order like the doubles, so an integer
-// atomic max gives the float max (NaN entries are skipped by fmax)
+// atomic max (one per workgroup) gives the float max (NaN entries are skipped
+// by fmax)
 __global__ __launch_bounds__(256) void k_maxabs(int64_t m, int64_t n, const double2* __restrict__ a,
                                                 int64_t lda, int64_t sa,
                                                 unsigned long long* __restrict__ mxb) {
@@ -112,8 +113,13 @@ __global__ __launch_bounds__(256) void k_maxabs(int64_t m, int64_t n, const doub
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  if ((threadIdx.x & 63) == 0 && v > 0.0)
-    atomicMax(mxb + blockIdx.y, (unsigned long long)__double_as_longlong(v));
+  __shared__ double wmax[4];
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+    if (v > 0.0) atomicMax(mxb + blockIdx.y, (unsigned long long)__double_as_longlong(v));
+  }
 }
 
 // the exact power of two that brings the largest entry into [0.5, 1) when it
@@ -683,7 +689,8 @@ int qr_factor(QrFact& f, const double2* a, int64_t lda, hipStream_t s, int64_t s
   k_zero_u64<<<(unsigned)((f.batch + 255) / 256), 256, 0, s>>>(f.batch, f.mxb);
   {
     int64_t nb = (m * n + 255) / 256;
-    if (nb > 1024) nb = 1024;
+    const int64_t cap = f.batch > 16 ? 16 : 256;   // workgroups per matrix (one atomic each)
+    if (nb > cap) nb = cap;
     k_maxabs<<<dim3((unsigned)nb, B), 256, 0, s>>>(m, n, a, lda, sa, f.mxb);
   }
   k_to_colmajor<<<dim3((unsigned)((m * n + 255) / 256), B), 256, 0, s>>>(m, n, a, lda, f.W, sa,
