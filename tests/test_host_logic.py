@@ -1,0 +1,42 @@
+"""
+Host-side logic of the GPU path that runs without a GPU: the grouping of
+quantum-number sectors into batched SVDs, and the sector bookkeeping that
+block_sparse_svd/qr share with the reference (block_sparse_util.py:67-103).
+"""
+
+import numpy as np
+
+from pytenet_amd import backend
+
+
+def test_svd_classes_partition_the_sectors():
+    rng = np.random.default_rng(5)
+    shapes = [(int(m), int(n)) for m, n in rng.integers(1, 1500, size=(60, 2))]
+    groups = backend._svd_classes(shapes)
+    flat = sorted(i for g in groups for i in g)
+    assert flat == list(range(len(shapes)))
+    small = [i for i, (m, n) in enumerate(shapes) if min(m, n) <= 512]
+    # every latency-bound sector (smaller dimension <= 512) shares one batch
+    assert any(sorted(g) == small for g in groups)
+    for g in groups:
+        if sorted(g) == small:
+            continue
+        # larger sectors: one half-octave of the smaller dimension per batch
+        ks = {int(np.ceil(2 * np.log2(min(shapes[i])))) for i in g}
+        assert len(ks) == 1 and all(min(shapes[i]) > 512 for i in g)
+
+
+def test_svd_classes_threshold_is_inclusive():
+    groups = backend._svd_classes([(600, 512), (513, 700), (40, 3)])
+    assert sorted(map(sorted, groups)) == [[0, 2], [1]]
+
+
+def test_sectors_follow_the_stable_sort_of_the_quantum_numbers():
+    q0 = np.array([1, -1, 0, 1, 0, 2])
+    q1 = np.array([0, 1, 1, -1, 0])
+    secs = backend._sectors(q0, q1)
+    # common quantum numbers in ascending order; rows and columns of a sector
+    # in their original order
+    assert [int(qn) for qn, _, _ in secs] == [-1, 0, 1]
+    assert [list(r) for _, r, _ in secs] == [[1], [2, 4], [0, 3]]
+    assert [list(c) for _, _, c in secs] == [[3], [0, 4], [1, 2]]
