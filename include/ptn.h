@@ -100,7 +100,8 @@ int ptn_env_right(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, 
  * pair products pytenet/mps.py:528-535 (merge), pytenet/tdvp.py:84,112 and
  * pytenet/mps.py:472,490 (absorbing R or C into a neighbour).
  * C[m*c_m + n*c_n] (+)= sum_k opA(m,k) opB(k,n), opA(m,k) = A[m*a_m + k*a_k] (conj if conjA),
- * opB(k,n) = B[k*b_k + n*b_n] (conj if conjB). Strides in complex elements. */
+ * opB(k,n) = B[k*b_k + n*b_n] (conj if conjB). Strides in complex elements.
+ * The workspace is 0 bytes (ws may be NULL; it is not used). */
 size_t ptn_zgemm_workspace(int64_t M, int64_t N, int64_t K);
 int ptn_zgemm(int64_t M, int64_t N, int64_t K,
               const void* A, int64_t a_m, int64_t a_k, int conjA,

@@ -230,8 +230,9 @@ int ptn_zgemm(int64_t M, int64_t N, int64_t K, const void* A, int64_t a_m, int64
               const void* B, int64_t b_k, int64_t b_n, int conjB, void* C, int64_t c_m,
               int64_t c_n, int accumulate, void* ws, size_t ws_bytes, void* stream) {
   ZGemm g = {M, N, K, A, a_m, a_k, conjA, B, b_k, b_n, conjB, C, c_m, c_n, accumulate, 1, 0, 0, 0};
-  return zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, ws, ws_bytes,
-               (hipStream_t)stream);
+  (void)ws;   // the dense product needs no workspace (ptn_zgemm_workspace is 0)
+  (void)ws_bytes;
+  return zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, (hipStream_t)stream);
 }
 
 size_t ptn_krylov_workspace(const ptn_local_op* op, int numiter) {
