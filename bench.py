@@ -425,6 +425,8 @@ def main():
                          "workloads (single-site DMRG, two-site DMRG with U(1) sectors, "
                          "many independent chains, the bond SVD) and print one JSON line each")
     ap.add_argument("--sweeps", type=int, default=None)
+    ap.add_argument("--autotune", action="store_true",
+                    help="switch on the tile autotuner of the complex products")
     ap.add_argument("--steps-chains", type=int, default=20)
     ap.add_argument("--mixed", default="fp32")
     ap.add_argument("--seed", type=int, default=2026)
@@ -443,6 +445,9 @@ def main():
     import pytenet_amd as ptn
 
     torch.cuda.set_device(0)
+    if args.autotune:
+        from pytenet_amd import backend
+        backend.set_autotune(True)
     if args.workload != "tdvp2":
         print(json.dumps(WORKLOADS[args.workload](args)), flush=True)
         return

@@ -103,6 +103,13 @@ int ptn_env_right(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml, 
  * opB(k,n) = B[k*b_k + n*b_n] (conj if conjB). Strides in complex elements.
  * The workspace is 0 bytes (ws may be NULL; it is not used). */
 size_t ptn_zgemm_workspace(int64_t M, int64_t N, int64_t K);
+/* Tile autotuner of all complex products (no counterpart in the reference,
+ * whose products are BLAS calls): with on != 0, the first product of each
+ * shape times the tile configurations on the caller's stream (synchronising
+ * it once) and keeps the fastest for that shape; products that accumulate
+ * into C or skip zero tiles keep the fixed choice. C must not alias A or B
+ * while it is on. Process-wide; returns the previous setting. */
+int ptn_set_autotune(int on);
 int ptn_zgemm(int64_t M, int64_t N, int64_t K,
               const void* A, int64_t a_m, int64_t a_k, int conjA,
               const void* B, int64_t b_k, int64_t b_n, int conjB,

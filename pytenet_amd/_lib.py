@@ -85,6 +85,7 @@ SIGNATURES = [
     ("ptn_qr_batched", c_int, [c_i64, c_i64, c_i64, c_p, c_p, c_p, c_p, c_sz, c_p]),
     ("ptn_svd_workspace_batched", c_sz, [c_i64, c_i64, c_i64]),
     ("ptn_svd_batched", c_int, [c_i64, c_i64, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
+    ("ptn_set_autotune", c_int, [c_int]),
     ("ptn_svd_workspace_precond32", c_sz, [c_i64, c_i64, c_i64]),
     ("ptn_svd_precond32", c_int, [c_i64, c_i64, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
 ]

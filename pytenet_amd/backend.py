@@ -285,6 +285,13 @@ def qr(a):
     return q, r
 
 
+def set_autotune(on=True):
+    """Switch the tile autotuner of the complex products on or off (process-wide;
+    the first product of each shape then times the tile shapes once). Returns
+    the previous setting."""
+    return bool(_lib.load().ptn_set_autotune(int(bool(on))))
+
+
 def svd(a, return_info=False, precond32=False):
     """Thin SVD of a 2-D complex128 device tensor: (u, s, vh), s descending (device float64).
     Synchronises the stream (the Jacobi sweeps are checked for convergence on the host).

@@ -19,6 +19,8 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
             int64_t batch, int64_t sa, int64_t su, int64_t ss, int64_t svh, bool precond32);
 
 static thread_local hipError_t g_last_hip = hipSuccess;
+static bool g_autotune = false;
+bool autotune_enabled() { return g_autotune; }
 void ptn_set_last_hip_error(hipError_t e) { g_last_hip = e; }
 
 // a (n0, n1, n2) *= s along axis 0 (s[i]) or axis 2 (s[k])
@@ -219,6 +221,12 @@ int ptn_env_right_batched(int precision, int64_t batch, int64_t Dl, int64_t d, i
                           const void* r, void* out, void* ws, size_t ws_bytes, void* stream) {
   return env_step_b(0, precision, batch, Dl, d, Dr, Ml, Mr, a, b, w, r, out, ws, ws_bytes,
                     (hipStream_t)stream);
+}
+
+int ptn_set_autotune(int on) {
+  const int prev = g_autotune ? 1 : 0;
+  g_autotune = on != 0;
+  return prev;
 }
 
 size_t ptn_zgemm_workspace(int64_t M, int64_t N, int64_t K) {

@@ -12,6 +12,8 @@
 // column lane%16 and, in accumulator element v, row lane/16 + 4v (FP64) or
 // row 4*(lane/16) + v (FP32).
 #pragma once
+#include <mutex>
+#include <unordered_map>
 #include "common.h"
 
 namespace ptn {
@@ -367,15 +369,13 @@ inline int launch_cfg(const ZGemm& z, hipStream_t s, void* mws = nullptr, size_t
   return PTN_OK;
 }
 
+// opt-in autotuning of the tile shape (ptn_set_autotune): set once per process
+bool autotune_enabled();
+
 // tile choice: the large tile (128 x 64 complex) when it still gives at least
 // two workgroups per CU, else the small one (64 x 32)
 template <typename Tc, typename TA, typename TB, typename TC>
-inline int launch_typed(const ZGemm& z, hipStream_t s, void* mws = nullptr, size_t mws_bytes = 0) {
-  // tile-skipping products (mask workspace given): the small tile skips the
-  // zero blocks of sector-structured operands at finer granularity, which
-  // outweighs its lower efficiency per flop (U(1) two-site DMRG at L=128,
-  // D=2048: 76 s against 82 s per sweep)
-  if (mws && z.batch == 1) return launch_cfg<Tc, TA, TB, TC, 2, 2>(z, s, mws, mws_bytes);
+inline int launch_heuristic(const ZGemm& z, hipStream_t s, void* mws, size_t mws_bytes) {
   // FP32: 128 x 128 complex tiles (half the operand bytes per flop of 128 x 64)
   // when they still give two workgroups per CU
   if constexpr (sizeof(Tc) == 4) {
@@ -385,6 +385,83 @@ inline int launch_typed(const ZGemm& z, hipStream_t s, void* mws = nullptr, size
   int64_t big = ((z.M + 127) / 128) * ((z.N + 63) / 64) * z.batch;
   if (big >= 512) return launch_cfg<Tc, TA, TB, TC, 4, 4>(z, s, mws, mws_bytes);
   return launch_cfg<Tc, TA, TB, TC, 2, 2>(z, s, mws, mws_bytes);
+}
+
+template <typename Tc, typename TA, typename TB, typename TC>
+inline int launch_choice(int c, const ZGemm& z, hipStream_t s) {
+  if (c == 0) return launch_cfg<Tc, TA, TB, TC, 2, 2>(z, s);
+  if (c == 1) return launch_cfg<Tc, TA, TB, TC, 4, 4>(z, s);
+  if constexpr (sizeof(Tc) == 4) return launch_cfg<Tc, TA, TB, TC, 4, 8>(z, s);
+  return PTN_ERR_ARG;
+}
+
+// Tile autotuner: the first product of a given shape (M, N, K, batch and the
+// contiguity of the operands) times every tile configuration (two timed
+// launches each after one untimed, HIP events on the caller's stream) and
+// keeps the fastest for that shape. Only products that overwrite C are tuned
+// (a repeated launch then leaves the same result); the tuning launch of the
+// chosen configuration is the product itself. It synchronises the stream
+// once per new shape, hence opt-in.
+template <typename Tc, typename TA, typename TB, typename TC>
+inline int launch_tuned(const ZGemm& z, hipStream_t s) {
+  struct Key {
+    int64_t M, N, K, batch, flags;
+    bool operator==(const Key& o) const {
+      return M == o.M && N == o.N && K == o.K && batch == o.batch && flags == o.flags;
+    }
+  };
+  struct Hash {
+    size_t operator()(const Key& k) const {
+      return std::hash<int64_t>()(k.M * 1000003 ^ k.N * 10007 ^ k.K * 101 ^ k.batch * 7 ^ k.flags);
+    }
+  };
+  static std::mutex mu;
+  static std::unordered_map<Key, int, Hash> best;
+  const Key key{z.M, z.N, z.K, z.batch,
+                (int64_t)(z.a_k == 1) | (int64_t)(z.a_m == 1) << 1 | (int64_t)(z.b_n == 1) << 2 |
+                    (int64_t)(z.b_k == 1) << 3};
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = best.find(key);
+    if (it != best.end()) return launch_choice<Tc, TA, TB, TC>(it->second, z, s);
+  }
+  const int nc = sizeof(Tc) == 4 ? 3 : 2;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return PTN_ERR_HIP;
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return PTN_ERR_HIP; }
+  int bestc = 0, st = PTN_OK;
+  float bestms = 0.0f;
+  for (int c = 0; c < nc && st == PTN_OK; ++c) {
+    if ((st = launch_choice<Tc, TA, TB, TC>(c, z, s))) break;
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < 2 && st == PTN_OK; ++r) st = launch_choice<Tc, TA, TB, TC>(c, z, s);
+    (void)hipEventRecord(e1, s);
+    if (hipEventSynchronize(e1) != hipSuccess) { st = PTN_ERR_HIP; break; }
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (c == 0 || ms < bestms) { bestms = ms; bestc = c; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    best[key] = bestc;
+  }
+  // C holds the last configuration's (identical) result; write it once more
+  // with the chosen one so that the output never depends on tuning order
+  return launch_choice<Tc, TA, TB, TC>(bestc, z, s);
+}
+
+template <typename Tc, typename TA, typename TB, typename TC>
+inline int launch_typed(const ZGemm& z, hipStream_t s, void* mws = nullptr, size_t mws_bytes = 0) {
+  // tile-skipping products (mask workspace given): the small tile skips the
+  // zero blocks of sector-structured operands at finer granularity, which
+  // outweighs its lower efficiency per flop (U(1) two-site DMRG at L=128,
+  // D=2048: 76 s against 82 s per sweep)
+  if (mws && z.batch == 1) return launch_cfg<Tc, TA, TB, TC, 2, 2>(z, s, mws, mws_bytes);
+  if (!z.accumulate && autotune_enabled()) return launch_tuned<Tc, TA, TB, TC>(z, s);
+  return launch_heuristic<Tc, TA, TB, TC>(z, s, mws, mws_bytes);
 }
 
 }  // namespace ptn

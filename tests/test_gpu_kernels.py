@@ -50,6 +50,25 @@ def test_zgemm_strides_and_ragged_shapes(M, N, K, ta, tb):
                   refA.conj() @ refB.conj()) < FP64_TOL
 
 
+@pytest.mark.parametrize("M,N,K", [(300, 200, 64), (1024, 768, 256)])
+def test_autotuned_products_match(M, N, K):
+    """With the tile autotuner on, products (first call: tuning; second call:
+    the cached choice) equal those of the fixed tile choice."""
+    rng = np.random.default_rng(M + N)
+    A, B = dev(orc.crandn((M, K), rng)), dev(orc.crandn((K, N), rng))
+    a, w, l, r = _heff_problem(3, Dl=40, d=4, Dr=36, M=5)
+    ref = backend.to_host(backend.matmul(A, B))
+    ref32 = backend.to_host(backend.heff_apply(a, w, l, r, "fp32"))
+    prev = backend.set_autotune(True)
+    try:
+        for _ in range(2):
+            assert relerr(backend.matmul(A, B), ref) < 1e-14
+            assert relerr(backend.heff_apply(a, w, l, r, "fp32"), ref32) < 1e-6
+    finally:
+        backend.set_autotune(prev)
+    assert backend.set_autotune(prev) == prev
+
+
 def test_zgemm_accumulate():
     rng = np.random.default_rng(2)
     A, B, C = orc.crandn((40, 30), rng), orc.crandn((30, 50), rng), orc.crandn((40, 50), rng)
