@@ -260,6 +260,25 @@ def test_orthonormalize_of_a_tiny_state_matches_oracle():
         assert np.abs(backend.to_host(x) - y).max() < 1e-12
 
 
+@pytest.mark.parametrize("precond32", [False, True])
+def test_svd_nonfinite_input_raises(precond32):
+    """A NaN or an infinity in the input is reported (FloatingPointError), in
+    the plain and the FP32-preconditioned Jacobi SVD and in a batch."""
+    rng = np.random.default_rng(4)
+    for bad in (np.nan, np.inf):
+        a = orc.crandn((150, 140), rng)
+        a[7, 9] = bad
+        with pytest.raises(FloatingPointError):
+            backend.svd(dev(a), precond32=precond32)
+        b = np.stack([orc.crandn((150, 140), rng), a])
+        with pytest.raises(FloatingPointError):
+            backend.svd_batched(b, precond32=precond32)
+    # the library stays usable afterwards
+    c = orc.crandn((150, 140), rng)
+    _, s, _ = backend.svd(dev(c), precond32=precond32)
+    np.testing.assert_allclose(s.cpu().numpy(), np.linalg.svd(c, compute_uv=False), rtol=1e-11)
+
+
 def test_qr_rank_deficient_matches_lapack():
     rng = np.random.default_rng(77)
     a = orc.crandn((12, 8), rng)
