@@ -90,6 +90,24 @@ def test_dmrg_twosite_reference_trajectory_quantum_numbers():
     np.testing.assert_allclose(_phase_aligned(psi.to_vector(), ref), ref, atol=1e-8)
 
 
+@pytest.mark.parametrize("precision,rtol", [("fp32", 1e-6), ("bf16x3", 1e-4)])
+def test_dmrg_twosite_quantum_numbers_low_precision(precision, rtol):
+    """U(1) sectors with FP32 / split-bf16 tile-skipping products: the energies
+    of the reference trajectory to the precision of the contractions (split
+    bf16 keeps about 16 bits per product; measured 1.7e-5 here), and the same
+    sector labels."""
+    z = load_fixture("traj_dmrg2_xxz_L10_qn")
+    psi = _mps(*mps_arrays(z, "init_"))
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        en = ptn.dmrg_twosite(_mpo(z), psi, int(z["numsweeps"]), numiter_lanczos=int(z["numiter"]),
+                              precision=precision)
+    np.testing.assert_allclose(en, z["en_min"], rtol=rtol)
+    _, qb, _ = mps_arrays(z, "final_")
+    for i in range(len(qb)):
+        assert np.array_equal(np.sort(psi.qbonds[i]), np.sort(qb[i]))
+
+
 def test_tdvp_quantum_numbers_reference_trajectory():
     z = load_fixture("traj_tdvp_xxz_L8_qn")
     H = _mpo(z)
