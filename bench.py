@@ -29,6 +29,7 @@ import time
 
 import numpy as np
 
+T_START = time.perf_counter()
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -64,26 +65,65 @@ def prewarm():
                          max_bond=8)
 
 
-def run_mode(H, psi0, prec, steps, warmup, D, numiter, svd_precond32=None):
+def run_leg(H, psi0, prec, warmup, steps, D, numiter, svd_precond32=None, decide=None):
+    """One precision leg of the headline: setup, `warmup` untimed sweeps, then
+    timed sweeps with the live launch timing of the complex products switched
+    on. With `decide`, the number of timed sweeps is decide(t) for t = the
+    duration of the first sweep run (warmup or timed), capped by `steps`.
+    Returns (psi, seconds per timed sweep, timed sweeps, per-kernel timings)."""
     import torch
     import pytenet_amd as ptn
+    from pytenet_amd import backend
     psi = copy.deepcopy(psi0)
     run = ptn.TwoSiteTDVP(H, psi, -0.05j, numiter, 1e-10, precision=prec, max_bond=D,
                           svd_precond32=svd_precond32)
     run.setup()
-    for _ in range(warmup):
-        run.sweep()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(steps):
+    first = None
+    for k in range(warmup):
+        t0 = time.perf_counter()
         run.sweep()
-        print(f"[bench] {prec}: sweep {k + 1}/{steps} done, bonds max {max(psi.bond_dims)}",
+        torch.cuda.synchronize()
+        first = first or time.perf_counter() - t0
+        print(f"[bench] {prec}: warmup sweep {k + 1}/{warmup} {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+    if decide is not None and first is not None:
+        steps = max(1, min(steps, decide(first)))
+    backend.timing_begin()
+    t0 = time.perf_counter()
+    k = 0
+    while k < steps:
+        t1 = time.perf_counter()
+        run.sweep()
+        k += 1
+        if decide is not None and first is None:
+            torch.cuda.synchronize()
+            first = time.perf_counter() - t1
+            steps = max(1, min(steps, decide(first)))
+        print(f"[bench] {prec}: sweep {k}/{steps} done, bonds max {max(psi.bond_dims)}",
               file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
+    timings = backend.timing_end()
     run.finish()
     del run
-    return psi, dt
+    return psi, dt, steps, timings
+
+
+def kernel_roofline(timings, prec):
+    """Roofline entry of the dominant complex-product kernel instance of a timed
+    leg: algorithmic flops of its launches / their summed device time."""
+    if not timings:
+        return None
+    top = timings[0]
+    ach = top["flop"] / (top["total_ms"] * 1e-3) / 1e12
+    return {"kernel": top["name"], "bound": "mfma", "achieved": round(ach, 2),
+            "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s", "frac": round(ach / PEAK_TFLOPS[prec], 4),
+            "launches": top["launches"], "avg_us": round(top["avg_ms"] * 1e3, 2),
+            "flop_per_launch": top["flop"] / top["launches"],
+            "bytes_per_launch": top["bytes"] / top["launches"],
+            "share_of_product_time": round(top["total_ms"] / sum(t["total_ms"] for t in timings), 4),
+            "source": "HIP events around every launch on its stream, over the timed sweeps"}
 
 
 def heff_roofline(D, prec, reps=10):
@@ -110,18 +150,18 @@ def heff_roofline(D, prec, reps=10):
     return ms, heff_flops(D, 4, D, 5, 5) / (ms * 1e-3) / 1e12
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "heff_2site_D1024_hbm_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "zgemm_tdvp2_D1024_hbm_traffic.txt")
 
 
-def profiled_traffic(prec, D):
-    """HBM bytes per two-site H_eff application at D=1024 from the committed
-    FETCH_SIZE / WRITE_SIZE counter runs (calibrated against a plain copy);
-    None for other sizes or when the summary is absent."""
-    if D != 1024 or not os.path.exists(TRAFFIC_FILE):
+def profiled_traffic(kernel):
+    """HBM bytes per launch of a product kernel instance from the committed
+    FETCH_SIZE / WRITE_SIZE counter summary (calibrated against a plain
+    copy); None when the summary has no line for it."""
+    if not os.path.exists(TRAFFIC_FILE):
         return None
     for line in open(TRAFFIC_FILE):
-        if line.startswith(f"traffic_bytes_{prec}:"):
-            return float(line.split(":")[1])
+        if line.startswith("traffic_bytes_per_launch ") and kernel in line:
+            return float(line.rsplit(":", 1)[1])
     return None
 
 
@@ -439,7 +479,14 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--strict-svd", action="store_true",
-                    help="also time the mixed sweep with the plain FP64 Jacobi SVD")
+                    help="with --extras: also time the mixed sweep with the plain FP64 Jacobi SVD")
+    ap.add_argument("--extras", action="store_true",
+                    help="after the JSON line, time the bond SVD alone and print it to stderr")
+    ap.add_argument("--time-budget", type=float, default=420.0,
+                    help="wall-clock seconds (from start) the headline run aims to stay within; "
+                         "the number of timed sweeps is cut to fit")
+    ap.add_argument("--tail-s", type=float, default=80.0,
+                    help="seconds reserved after the FP64 leg's sweeps for the fixed tail")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only time the two-site effective-Hamiltonian kernels (for profiling)")
     args = ap.parse_args()
@@ -476,14 +523,21 @@ def main():
         return
     prewarm()
     H, psi0 = build_state(args.L, args.D, args.seed)
-    print(f"[bench] state ready: L={args.L}, bonds max {max(psi0.bond_dims)}", file=sys.stderr,
-          flush=True)
-    psi64, t64 = run_mode(H, psi0, "fp64", args.steps, args.warmup, args.D, args.numiter)
-    psimx, tmx = run_mode(H, psi0, args.mixed, args.steps, args.warmup, args.D, args.numiter)
-    tstrict = None
-    if args.strict_svd:
-        _, tstrict = run_mode(H, psi0, args.mixed, args.steps, args.warmup, args.D, args.numiter,
-                              svd_precond32=False)
+    print(f"[bench] state ready: L={args.L}, bonds max {max(psi0.bond_dims)} "
+          f"({time.perf_counter() - T_START:.0f} s)", file=sys.stderr, flush=True)
+    warm = min(args.warmup, 1)
+
+    def decide(t_sweep):
+        # timed sweeps per leg that fit the wall-clock budget: both legs run
+        # warm + k sweeps (the mixed ones are no slower than the FP64 ones), plus
+        # the fixed tail (mixed setup, CPU baseline, roofline bookkeeping)
+        left = args.time_budget - (time.perf_counter() - T_START) - args.tail_s
+        left -= warm * t_sweep + (0 if warm else t_sweep)
+        return int(left // (2 * t_sweep)) + (0 if warm else 1)
+
+    psi64, t64, steps, tim64 = run_leg(H, psi0, "fp64", warm, args.steps, args.D, args.numiter,
+                                       decide=decide)
+    psimx, tmx, _, timmx = run_leg(H, psi0, args.mixed, warm, steps, args.D, args.numiter)
     ov = ptn.mps_vdot(psi64, psimx)
     n64, nmx = ptn.mps_norm(psi64), ptn.mps_norm(psimx)
     infid = abs(1.0 - abs(ov)**2 / (n64**2 * nmx**2))
@@ -492,23 +546,19 @@ def main():
     rel_e = abs(emx - e64) / abs(e64)
     del psi64, psimx
 
-    heff_ms_mx, heff_tf_mx = heff_roofline(args.D, args.mixed)
-    heff_ms_64, heff_tf_64 = heff_roofline(args.D, "fp64")
-    try:
-        apply_rl = jacobi_apply_roofline(args.D)
-    except Exception as exc:   # the profiler is optional: the line is printed regardless
-        apply_rl = {"error": f"{type(exc).__name__}: {exc}"}
-    svd_s, svd_sw, _ = svd_time(args.D, False)
-    svdp_s, svdp_sw64, svdp_sw32 = svd_time(args.D, True)
-
     nupd = 2 * (args.L - 2) + 1
+    roof = kernel_roofline(timmx, args.mixed)
+    if roof is not None:
+        roof["traffic"] = profiled_traffic(roof["kernel"])
+        roof["traffic_source"] = ("rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch of this kernel, "
+                                  "copy-calibrated (profiles/" + os.path.basename(TRAFFIC_FILE) + ")")
     out = {
         "metric": METRIC,
         "value": round(tmx, 4),
         "unit": "s/sweep",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warm,
         "ms_per_step": round(tmx * 1e3, 1),
         "higher_is_better": False,
         "scaling": "weak",
@@ -525,38 +575,26 @@ def main():
         "infidelity_vs_fp64": infid,
         "energy_rel_err_vs_fp64": rel_e,
         "site_updates_per_s": round(nupd / tmx, 3),
-        "svd_2D_x_2D_s": round(svd_s, 4),
-        "svd_2D_x_2D_sweeps": svd_sw,
-        "svd_precond32_2D_x_2D_s": round(svdp_s, 4),
-        "svd_precond32_sweeps_fp32_fp64": [svdp_sw32, svdp_sw64],
-        "svd_roofline": {"kernel": "bond SVD of a 2D x 2D complex matrix (two QR "
-                                   "preconditioners + blocked one-sided Jacobi, FP64)",
-                         "model_flop": 22.0 * 4 * (2 * args.D)**3,
-                         "achieved": round(22.0 * 4 * (2 * args.D)**3 / svd_s / 1e12, 3),
-                         "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
-                         "frac": round(22.0 * 4 * (2 * args.D)**3 / svd_s / 1e12
-                                       / PEAK_TFLOPS["fp64"], 4),
-                         "note": "model = LAPACK complex SVD count 22 m n^2 x 4 real flop per "
-                                 "complex flop; Jacobi does more work and is bound by its "
-                                 "127 dependent rounds per sweep"},
-        "svd_apply_roofline": apply_rl,
-        "roofline": {"kernel": "two-site effective Hamiltonian (A.R, W mix, L^T.T2)",
-                     "bound": "mfma", "achieved": round(heff_tf_mx, 2),
-                     "peak": PEAK_TFLOPS[args.mixed], "unit": "TFLOP/s",
-                     "frac": round(heff_tf_mx / PEAK_TFLOPS[args.mixed], 4),
-                     "traffic": profiled_traffic(args.mixed, args.D),
-                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated "
-                                       "(profiles/heff_2site_D1024_hbm_traffic.txt)",
-                     "avg_ms": round(heff_ms_mx, 3),
-                     "fp64_achieved": round(heff_tf_64, 2),
-                     "fp64_frac": round(heff_tf_64 / PEAK_TFLOPS["fp64"], 4)},
+        "roofline": roof,
+        "fp64_roofline": kernel_roofline(tim64, "fp64"),
+        "product_kernels_mixed": [{k: (round(v, 3) if isinstance(v, float) else v)
+                                   for k, v in t.items() if k in ("name", "launches", "total_ms")}
+                                  for t in timmx[:6]],
     }
-    if tstrict is not None:
-        out["mixed_sweep_fp64_svd_s"] = round(tstrict, 4)
-        out["speedup_vs_fp64_with_fp64_svd"] = round(t64 / tstrict, 3)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.D, args.L, args.numiter)
+    out["wall_s"] = round(time.perf_counter() - T_START, 1)
     print(json.dumps(out), flush=True)
+    if args.extras:
+        # optional legs after the line: SVD timings, live Jacobi-apply roofline,
+        # the mixed sweep with the plain FP64 SVD
+        extra = {"svd": workload_svd(args)}
+        if args.strict_svd:
+            _, tstrict, _, _ = run_leg(H, psi0, args.mixed, warm, steps, args.D, args.numiter,
+                                       svd_precond32=False)
+            extra["mixed_sweep_fp64_svd_s"] = round(tstrict, 4)
+            extra["speedup_vs_fp64_with_fp64_svd"] = round(t64 / tstrict, 3)
+        print("[bench] extras " + json.dumps(extra), file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":

@@ -62,8 +62,11 @@ extern "C" {
 /* Krylov status words (int32[4] in device memory):
  *   [0] number of Lanczos vectors used (numiter, or j+1 after a breakdown at step j)
  *   [1] j+1 if beta[j] < 100*n*eps stopped the iteration at step j, else 0
- *   [2] 1 if a norm or inner product was not finite, else 0
- *   [3] reserved (0) */
+ *   [2] bit 0: a norm or inner product was not finite;
+ *       bit 1: the tridiagonal QL iteration did not converge (the result is unusable)
+ *   [3] 1 if the start vector has norm zero (the reference asserts, krylov.py:30), else 0
+ * After a stop (breakdown, non-finite value, zero start) the remaining
+ * iterations launch no further work on the stopped problem. */
 
 int ptn_version(void);
 const char* ptn_status_string(int status);
@@ -219,6 +222,23 @@ int ptn_svd_batched(int64_t batch, int64_t m, int64_t n, const void* a, void* u,
 size_t ptn_svd_workspace_precond32(int64_t batch, int64_t m, int64_t n);
 int ptn_svd_precond32(int64_t batch, int64_t m, int64_t n, const void* a, void* u, double* s,
                       void* vh, int32_t* info, void* ws, size_t ws_bytes, void* stream);
+
+/* ---- Live launch timing of the complex products (no counterpart in the
+ * reference, which has no device code): between ptn_timing_begin() and
+ * ptn_timing_end(), every complex-product launch is bracketed by two HIP
+ * events on the stream it runs on. ptn_timing_end() waits for them and
+ * writes, per kernel instance (largest total time first, at most max_out
+ * rows), the launch count, the summed device time and the summed algorithmic
+ * flops (8 per complex multiply-add) and bytes (each operand once). */
+typedef struct {
+  char name[160];    /* e.g. "zgemm_kernel<float, c64, c64, c128, 4, 8, true, true>" */
+  int64_t launches;
+  double total_ms;
+  double flop;
+  double bytes;
+} ptn_kernel_timing;
+int ptn_timing_begin(void);
+int ptn_timing_end(ptn_kernel_timing* out, int max_out, int* n_out);
 
 #ifdef __cplusplus
 }

@@ -38,6 +38,12 @@ class LocalOp(ctypes.Structure):
                 ("l", c_p), ("w", c_p), ("r", c_p)]
 
 
+class KernelTiming(ctypes.Structure):
+    """ptn_kernel_timing (include/ptn.h)."""
+    _fields_ = [("name", ctypes.c_char * 160), ("launches", c_i64), ("total_ms", c_dbl),
+                ("flop", c_dbl), ("bytes", c_dbl)]
+
+
 # (name, restype, argtypes) for every exported symbol of include/ptn.h
 SIGNATURES = [
     ("ptn_version", c_int, []),
@@ -88,6 +94,8 @@ SIGNATURES = [
     ("ptn_set_autotune", c_int, [c_int]),
     ("ptn_svd_workspace_precond32", c_sz, [c_i64, c_i64, c_i64]),
     ("ptn_svd_precond32", c_int, [c_i64, c_i64, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
+    ("ptn_timing_begin", c_int, []),
+    ("ptn_timing_end", c_int, [ctypes.POINTER(KernelTiming), c_int, ctypes.POINTER(c_int)]),
 ]
 
 _lib = None

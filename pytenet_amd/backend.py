@@ -254,13 +254,19 @@ def krylov_local(kind, v, l, w, r, numiter, mode, dt=0.0, precision="fp64", info
 
 def report_krylov_status(info_rows):
     """Emit the reference's breakdown warnings (krylov.py:44-50) and raise on
-    non-finite values or a zero start vector (krylov.py:30), from a host copy
-    of the status words (rows of 4 int32)."""
+    non-finite values or a zero start vector (krylov.py:30), or when the
+    tridiagonal eigensolver did not converge (where the reference's
+    np.linalg.eigh, krylov.py:150, raises LinAlgError), from a host copy of
+    the status words (rows of 4 int32: vectors used, breakdown step + 1,
+    bit 0 non-finite | bit 1 no convergence, zero start)."""
     for row in np.asarray(info_rows).reshape(-1, 4):
         if row[3]:
             raise AssertionError("Lanczos start vector has norm zero")
         if row[2] & 1:
             raise FloatingPointError("non-finite value encountered during Lanczos iteration")
+        if row[2] & 2:
+            raise np.linalg.LinAlgError("Eigenvalues did not converge (Lanczos tridiagonal "
+                                        "matrix)")
         if row[1]:
             warnings.warn(f"beta[{row[1] - 1}] ~= 0 encountered during Lanczos iteration.",
                           RuntimeWarning)
@@ -283,6 +289,28 @@ def qr(a):
     _check(lib.ptn_qr(m, n, ptr(a), a.stride(0), ptr(q), k, ptr(r), n, ptr(ws), ws.numel(),
                       stream()), "ptn_qr")
     return q, r
+
+
+def timing_begin():
+    """Start the live launch timing of the complex products (HIP events on the
+    stream of every launch; no synchronisation)."""
+    _check(_lib.load().ptn_timing_begin(), "ptn_timing_begin")
+
+
+def timing_end(max_rows=64):
+    """Stop the launch timing; per kernel instance (largest total time first):
+    dict(name, launches, total_ms, avg_ms, flop, bytes). Waits for the events."""
+    lib = _lib.load()
+    rows = (_lib.KernelTiming * max_rows)()
+    n = ctypes.c_int(0)
+    _check(lib.ptn_timing_end(rows, max_rows, ctypes.byref(n)), "ptn_timing_end")
+    out = []
+    for r in rows[:n.value]:
+        out.append({"name": r.name.decode(), "launches": int(r.launches),
+                    "total_ms": float(r.total_ms),
+                    "avg_ms": float(r.total_ms) / max(int(r.launches), 1),
+                    "flop": float(r.flop), "bytes": float(r.bytes)})
+    return out
 
 
 def set_autotune(on=True):

@@ -22,6 +22,10 @@ static thread_local hipError_t g_last_hip = hipSuccess;
 static bool g_autotune = false;
 bool autotune_enabled() { return g_autotune; }
 void ptn_set_last_hip_error(hipError_t e) { g_last_hip = e; }
+SkipFlag& skip_flag() {
+  static thread_local SkipFlag f;
+  return f;
+}
 
 // a (n0, n1, n2) *= s along axis 0 (s[i]) or axis 2 (s[k])
 __global__ void k_scale_axis(int64_t n0, int64_t n1, int64_t n2, double2* __restrict__ a,

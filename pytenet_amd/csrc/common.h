@@ -46,6 +46,32 @@ __device__ __forceinline__ void store_c(float2* p, double re, double im) {
 
 void ptn_set_last_hip_error(hipError_t e);
 
+// live launch timing (timing.hip): timing_pre records a start event on s and
+// returns a slot (-1 when timing is off); timing_post records the end event
+// with the launch's kernel name and algorithmic flops / bytes
+int timing_pre(hipStream_t s);
+void timing_post(int slot, hipStream_t s, const char* name, double flop, double bytes);
+
+// Device-side predication of the matvec products: while set (on the calling
+// host thread), the product and mode-mix kernels of batch problem z return at
+// once when p[z * stride] != 0 (e.g. the Lanczos "stopped" word of that
+// problem), so no work is done after a breakdown.
+struct SkipFlag {
+  const int* p = nullptr;
+  int64_t stride = 0;
+};
+SkipFlag& skip_flag();
+struct SkipGuard {
+  SkipGuard(const int* p, int64_t stride) { skip_flag() = SkipFlag{p, stride}; }
+  ~SkipGuard() { skip_flag() = SkipFlag{}; }
+};
+
+template <typename T> struct type_tag;
+template <> struct type_tag<double> { static constexpr const char* s = "double"; };
+template <> struct type_tag<float> { static constexpr const char* s = "float"; };
+template <> struct type_tag<double2> { static constexpr const char* s = "c128"; };
+template <> struct type_tag<float2> { static constexpr const char* s = "c64"; };
+
 // ---------------------------------------------------------------------------
 // Complex GEMM descriptor. Element (m, k) of op(A) is A[m*a_m + k*a_k]
 // (conjugated if conjA), element (k, n) of op(B) is B[k*b_k + n*b_n]; C[m*c_m + n*c_n].

@@ -53,6 +53,7 @@ __global__ __launch_bounds__(256) void zgemm_bf16x3_kernel(GemmKArgs g) {
   // A: row-major, real k contiguous; B: column-major (complex column n), real k contiguous
   __shared__ __align__(16) float As[BM * LDK];
   __shared__ __align__(16) float Bs[BN * LDK];
+  if (g.skip && g.skip[blockIdx.z * g.skip_stride]) return;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -213,6 +214,8 @@ inline int launch_cfg_bf16(const ZGemm& z, hipStream_t s, void* mws, size_t mws_
   g.bz_a = z.bz_a; g.bz_b = z.bz_b; g.bz_c = z.bz_c;
   g.tiles_m = (int)((z.M + T::BM - 1) / T::BM);
   g.tiles_n = (int)((z.N + T::BN - 1) / T::BN);
+  g.skip = skip_flag().p;
+  g.skip_stride = skip_flag().stride;
   int64_t nwg = (int64_t)g.tiles_m * g.tiles_n;
   if (nwg > 0x7fffffff || z.batch > 65535) return PTN_ERR_ARG;
   int st = build_masks<TA, TB>(z, g, T::BM, T::BN, T::BK, mws, mws_bytes, s);
@@ -220,6 +223,7 @@ inline int launch_cfg_bf16(const ZGemm& z, hipStream_t s, void* mws, size_t mws_
   dim3 grid((unsigned)nwg, 1, (unsigned)z.batch);
   const bool ak = (z.a_k == 1) || (z.a_m != 1);
   const bool bn = (z.b_n == 1) || (z.b_k != 1);
+  const int slot = timing_pre(s);
   if (ak && bn)
     zgemm_bf16x3_kernel<TA, TB, TC, TM, TN, true, true><<<grid, 256, 0, s>>>(g);
   else if (ak && !bn)
@@ -229,6 +233,10 @@ inline int launch_cfg_bf16(const ZGemm& z, hipStream_t s, void* mws, size_t mws_
   else
     zgemm_bf16x3_kernel<TA, TB, TC, TM, TN, false, false><<<grid, 256, 0, s>>>(g);
   PTN_LAUNCH_CHECK();
+  if (slot >= 0)
+    timing_post(slot, s,
+                gemm_kernel_name<float, TA, TB, TC, TM, TN>(ak, bn, "zgemm_bf16x3_kernel"),
+                gemm_flop(z), gemm_bytes<TA, TB, TC>(z));
   return PTN_OK;
 }
 

@@ -13,6 +13,9 @@
 // row 4*(lane/16) + v (FP32).
 #pragma once
 #include <mutex>
+#include <stdio.h>
+#include <string.h>
+#include <string>
 #include <unordered_map>
 #include "common.h"
 
@@ -66,6 +69,10 @@ struct GemmKArgs {
   const uint64_t* maskA;
   const uint64_t* maskB;
   int mwords;
+  // predication: the workgroups of batch problem z return at once when
+  // skip[z * skip_stride] != 0
+  const int* skip;
+  int64_t skip_stride;
 };
 
 // masked products keep their stage list in LDS: K / BK at most this many stages
@@ -155,6 +162,7 @@ __global__ __launch_bounds__(256) void zgemm_kernel(GemmKArgs g) {
 
   __shared__ Tc As[2][2 * BK][BM + 1];
   __shared__ Tc Bs[2][BK][2 * BN + 2];
+  if (g.skip && g.skip[blockIdx.z * g.skip_stride]) return;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -314,6 +322,36 @@ __global__ __launch_bounds__(256) void zgemm_kernel(GemmKArgs g) {
   }
 }
 
+// algorithmic work of one product launch: 8 real flops per complex
+// multiply-add; bytes = each operand once, C written (and read if accumulating)
+inline double gemm_flop(const ZGemm& z) {
+  return 8.0 * (double)z.M * (double)z.N * (double)z.K * (double)z.batch;
+}
+template <typename TA, typename TB, typename TC>
+inline double gemm_bytes(const ZGemm& z) {
+  return (double)z.batch * ((double)z.M * z.K * sizeof(TA) + (double)z.K * z.N * sizeof(TB) +
+                            (double)z.M * z.N * sizeof(TC) * (z.accumulate ? 2 : 1));
+}
+// kernel instance name as a profiler lists it (template arguments, without
+// the namespace and the argument list), e.g.
+// zgemm_kernel<float, c64, c64, c128, 4, 8, true, true>
+template <typename Tc, typename TA, typename TB, typename TC, int TM, int TN>
+inline const char* gemm_kernel_name(bool ak, bool bn, const char* kernel = "zgemm_kernel") {
+  static std::mutex mu;
+  static std::unordered_map<std::string, std::string> names;
+  char buf[160];
+  // the split-bf16 kernel has no compute-type argument
+  const bool with_tc = strcmp(kernel, "zgemm_kernel") == 0;
+  snprintf(buf, sizeof(buf), "%s<%s%s%s, %s, %s, %d, %d, %s, %s>", kernel,
+           with_tc ? type_tag<Tc>::s : "", with_tc ? ", " : "", type_tag<TA>::s,
+           type_tag<TB>::s, type_tag<TC>::s, TM, TN, ak ? "true" : "false",
+           bn ? "true" : "false");
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = names.find(buf);
+  if (it == names.end()) it = names.emplace(buf, buf).first;
+  return it->second.c_str();
+}
+
 // masks for the chosen tile shape (BM rows of A, BN columns of B, BK-wide
 // k-stages) in `mws` (gemm_mask_bytes bytes), batch 1 only
 template <typename TA, typename TB>
@@ -349,6 +387,8 @@ inline int launch_cfg(const ZGemm& z, hipStream_t s, void* mws = nullptr, size_t
   g.bz_a = z.bz_a; g.bz_b = z.bz_b; g.bz_c = z.bz_c;
   g.tiles_m = (int)((z.M + T::BM - 1) / T::BM);
   g.tiles_n = (int)((z.N + T::BN - 1) / T::BN);
+  g.skip = skip_flag().p;
+  g.skip_stride = skip_flag().stride;
   int64_t nwg = (int64_t)g.tiles_m * g.tiles_n;
   if (nwg > 0x7fffffff || z.batch > 65535) return PTN_ERR_ARG;
   int st = build_masks<TA, TB>(z, g, T::BM, T::BN, T::BK, mws, mws_bytes, s);
@@ -357,6 +397,7 @@ inline int launch_cfg(const ZGemm& z, hipStream_t s, void* mws = nullptr, size_t
   // a K-contiguous A when its k stride is 1 (or K == 1), else M-contiguous
   const bool ak = (z.a_k == 1) || (z.a_m != 1);
   const bool bn = (z.b_n == 1) || (z.b_k != 1);
+  const int slot = timing_pre(s);
   if (ak && bn)
     zgemm_kernel<Tc, TA, TB, TC, TM, TN, true, true><<<grid, 256, 0, s>>>(g);
   else if (ak && !bn)
@@ -366,6 +407,9 @@ inline int launch_cfg(const ZGemm& z, hipStream_t s, void* mws = nullptr, size_t
   else
     zgemm_kernel<Tc, TA, TB, TC, TM, TN, false, false><<<grid, 256, 0, s>>>(g);
   PTN_LAUNCH_CHECK();
+  if (slot >= 0)
+    timing_post(slot, s, gemm_kernel_name<Tc, TA, TB, TC, TM, TN>(ak, bn), gemm_flop(z),
+                gemm_bytes<TA, TB, TC>(z));
   return PTN_OK;
 }
 

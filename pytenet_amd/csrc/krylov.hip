@@ -476,6 +476,10 @@ int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, do
   if (gblocks < 1) gblocks = 1;
   const unsigned B = (unsigned)batch;
 
+  // after a stop (breakdown, non-finite value, zero start) the products of the
+  // remaining matvecs of that problem do no work
+  static_assert(sizeof(LzState) % sizeof(int) == 0, "LzState stride in ints");
+  SkipGuard skip(&st->stopped, (int64_t)(sizeof(LzState) / sizeof(int)));
   k_state_init<<<B, 256, 0, s>>>(st, numiter);
   k_sumsq_partial<<<dim3(rb, B), 256, 0, s>>>(n, (const double2*)v, n, pa, rb);
   k_start<<<dim3((unsigned)gblocks, B), 256, 0, s>>>(n, (const double2*)v, n, V, n, pa, rb, st);

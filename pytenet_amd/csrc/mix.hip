@@ -16,8 +16,11 @@ __global__ __launch_bounds__(256) void mode_mix_kernel(int64_t nl, int ni, int n
                                                        const double2* __restrict__ wm,
                                                        const TX* __restrict__ x,
                                                        TY* __restrict__ y, int64_t bw,
-                                                       int64_t bx, int64_t by) {
+                                                       int64_t bx, int64_t by,
+                                                       const int* __restrict__ skip,
+                                                       int64_t skip_stride) {
   __shared__ double2 W[MIX_MAX * MIX_MAX];
+  if (skip && skip[blockIdx.z * skip_stride]) return;
   wm += blockIdx.z * bw;
   x += blockIdx.z * bx;
   y += blockIdx.z * by;
@@ -55,22 +58,23 @@ int mode_mix(int64_t nl, int64_t ni, int64_t no, int64_t nr, const void* wm, con
   if (nl > 65535 || batch > 65535) return PTN_ERR_ARG;
   dim3 grid((unsigned)((nr + 255) / 256), (unsigned)nl, (unsigned)batch);
   const int64_t bx = nl * ni * nr, by = nl * no * nr;
+  const SkipFlag sk = skip_flag();
   if (tx == PTN_T_C128 && ty == PTN_T_C128)
     mode_mix_kernel<double2, double2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
                                                            (const double2*)wm, (const double2*)x,
-                                                           (double2*)y, bw, bx, by);
+                                                           (double2*)y, bw, bx, by, sk.p, sk.stride);
   else if (tx == PTN_T_C64 && ty == PTN_T_C64)
     mode_mix_kernel<float2, float2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
                                                          (const double2*)wm, (const float2*)x,
-                                                         (float2*)y, bw, bx, by);
+                                                         (float2*)y, bw, bx, by, sk.p, sk.stride);
   else if (tx == PTN_T_C128 && ty == PTN_T_C64)
     mode_mix_kernel<double2, float2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
                                                           (const double2*)wm, (const double2*)x,
-                                                          (float2*)y, bw, bx, by);
+                                                          (float2*)y, bw, bx, by, sk.p, sk.stride);
   else
     mode_mix_kernel<float2, double2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
                                                           (const double2*)wm, (const float2*)x,
-                                                          (double2*)y, bw, bx, by);
+                                                          (double2*)y, bw, bx, by, sk.p, sk.stride);
   PTN_LAUNCH_CHECK();
   return PTN_OK;
 }

@@ -474,3 +474,55 @@ def test_block_sparse_contractions_match_dense(Dl, Dr, two_site, precision, tol)
     rb = _qsparse(rng, (Dl, len(qm_l), Dl), [ql, qm_l, -ql])
     assert relerr(backend.bond_apply(c, l, rb, precision, block_sparse=True),
                   orc.apply_local_bond_contraction(c, l, rb)) < tol
+
+
+def _chain_pair(L, D, seeds):
+    out = []
+    for seed in seeds:
+        bonds = [min(D, 2**min(i, 30), 2**min(L - i, 30)) for i in range(L + 1)]
+        out.append(ptn.MPS(np.array([0, 0]), [np.zeros(b, dtype=int) for b in bonds],
+                           fill="random", rng=np.random.default_rng(seed)))
+    return out
+
+
+@pytest.mark.parametrize("L,D", [(8, 8), (16, 96)])
+def test_mps_vdot_and_norm_match_oracle(L, D):
+    """ptn.mps_vdot / mps_norm (the fidelity measure of the drivers and the
+    bench) against the oracle (mps.py:430-457) at D >= 64, chi != psi, and
+    against np.vdot of the full vectors where those fit."""
+    chi, psi = _chain_pair(L, D, (21, 22))
+    ochi = orc.MPS(chi.qsite, chi.qbonds, fill="postpone")
+    ochi.a = [a.copy() for a in chi.a]
+    opsi = orc.MPS(psi.qsite, psi.qbonds, fill="postpone")
+    opsi.a = [a.copy() for a in psi.a]
+    ref = orc.mps_vdot(ochi, opsi)
+    scale = orc.mps_norm(ochi) * orc.mps_norm(opsi)
+    gchi, gpsi = ptn.MPS(chi.qsite, chi.qbonds, fill="postpone"), ptn.MPS(psi.qsite, psi.qbonds,
+                                                                          fill="postpone")
+    gchi.a, gpsi.a = [a.copy() for a in chi.a], [a.copy() for a in psi.a]
+    gchi.to_device(); gpsi.to_device()
+    got = ptn.mps_vdot(gchi, gpsi)
+    assert abs(got - ref) <= 1e-12 * scale
+    assert abs(ptn.mps_vdot(gpsi, gchi) - np.conj(ref)) <= 1e-12 * scale
+    assert abs(ptn.mps_norm(gpsi) - orc.mps_norm(opsi)) <= 1e-12 * orc.mps_norm(opsi)
+    if L <= 10:
+        x, y = ochi.to_vector(), opsi.to_vector()
+        assert abs(got - np.vdot(x, y)) <= 1e-12 * np.linalg.norm(x) * np.linalg.norm(y)
+
+
+def test_live_launch_timing_counts_products():
+    """ptn_timing_begin/end: every complex-product launch between them is
+    counted once per kernel instance with its algorithmic flops."""
+    rng = np.random.default_rng(3)
+    A, B = dev(orc.crandn((300, 70), rng)), dev(orc.crandn((70, 90), rng))
+    backend.timing_begin()
+    for _ in range(3):
+        backend.matmul(A, B)
+    rows = backend.timing_end()
+    assert sum(r["launches"] for r in rows) == 3
+    assert abs(sum(r["flop"] for r in rows) - 3 * 8.0 * 300 * 90 * 70) < 1
+    assert all(r["total_ms"] > 0 and r["name"].startswith("zgemm_kernel<double, c128") for r in rows)
+    # off again: further launches are not recorded
+    backend.matmul(A, B)
+    backend.timing_begin()
+    assert backend.timing_end() == []

@@ -40,3 +40,28 @@ def test_sectors_follow_the_stable_sort_of_the_quantum_numbers():
     assert [int(qn) for qn, _, _ in secs] == [-1, 0, 1]
     assert [list(r) for _, r, _ in secs] == [[1], [2, 4], [0, 3]]
     assert [list(c) for _, _, c in secs] == [[3], [0, 4], [1, 2]]
+
+
+def test_krylov_status_words_map_to_the_reference_errors():
+    """Each status bit of the device Lanczos (include/ptn.h) maps to the
+    reference's behaviour: breakdown -> RuntimeWarning (krylov.py:44-50), zero
+    start -> AssertionError (krylov.py:30), no convergence of the tridiagonal
+    eigensolver -> LinAlgError (np.linalg.eigh at krylov.py:150), non-finite
+    value -> FloatingPointError."""
+    import warnings
+    import pytest
+    ok = np.array([[8, 0, 0, 0]], dtype=np.int32)
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        backend.report_krylov_status(ok)
+    with pytest.warns(RuntimeWarning, match=r"beta\[3\] ~= 0"):
+        backend.report_krylov_status(np.array([[4, 4, 0, 0]], dtype=np.int32))
+    with pytest.raises(AssertionError):
+        backend.report_krylov_status(np.array([[8, 0, 0, 1]], dtype=np.int32))
+    with pytest.raises(FloatingPointError):
+        backend.report_krylov_status(np.array([[8, 0, 1, 0]], dtype=np.int32))
+    with pytest.raises(np.linalg.LinAlgError):
+        backend.report_krylov_status(np.array([[8, 0, 2, 0]], dtype=np.int32))
+    # every row of a batch is checked
+    with pytest.raises(np.linalg.LinAlgError):
+        backend.report_krylov_status(np.array([[8, 0, 0, 0], [8, 0, 2, 0]], dtype=np.int32))

@@ -14,6 +14,7 @@ from scipy.linalg import expm
 import oracle as orc
 from conftest import load_fixture, mps_arrays, mpo_arrays
 from pytenet_amd.hamiltonian import heisenberg_xxz_1d_mpo, ising_1d_mpo
+import pytenet_amd as ptn
 
 MPO_FIXTURES = ["mpo_xxz_L2", "mpo_xxz_L3", "mpo_xxz_L5", "mpo_xxz_L10", "mpo_xxz_h0_L2",
                 "mpo_xxz_h0_L3", "mpo_xxz_h0_L6", "mpo_ising_L1", "mpo_ising_L2", "mpo_ising_L4",
@@ -367,3 +368,52 @@ def test_dmrg_matches_exact_diagonalisation():
     e_exact = np.linalg.eigvalsh(xxz_kron(L, J, D, h))[0]
     assert abs(en[-1] - e_exact) < 1e-12
 
+
+
+# ---------------------------------------------------------------------------
+# the fidelity measure and the seeded constructors of the product package
+
+def _random_chain(L, D, seed):
+    rng = np.random.default_rng(seed)
+    bonds = [min(D, 2**i, 2**(L - i)) for i in range(L + 1)]
+    return orc.MPS(np.array([0, 0]), [np.zeros(b, dtype=int) for b in bonds], fill="random",
+                   rng=rng)
+
+
+def test_oracle_vdot_and_norm_equal_dense_vectors():
+    """<chi|psi> and |psi| of the oracle (mps.py:430-457) equal np.vdot of the
+    full state vectors, for chi != psi and for the norm."""
+    chi, psi = _random_chain(8, 6, 1), _random_chain(8, 6, 2)
+    x, y = chi.to_vector(), psi.to_vector()
+    assert abs(orc.mps_vdot(chi, psi) - np.vdot(x, y)) < 1e-13 * np.linalg.norm(x) * np.linalg.norm(y)
+    assert abs(orc.mps_vdot(psi, chi) - np.vdot(y, x)) < 1e-13 * np.linalg.norm(x) * np.linalg.norm(y)
+    assert abs(orc.mps_norm(psi) - np.linalg.norm(y)) < 1e-13 * np.linalg.norm(y)
+    # conjugate symmetry and a planted mistake (no conjugation of chi) that the check catches
+    assert abs(orc.mps_vdot(chi, psi) - np.conj(orc.mps_vdot(psi, chi))) < 1e-13
+    assert abs(np.sum(x * y) - np.vdot(x, y)) > 1e-3 * abs(np.vdot(x, y))
+
+
+def test_product_seeded_mps_matches_reference_fixture():
+    """pytenet_amd.MPS(fill="random", rng) builds the reference's seeded state
+    bit for bit (pytenet/mps.py:28-71, util.py:10-17), on the host."""
+    z = load_fixture("traj_dmrg1_xxz_L16")
+    qsite, qb, a = mps_arrays(z, "init_")
+    psi = ptn.MPS(qsite, [np.zeros(len(q), dtype=int) for q in qb], fill="random",
+                  rng=np.random.default_rng(int(z["seed"])))
+    assert not psi.on_device
+    for i in range(len(a)):
+        assert psi.a[i].dtype == a[i].dtype and np.array_equal(psi.a[i], a[i])
+
+
+def test_product_construct_random_matches_reference_fixture():
+    """MPS.construct_random with U(1) quantum numbers (pytenet/mps.py:73-113):
+    the same bond labels and bit-identical entries, incl. the zeroed entries of
+    the quantum-number sparsity pattern."""
+    z = load_fixture("traj_dmrg2_xxz_L10_qn")
+    qsite, qb, a = mps_arrays(z, "init_")
+    psi = ptn.MPS.construct_random(len(a), qsite, 4, max_vdim=18, dtype="real",
+                                   rng=np.random.default_rng(int(z["seed"])))
+    for i in range(len(qb)):
+        assert np.array_equal(np.asarray(psi.qbonds[i]), qb[i])
+    for i in range(len(a)):
+        assert np.array_equal(psi.a[i], a[i])
