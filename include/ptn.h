@@ -41,6 +41,7 @@ extern "C" {
 #define PTN_ERR_WORKSPACE    2   /* workspace too small */
 #define PTN_ERR_HIP          3   /* HIP runtime error (see ptn_last_hip_error) */
 #define PTN_ERR_UNSUPPORTED  4   /* precision / shape combination not built */
+#define PTN_ERR_NONFINITE    5   /* a singular value was not finite (ptn_svd_split) */
 
 /* precision modes */
 #define PTN_FP64    0
@@ -71,6 +72,12 @@ extern "C" {
 int ptn_version(void);
 const char* ptn_status_string(int status);
 const char* ptn_last_hip_error(void);
+
+/* Shape rules of K1-K4: virtual bonds Dl, Dr >= 0 (a zero-width bond, as a
+ * split that keeps no singular value leaves it, pytenet/bond_ops.py:28-29,
+ * gives an empty output or zeros, as NumPy's tensordot does); d, Ml, Mr >= 1
+ * with d*Ml <= 32 and d*Mr <= 32 (the MPO mode mix holds one W slice in LDS;
+ * two-site operators pass d = d0*d1). Other shapes return PTN_ERR_ARG. */
 
 /* ---- K1: effective Hamiltonian, replaces pytenet/chain_ops.py:237-279 apply_local_hamiltonian(a, w, l, r)
  * out[lb, sb, rb] = sum L[l,m,lb] W[m,sb,s,m'] A[l,s,r] R[r,m',rb] */
@@ -161,6 +168,33 @@ size_t ptn_svd_workspace(int64_t m, int64_t n);
 int ptn_svd(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t ldu,
             double* s, void* vh, int64_t ldvh, int32_t* info,
             void* ws, size_t ws_bytes, void* stream);
+
+/* ---- K12 + K13: truncating two-site split of one quantum-number sector (or
+ * of a tensor with zero quantum numbers); replaces mps_split_tensor_svd with
+ * svd_distr "left" / "right" (pytenet/mps.py:538-568) together with
+ * split_block_sparse_matrix_svd / retained_bond_indices
+ * (pytenet/bond_ops.py:23-54) and the SVD at pytenet/block_sparse_util.py:294.
+ * Theta (m x n row-major, leading dimension lda) -> k, the number of singular
+ * values kept by the reference's discarded-weight rule with `tol` and then at
+ * most `max_bond` of them (max_bond <= 0: no cap), and
+ *   distr PTN_SPLIT_RIGHT: a0 = U_k (m x k), a1 = Sigma_k V_k^H (k x n);
+ *   distr PTN_SPLIT_LEFT:  a0 = U_k Sigma_k (m x k), a1 = V_k^H (k x n),
+ * row-major; a0 / a1 need room for m * kmax / kmax * n elements, kmax =
+ * min(m, n, max_bond). s_host (host double[min(m, n)]) receives all singular
+ * values in descending order; *k_out the kept count; info_host (optional host
+ * int32[4]): FP64 Jacobi sweeps, not converged, non-finite, FP32 sweeps.
+ * Only the isometric factor's singular vectors are computed (one-sided Jacobi
+ * on the QR-preconditioned matrix without accumulating the rotations); the
+ * other factor is the projection of Theta onto them (one product).
+ * precond32 != 0 starts the Jacobi iteration with FP32 sweeps (the FP64 output
+ * accuracy is kept). The call synchronises the stream (k sizes the outputs).
+ * Returns PTN_ERR_NONFINITE when a singular value is not finite. */
+#define PTN_SPLIT_LEFT  0
+#define PTN_SPLIT_RIGHT 1
+size_t ptn_svd_split_workspace(int64_t m, int64_t n, int distr, int precond32);
+int ptn_svd_split(int64_t m, int64_t n, const void* theta, int64_t lda, int distr, double tol,
+                  int64_t max_bond, int precond32, void* a0, void* a1, double* s_host,
+                  int64_t* k_out, int32_t* info_host, void* ws, size_t ws_bytes, void* stream);
 
 /* ---- K14: multiply a (n0, n1, n2) complex tensor by real factors along axis 0
  * or axis 2, in place; replaces the broadcasts at pytenet/mps.py:558-566. */

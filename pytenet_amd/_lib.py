@@ -15,6 +15,7 @@ PTN_ERR_ARG = 1
 PTN_ERR_WORKSPACE = 2
 PTN_ERR_HIP = 3
 PTN_ERR_UNSUPPORTED = 4
+PTN_ERR_NONFINITE = 5
 
 PTN_FP64 = 0
 PTN_FP32 = 1
@@ -23,6 +24,9 @@ PTN_BLOCK_SPARSE = 0x100
 
 PTN_OP_HEFF = 0
 PTN_OP_BOND = 1
+
+PTN_SPLIT_LEFT = 0
+PTN_SPLIT_RIGHT = 1
 
 c_i64 = ctypes.c_int64
 c_p = ctypes.c_void_p
@@ -94,6 +98,9 @@ SIGNATURES = [
     ("ptn_set_autotune", c_int, [c_int]),
     ("ptn_svd_workspace_precond32", c_sz, [c_i64, c_i64, c_i64]),
     ("ptn_svd_precond32", c_int, [c_i64, c_i64, c_i64, c_p, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
+    ("ptn_svd_split_workspace", c_sz, [c_i64, c_i64, c_int, c_int]),
+    ("ptn_svd_split", c_int, [c_i64, c_i64, c_p, c_i64, c_int, c_dbl, c_i64, c_int, c_p, c_p,
+                              c_p, ctypes.POINTER(c_i64), c_p, c_p, c_sz, c_p]),
     ("ptn_timing_begin", c_int, []),
     ("ptn_timing_end", c_int, [ctypes.POINTER(KernelTiming), c_int, ctypes.POINTER(c_int)]),
 ]

@@ -229,6 +229,9 @@ def krylov_local(kind, v, l, w, r, numiter, mode, dt=0.0, precision="fp64", info
     v, l, r = as_dev(v), as_dev(l), as_dev(r)
     if w is not None:
         w = as_dev(w)
+    if v.numel() == 0:
+        # an empty start vector has norm zero (the reference asserts, krylov.py:30)
+        raise AssertionError("Lanczos start vector has norm zero")
     if kind == _lib.PTN_OP_HEFF:
         Dl, d, Dr = v.shape
     else:
@@ -350,6 +353,38 @@ def svd(a, return_info=False, precond32=False):
     if return_info:
         return u, s, vh, inf
     return u, s, vh
+
+
+def svd_split(a, distr, tol=0.0, max_bond=None, precond32=False):
+    """Truncating split of a 2-D complex128 device tensor (one sector):
+    (a0, a1, s) with distr "right": a0 = U_k, a1 = Sigma_k V_k^H; "left":
+    a0 = U_k Sigma_k, a1 = V_k^H; s = all singular values (host, descending).
+    k follows the reference's discarded-weight rule with `tol` (bond_ops.py:23-38)
+    and the cap `max_bond`. Synchronises the stream."""
+    a = as_dev(a)
+    m, n = a.shape
+    code = {"left": _lib.PTN_SPLIT_LEFT, "right": _lib.PTN_SPLIT_RIGHT}[distr]
+    q = min(m, n)
+    kmax = q if max_bond is None else max(min(q, int(max_bond)), 0)
+    a0 = torch.empty(max(m * kmax, 1), dtype=torch.complex128, device=a.device)
+    a1 = torch.empty(max(kmax * n, 1), dtype=torch.complex128, device=a.device)
+    s = np.zeros(q, dtype=np.float64)
+    if q == 0 or kmax == 0:
+        return a0[:0].reshape(m, 0), a1[:0].reshape(0, n), s
+    lib = _lib.load()
+    ws = workspace(lib.ptn_svd_split_workspace(m, n, code, int(bool(precond32))))
+    k = ctypes.c_int64(0)
+    info = np.zeros(4, dtype=np.int32)
+    st = lib.ptn_svd_split(m, n, ptr(a), a.stride(0), code, float(tol),
+                           0 if max_bond is None else int(max_bond), int(bool(precond32)),
+                           ptr(a0), ptr(a1), s.ctypes.data_as(c_p), ctypes.byref(k),
+                           info.ctypes.data_as(c_p), ptr(ws), ws.numel(), stream())
+    if st == _lib.PTN_ERR_NONFINITE:
+        raise FloatingPointError("non-finite singular value")
+    _check(st, "ptn_svd_split")
+    _svd_status(info, f"a {m} x {n} matrix")
+    k = k.value
+    return a0[:m * k].reshape(m, k), a1[:k * n].reshape(k, n), s
 
 
 def scale_axis(t, s, axis):

@@ -10,6 +10,10 @@ size_t krylov_ws_bytes(const ptn_local_op* op, int numiter, int64_t batch);
 int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, double dt_im,
                int numiter, void* out, double* energy, int32_t* info, double* alpha_beta,
                void* ws, size_t ws_bytes, hipStream_t s, int64_t batch);
+size_t svd_split_ws_bytes(int64_t m, int64_t n, int distr, bool precond32);
+int svd_split_run(int64_t m, int64_t n, const void* theta, int64_t lda, int distr, double tol,
+                  int64_t max_bond, bool precond32, void* a0, void* a1, double* s_host,
+                  int64_t* k_out, int32_t* info_host, void* ws, size_t ws_bytes, hipStream_t st);
 size_t qr_ws_bytes(int64_t m, int64_t n, int64_t batch);
 int qr_run(int64_t m, int64_t n, const void* a, int64_t lda, void* q, int64_t ldq, void* r,
            int64_t ldr, void* ws, size_t ws_bytes, hipStream_t s, int64_t batch);
@@ -89,6 +93,7 @@ const char* ptn_status_string(int status) {
     case PTN_ERR_WORKSPACE: return "workspace too small";
     case PTN_ERR_HIP: return "HIP runtime error";
     case PTN_ERR_UNSUPPORTED: return "unsupported precision or shape";
+    case PTN_ERR_NONFINITE: return "non-finite singular value";
   }
   return "unknown status";
 }
@@ -115,7 +120,7 @@ int ptn_heff_apply(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml,
                    size_t ws_bytes, void* stream) {
   if (!precision_ok(precision)) return PTN_ERR_UNSUPPORTED;
   Dims D = {Dl, d, Dr, Ml, Mr};
-  if (Dl <= 0 || d <= 0 || Dr <= 0 || Ml <= 0 || Mr <= 0) return PTN_ERR_ARG;
+  if (Dl < 0 || d <= 0 || Dr < 0 || Ml <= 0 || Mr <= 0) return PTN_ERR_ARG;
   if (ws_bytes < ptn_heff_workspace(precision, Dl, d, Dr, Ml, Mr)) return PTN_ERR_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
   const size_t base = al256(heff_ws_bytes(precision, D));
@@ -140,7 +145,7 @@ size_t ptn_bond_workspace(int precision, int64_t Dl, int64_t Dr, int64_t M) {
 int ptn_bond_apply(int precision, int64_t Dl, int64_t Dr, int64_t M, const void* l, const void* r,
                    const void* c, void* out, void* ws, size_t ws_bytes, void* stream) {
   if (!precision_ok(precision)) return PTN_ERR_UNSUPPORTED;
-  if (Dl <= 0 || Dr <= 0 || M <= 0) return PTN_ERR_ARG;
+  if (Dl < 0 || Dr < 0 || M <= 0) return PTN_ERR_ARG;
   if (ws_bytes < ptn_bond_workspace(precision, Dl, Dr, M)) return PTN_ERR_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
   const size_t base = al256(bond_ws_bytes(precision, Dl, Dr, M));
@@ -172,7 +177,7 @@ static int env_step_b(int left, int precision, int64_t batch, int64_t Dl, int64_
                       const void* blk, void* out, void* ws, size_t ws_bytes, hipStream_t s) {
   if (!precision_ok(precision)) return PTN_ERR_UNSUPPORTED;
   Dims D = {Dl, d, Dr, Ml, Mr};
-  if (Dl <= 0 || d <= 0 || Dr <= 0 || Ml <= 0 || Mr <= 0 || batch < 1) return PTN_ERR_ARG;
+  if (Dl < 0 || d <= 0 || Dr < 0 || Ml <= 0 || Mr <= 0 || batch < 1) return PTN_ERR_ARG;
   if (ws_bytes < env_workspace_b(precision, batch, Dl, d, Dr, Ml, Mr)) return PTN_ERR_WORKSPACE;
   const size_t base = al256(env_ws_bytes(precision, D, batch));
   const void* e = blk;
@@ -277,6 +282,17 @@ int ptn_svd(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
             void* vh, int64_t ldvh, int32_t* info, void* ws, size_t ws_bytes, void* stream) {
   return svd_run(m, n, a, lda, u, ldu, s, vh, ldvh, info, ws, ws_bytes, (hipStream_t)stream, 1,
                  0, 0, 0, 0, false);
+}
+
+size_t ptn_svd_split_workspace(int64_t m, int64_t n, int distr, int precond32) {
+  return svd_split_ws_bytes(m, n, distr, precond32 != 0);
+}
+
+int ptn_svd_split(int64_t m, int64_t n, const void* theta, int64_t lda, int distr, double tol,
+                  int64_t max_bond, int precond32, void* a0, void* a1, double* s_host,
+                  int64_t* k_out, int32_t* info_host, void* ws, size_t ws_bytes, void* stream) {
+  return svd_split_run(m, n, theta, lda, distr, tol, max_bond, precond32 != 0, a0, a1, s_host,
+                       k_out, info_host, ws, ws_bytes, (hipStream_t)stream);
 }
 
 size_t ptn_svd_workspace_batched(int64_t batch, int64_t m, int64_t n) {

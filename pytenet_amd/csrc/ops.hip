@@ -81,8 +81,11 @@ size_t bond_ws_bytes(int prec, int64_t Dl, int64_t Dr, int64_t M, int64_t batch)
          mask_bytes2(Dl, M * Dr, Dr, Dl, Dr, Dl * M) + 256;
 }
 
+// zero-width virtual bonds (Dl or Dr = 0, as a split that keeps no singular
+// value leaves them, pytenet/bond_ops.py:28-29) are valid: the products then
+// have an empty output or an empty sum (zeros), as NumPy's tensordot gives
 static bool dims_ok(const Dims& D) {
-  return D.Dl > 0 && D.d > 0 && D.Dr > 0 && D.Ml > 0 && D.Mr > 0 && D.d * D.Mr <= 32 &&
+  return D.Dl >= 0 && D.d > 0 && D.Dr >= 0 && D.Ml > 0 && D.Mr > 0 && D.d * D.Mr <= 32 &&
          D.Ml * D.d <= 32;
 }
 
@@ -201,7 +204,7 @@ int env_left(int prec, const Dims& D, const void* a, const void* b, const void* 
 int bond_apply(int prec, int64_t Dl, int64_t Dr, int64_t M, const void* l, int tl, const void* r,
                int tr, const void* c, int tcin, void* out, int tout, void* ws, size_t ws_bytes,
                hipStream_t s, int64_t batch) {
-  if (Dl <= 0 || Dr <= 0 || M <= 0 || batch < 1) return PTN_ERR_ARG;
+  if (Dl < 0 || Dr < 0 || M <= 0 || batch < 1) return PTN_ERR_ARG;
   if (ws_bytes < bond_ws_bytes(prec, Dl, Dr, M, batch)) return PTN_ERR_WORKSPACE;
   const bool sp = is_sparse(prec) && batch == 1;
   prec = base_prec(prec);

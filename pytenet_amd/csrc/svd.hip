@@ -353,25 +353,27 @@ __global__ __launch_bounds__(256) void k_jac_solve(int nblk, const typename JacT
   }
 }
 
-// Y <- Y J (pass blockIdx.z & 1 = 0, rows of X) or V <- V J (pass 1) for AROWS rows
-// per workgroup, 16 per wave, on MFMA: the rows' 32 complex entries are
+// Y <- Y J (pass 0, rows of X) or V <- V J (pass 1) for AROWS rows per
+// workgroup, 16 per wave, on MFMA: the rows' 32 complex entries are
 // the A operand (real k = 2 column + part, the interleaved complex layout) and
 // J's real embedding [[Re, Im], [-Im, Re]] the B operand, read from LDS.
 // Every wave loads all A operands of its rows before it stores any result.
-// Problem blockIdx.z >> 1 of a batch: X (p x qp) and V (qp x qp) stacked.
+// nmat = 2: problem blockIdx.z >> 1 of a batch, pass blockIdx.z & 1 (X and V);
+// nmat = 1: problem blockIdx.z, X only (the rotations are not accumulated).
+// X (p x qp) and V (qp x qp) stacked per problem.
 template <typename T>
 __global__ __launch_bounds__(256) void k_jac_apply(int64_t p, int64_t qp, int nblk, int r,
                                                    typename JacT<T>::c2* __restrict__ X,
                                                    typename JacT<T>::c2* __restrict__ V,
                                                    const typename JacT<T>::c2* __restrict__ Jg,
-                                                   const int* __restrict__ rotflag) {
+                                                   const int* __restrict__ rotflag, int nmat) {
   using C2 = typename JacT<T>::c2;
   using Acc = typename JacT<T>::acc;
   __shared__ C2 J[JP][JP + 1];
-  const int64_t prob = blockIdx.z >> 1;
+  const int64_t prob = nmat == 2 ? (blockIdx.z >> 1) : blockIdx.z;
   const int64_t pidx = prob * gridDim.y + blockIdx.y;
   if (!rotflag[pidx]) return;
-  const int pass = blockIdx.z & 1;
+  const int pass = nmat == 2 ? (blockIdx.z & 1) : 0;
   C2* M = pass ? V + prob * qp * qp : X + prob * p * qp;
   const int64_t ld = pass ? qp : p;
   const int64_t rowblk = (int64_t)blockIdx.x * AROWS;
@@ -731,6 +733,130 @@ static inline unsigned gs_blocks(int64_t total) {
 // the FP64 sweeps continue from X Q with V = Q. The FP64 sweeps and their
 // stopping rule are those of the plain path, so the result has the same FP64
 // accuracy; the FP32 sweeps only remove most of the rotation work.
+// Jacobi phase on X (q x qp column-major per problem, the preconditioned
+// matrix R2^H). With want_v the rotations are accumulated into V (qp x qp per
+// problem); without, V's buffer only serves the FP32 preconditioning (polar
+// factor). On return X points at the orthogonalised columns (another buffer
+// of the workspace after FP32 preconditioning).
+struct JacobiResult {
+  int sweeps = 0, sweeps32 = 0, converged = 0;
+};
+static int jacobi_core(const SvdLayout& L, char* base, double2*& X, double2* V, bool want_v,
+                       hipStream_t st, JacobiResult& res) {
+  const unsigned NB = (unsigned)L.batch;
+  const int npairs = L.nblk / 2;
+  const int64_t aseg = (L.qp + AROWS - 1) / AROWS;   // qp >= q rows
+  double2* Gp = (double2*)(base + L.gp);
+  double2* Jg = (double2*)(base + L.jg);
+  int* rflag = (int*)(base + L.rflag);
+  double* nrm = (double*)(base + L.nrm);
+  int* counters = (int*)(base + L.cnt);   // [0] rotations in the current sweep, [1] non-finite
+  const int64_t sx = L.q * L.qp;
+  const int batch = (int)L.batch;
+  int stt;
+  int* host_rot = pinned_counter();
+  if (!host_rot) return PTN_ERR_HIP;
+  k_set_ints<<<1, 64, 0, st>>>(counters, 2, 0);
+  if (L.pre32) {
+    float2* X32 = (float2*)(base + L.x32);
+    float2* V32 = (float2*)(base + L.v32);
+    float2* Gp32 = (float2*)Gp;
+    float2* Jg32 = (float2*)Jg;
+    double* scale = (double*)(base + L.scale);
+    k_colnorms<<<dim3((unsigned)L.q, NB), 256, 0, st>>>(L.q, X, nrm, sx, L.qp);
+    k_maxnorm<<<NB, 256, 0, st>>>(L.q, nrm, L.qp, scale);
+    k_narrow_scaled<<<dim3(gs_blocks(sx), NB), 256, 0, st>>>(sx, X, scale, X32);
+    k_eye32<<<dim3(gs_blocks(L.qp * L.qp), NB), 256, 0, st>>>(L.qp, V32);
+    PTN_LAUNCH_CHECK();
+    const float tol32 = 32.0f * FLT_EPSILON, big32 = 1e-3f;
+    const int max_sweeps32 = 20;
+    while (res.sweeps32 < max_sweeps32) {
+      k_set_ints<<<1, 64, 0, st>>>(counters, 1, 0);
+      for (int r = 0; r < L.nblk - 1; ++r) {
+        k_jac_gram<float><<<dim3((unsigned)L.nseg, (unsigned)npairs, NB), 256, 0, st>>>(
+            L.q, L.nblk, r, X32, Gp32, (int)L.nseg, sx);
+        k_jac_solve<float><<<dim3((unsigned)npairs, NB), 256, 0, st>>>(
+            L.nblk, Gp32, (int)L.nseg, tol32, big32, Jg32, rflag, counters);
+        k_jac_apply<float><<<dim3((unsigned)aseg, (unsigned)npairs, 2 * NB), 256, 0, st>>>(
+            L.q, L.qp, L.nblk, r, X32, V32, Jg32, rflag, 2);
+      }
+      PTN_LAUNCH_CHECK();
+      PTN_HIP_CHECK(hipMemcpyAsync(host_rot, counters, sizeof(int), hipMemcpyDeviceToHost, st));
+      PTN_HIP_CHECK(hipStreamSynchronize(st));
+      ++res.sweeps32;
+      if (*host_rot == 0) break;
+    }
+    // unitary polar factor of V32 (qp x qp column-major per problem) into V
+    const int64_t nv = L.qp * L.qp, nvb = nv * batch;
+    double2* Q = (double2*)(base + L.t1);
+    double2* QhQ = (double2*)(base + L.t2);
+    double2* QQ = (double2*)(base + L.t3);
+    k_widen<<<gs_blocks(nvb), 256, 0, st>>>(nvb, V32, Q);
+    PTN_LAUNCH_CHECK();
+    for (int it = 0; it < 2; ++it) {
+      ZGemm g1 = {L.qp, L.qp, L.qp, Q, L.qp, 1, 1, Q, 1, L.qp, 0, QhQ, 1, L.qp, 0,
+                  L.batch, nv, nv, nv};
+      if ((stt = zgemm(g1, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, st)))
+        return stt;
+      ZGemm g2 = {L.qp, L.qp, L.qp, Q, 1, L.qp, 0, QhQ, 1, L.qp, 0, QQ, 1, L.qp, 0,
+                  L.batch, nv, nv, nv};
+      if ((stt = zgemm(g2, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, st)))
+        return stt;
+      k_ns_update<<<gs_blocks(nvb), 256, 0, st>>>(nvb, Q, QQ, it == 1 ? V : Q);
+      PTN_LAUNCH_CHECK();
+    }
+    // X <- X V (q x qp column-major per problem), into t3
+    ZGemm g3 = {L.q, L.qp, L.qp, X, 1, L.q, 0, V, 1, L.qp, 0, QQ, 1, L.q, 0,
+                L.batch, sx, nv, sx};
+    if ((stt = zgemm(g3, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, st))) return stt;
+    X = QQ;
+  } else if (want_v) {
+    k_eye<<<dim3(gs_blocks(L.qp * L.qp), NB), 256, 0, st>>>(L.qp, V);
+    PTN_LAUNCH_CHECK();
+  }
+  const double tol = DBL_EPSILON * sqrt((double)L.q);
+  // the iteration stops after a sweep in which no pair needed a rotation above
+  // this relative size: convergence is quadratic, so the couplings left after
+  // such a sweep are of order big^2 (n big^2 at most), and a further sweep
+  // would only rotate by rounding-level angles
+  const double big = 1e-8;
+  const int max_sweeps = 60;
+  const int nmat = want_v ? 2 : 1;
+  res.converged = (L.q < 2);
+  while (!res.converged && res.sweeps < max_sweeps) {
+    k_set_ints<<<1, 64, 0, st>>>(counters, 1, 0);
+    for (int r = 0; r < L.nblk - 1; ++r) {
+      k_jac_gram<double><<<dim3((unsigned)L.nseg, (unsigned)npairs, NB), 256, 0, st>>>(
+          L.q, L.nblk, r, X, Gp, (int)L.nseg, sx);
+      k_jac_solve<double><<<dim3((unsigned)npairs, NB), 256, 0, st>>>(L.nblk, Gp, (int)L.nseg, tol,
+                                                                      big, Jg, rflag, counters);
+      k_jac_apply<double><<<dim3((unsigned)aseg, (unsigned)npairs, nmat * NB), 256, 0, st>>>(
+          L.q, L.qp, L.nblk, r, X, V, Jg, rflag, nmat);
+    }
+    PTN_LAUNCH_CHECK();
+    PTN_HIP_CHECK(hipMemcpyAsync(host_rot, counters, sizeof(int), hipMemcpyDeviceToHost, st));
+    PTN_HIP_CHECK(hipStreamSynchronize(st));
+    ++res.sweeps;
+    if (*host_rot == 0) res.converged = 1;
+  }
+  return PTN_OK;
+}
+
+// Thin SVD of `batch` m x n matrices of identical shape (problem b at a + b sa,
+// u + b su, s + b ss, vh + b svh). The Jacobi sweeps run on all problems at
+// once and stop together, after the first sweep in which no problem needed a
+// rotation above `big` (relative). info (if given) receives the number of
+// sweeps, whether the sweep limit was hit, and whether any singular value was
+// not finite.
+// With `precond32` (and at least PRE32_MIN_Q columns) the Jacobi phase starts
+// with FP32 sweeps: X32 = X / max column norm in complex64 is orthogonalised
+// on FP32 MFMA until no pair needs a rotation above 1e-3 relative, with the
+// rotations accumulated in V32. V32 is unitary only to FP32 rounding, so it is
+// replaced by its unitary polar factor Q (two Newton-Schulz steps in FP64,
+// Q <- 1.5 Q - 0.5 Q (Q^H Q), each squaring the distance from unitarity), and
+// the FP64 sweeps continue from X Q with V = Q. The FP64 sweeps and their
+// stopping rule are those of the plain path, so the result has the same FP64
+// accuracy; the FP32 sweeps only remove most of the rotation work.
 int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t ldu, double* s,
             void* vh, int64_t ldvh, int32_t* info, void* ws, size_t ws_bytes, hipStream_t st,
             int64_t batch, int64_t sa, int64_t su, int64_t ss, int64_t svh, bool precond32) {
@@ -750,12 +876,9 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
   double2* V = (double2*)(base + L.v);
   double2* UB = (double2*)(base + L.ub);
   double2* VB = (double2*)(base + L.vb);
-  double2* Gp = (double2*)(base + L.gp);
-  double2* Jg = (double2*)(base + L.jg);
-  int* rflag = (int*)(base + L.rflag);
   double* nrm = (double*)(base + L.nrm);
   int64_t* perm = (int64_t*)(base + L.perm);
-  int* counters = (int*)(base + L.cnt);   // [0] rotations in the current sweep, [1] non-finite
+  int* counters = (int*)(base + L.cnt);
   const int64_t sx = L.q * L.qp;          // X: q x qp column-major per problem
   int stt;
 
@@ -782,91 +905,8 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
   k_rh<<<dim3(gs_blocks(L.q * L.qp), NB), 256, 0, st>>>(L.q, L.q, L.q, L.qp, 1, f2.W, f2.rdiag, X,
                                                         f2.bs.W, f2.bs.k);
   PTN_LAUNCH_CHECK();
-  int* host_rot = pinned_counter();
-  if (!host_rot) return PTN_ERR_HIP;
-  int sweeps32 = 0;
-  k_set_ints<<<1, 64, 0, st>>>(counters, 2, 0);
-  if (L.pre32) {
-    float2* X32 = (float2*)(base + L.x32);
-    float2* V32 = (float2*)(base + L.v32);
-    float2* Gp32 = (float2*)Gp;
-    float2* Jg32 = (float2*)Jg;
-    double* scale = (double*)(base + L.scale);
-    k_colnorms<<<dim3((unsigned)L.q, NB), 256, 0, st>>>(L.q, X, nrm, sx, L.qp);
-    k_maxnorm<<<NB, 256, 0, st>>>(L.q, nrm, L.qp, scale);
-    k_narrow_scaled<<<dim3(gs_blocks(sx), NB), 256, 0, st>>>(sx, X, scale, X32);
-    k_eye32<<<dim3(gs_blocks(L.qp * L.qp), NB), 256, 0, st>>>(L.qp, V32);
-    PTN_LAUNCH_CHECK();
-    const float tol32 = 32.0f * FLT_EPSILON, big32 = 1e-3f;
-    const int max_sweeps32 = 20;
-    while (sweeps32 < max_sweeps32) {
-      k_set_ints<<<1, 64, 0, st>>>(counters, 1, 0);
-      for (int r = 0; r < L.nblk - 1; ++r) {
-        k_jac_gram<float><<<dim3((unsigned)L.nseg, (unsigned)npairs, NB), 256, 0, st>>>(
-            L.q, L.nblk, r, X32, Gp32, (int)L.nseg, sx);
-        k_jac_solve<float><<<dim3((unsigned)npairs, NB), 256, 0, st>>>(
-            L.nblk, Gp32, (int)L.nseg, tol32, big32, Jg32, rflag, counters);
-        k_jac_apply<float><<<dim3((unsigned)aseg, (unsigned)npairs, 2 * NB), 256, 0, st>>>(
-            L.q, L.qp, L.nblk, r, X32, V32, Jg32, rflag);
-      }
-      PTN_LAUNCH_CHECK();
-      PTN_HIP_CHECK(hipMemcpyAsync(host_rot, counters, sizeof(int), hipMemcpyDeviceToHost, st));
-      PTN_HIP_CHECK(hipStreamSynchronize(st));
-      ++sweeps32;
-      if (*host_rot == 0) break;
-    }
-    // unitary polar factor of V32 (qp x qp column-major per problem) into V
-    const int64_t nv = L.qp * L.qp, nvb = nv * batch;
-    double2* Q = (double2*)(base + L.t1);
-    double2* QhQ = (double2*)(base + L.t2);
-    double2* QQ = (double2*)(base + L.t3);
-    k_widen<<<gs_blocks(nvb), 256, 0, st>>>(nvb, V32, Q);
-    PTN_LAUNCH_CHECK();
-    for (int it = 0; it < 2; ++it) {
-      ZGemm g1 = {L.qp, L.qp, L.qp, Q, L.qp, 1, 1, Q, 1, L.qp, 0, QhQ, 1, L.qp, 0,
-                  batch, nv, nv, nv};
-      if ((stt = zgemm(g1, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, st)))
-        return stt;
-      ZGemm g2 = {L.qp, L.qp, L.qp, Q, 1, L.qp, 0, QhQ, 1, L.qp, 0, QQ, 1, L.qp, 0,
-                  batch, nv, nv, nv};
-      if ((stt = zgemm(g2, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, st)))
-        return stt;
-      k_ns_update<<<gs_blocks(nvb), 256, 0, st>>>(nvb, Q, QQ, it == 1 ? V : Q);
-      PTN_LAUNCH_CHECK();
-    }
-    // X <- X V (q x qp column-major per problem), into t3
-    ZGemm g3 = {L.q, L.qp, L.qp, X, 1, L.q, 0, V, 1, L.qp, 0, QQ, 1, L.q, 0,
-                batch, sx, nv, sx};
-    if ((stt = zgemm(g3, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, st))) return stt;
-    X = QQ;
-  } else {
-    k_eye<<<dim3(gs_blocks(L.qp * L.qp), NB), 256, 0, st>>>(L.qp, V);
-    PTN_LAUNCH_CHECK();
-  }
-  const double tol = DBL_EPSILON * sqrt((double)L.q);
-  // the iteration stops after a sweep in which no pair needed a rotation above
-  // this relative size: convergence is quadratic, so the couplings left after
-  // such a sweep are of order big^2 (n big^2 at most), and a further sweep
-  // would only rotate by rounding-level angles
-  const double big = 1e-8;
-  const int max_sweeps = 60;
-  int sweeps = 0, converged = (L.q < 2);
-  while (!converged && sweeps < max_sweeps) {
-    k_set_ints<<<1, 64, 0, st>>>(counters, 1, 0);
-    for (int r = 0; r < L.nblk - 1; ++r) {
-      k_jac_gram<double><<<dim3((unsigned)L.nseg, (unsigned)npairs, NB), 256, 0, st>>>(
-          L.q, L.nblk, r, X, Gp, (int)L.nseg, sx);
-      k_jac_solve<double><<<dim3((unsigned)npairs, NB), 256, 0, st>>>(L.nblk, Gp, (int)L.nseg, tol,
-                                                                      big, Jg, rflag, counters);
-      k_jac_apply<double><<<dim3((unsigned)aseg, (unsigned)npairs, 2 * NB), 256, 0, st>>>(
-          L.q, L.qp, L.nblk, r, X, V, Jg, rflag);
-    }
-    PTN_LAUNCH_CHECK();
-    PTN_HIP_CHECK(hipMemcpyAsync(host_rot, counters, sizeof(int), hipMemcpyDeviceToHost, st));
-    PTN_HIP_CHECK(hipStreamSynchronize(st));
-    ++sweeps;
-    if (*host_rot == 0) converged = 1;
-  }
+  JacobiResult jr;
+  if ((stt = jacobi_core(L, base, X, V, true, st, jr))) return stt;
   // 3. B = Q1 R2^H Q2^H = (Q1 [U_X; 0]) diag(sigma) (Q2 V)^H with U_X = X / sigma
   k_colnorms<<<dim3((unsigned)L.q, NB), 256, 0, st>>>(L.q, X, nrm, sx, L.qp);
   k_embed_u<<<dim3(gs_blocks(L.p * L.q), NB), 256, 0, st>>>(L.p, L.q, X, nrm, UB, sx, L.qp);
@@ -880,9 +920,229 @@ int svd_run(int64_t m, int64_t n, const void* a, int64_t lda, void* u, int64_t l
   k_svd_out<<<dim3((unsigned)((tot + 255) / 256), NB), 256, 0, st>>>(
       m, n, k, adjoint, L.p, L.q, UB, VB, perm, (double2*)u, ldu, (double2*)vh, ldvh, L.qp, su,
       svh);
-  if (info) k_svd_info<<<1, 64, 0, st>>>(info, sweeps, converged ? 0 : 1, counters, sweeps32);
+  if (info)
+    k_svd_info<<<1, 64, 0, st>>>(info, jr.sweeps, jr.converged ? 0 : 1, counters, jr.sweeps32);
   PTN_LAUNCH_CHECK();
   return PTN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Truncating split (one sector): the factors of the two-site split without the
+// unneeded half of the SVD.
+//
+// For sigma into the right factor ("right") the left factor is the isometry U_k
+// of the kept left singular vectors and the right one U_k^H Theta (= Sigma_k
+// V_k^H); for "left" the roles are mirrored on Theta^H: the right factor is
+// V_k^H and the left one Theta V_k (= U_k Sigma_k). Either way only the left
+// singular vectors of one matrix M (Theta or Theta^H) are needed: with M' = M
+// (tall) or M' = R0^H from M^H = Q0 R0 (wide; same left singular vectors),
+// M' = Q1 R1 and R1^H = Q2 R2 (the two QR preconditioners of svd_run), the
+// one-sided Jacobi iteration on X = R2^H gives U_X (X V = U_X Sigma) without
+// accumulating V, and U_M = Q1 [U_X; 0]. Only the kept columns of U_X go
+// through Q1, and the other factor is one product with Theta, which also
+// carries exactly the weight of the kept subspace (the projection of Theta).
+
+// UB (column-major p x k) = [X(:, perm[c]) / sigma_perm[c]; 0] for c < k
+__global__ void k_embed_u_perm(int64_t p, int64_t q, int64_t k, const double2* __restrict__ X,
+                               const double* __restrict__ nrm, const int64_t* __restrict__ perm,
+                               double2* __restrict__ UB) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < p * k;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = t / p, i = t % p, pc = perm[c];
+    double2 v = make_double2(0.0, 0.0);
+    const double sc = nrm[pc];
+    if (i < q && sc > 0) { const double2 x = X[pc * q + i]; v = make_double2(x.x / sc, x.y / sc); }
+    UB[t] = v;
+  }
+}
+
+// out (row-major p x k, optionally conjugated) from UB (column-major p x k)
+__global__ void k_colmajor_out(int64_t p, int64_t k, const double2* __restrict__ UB,
+                               double2* __restrict__ out, int conj) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < p * k;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / k, c = t % k;
+    const double2 v = UB[c * p + i];
+    out[t] = make_double2(v.x, conj ? -v.y : v.y);
+  }
+}
+
+// elementwise complex conjugate copy (n elements)
+__global__ void k_conj_copy(int64_t n, const double2* __restrict__ x, double2* __restrict__ y) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    y[t] = make_double2(x[t].x, -x[t].y);
+}
+
+struct SplitLayout {
+  int64_t pM, qM, pi, qi;   // M (pM x qM) and the tall inner matrix M' (pi x qi)
+  bool wide;
+  size_t adj, fact0, t0, inner, total;
+  SvdLayout L;
+};
+
+// dst = product of the two input scale factors (one problem)
+__global__ void k_mul_scale(const double* a, const double* b, double* dst) {
+  if (threadIdx.x == 0) *dst = *a * *b;
+}
+
+static SplitLayout split_layout(int64_t m, int64_t n, int distr, bool precond32) {
+  SplitLayout S;
+  S.pM = distr == PTN_SPLIT_LEFT ? n : m;
+  S.qM = distr == PTN_SPLIT_LEFT ? m : n;
+  S.wide = S.pM < S.qM;
+  S.pi = S.pM;
+  S.qi = S.wide ? S.pM : S.qM;
+  size_t off = 0;
+  // Theta^H (the QR input for "left", or M^H for a wide "right" split)
+  const bool need_adj = (distr == PTN_SPLIT_LEFT) != S.wide;
+  S.adj = off; off += need_adj ? al((size_t)m * n * 16) : 0;
+  S.fact0 = off; off += S.wide ? al(qr_fact_bytes(S.qM, S.pM, S.pM)) : 0;
+  S.t0 = off; off += S.wide ? al((size_t)S.pM * S.pM * 16) : 0;
+  off += al(16);   // combined input scale (the last 16 bytes before the inner layout)
+  S.inner = off;
+  S.L = svd_layout(S.pi, S.qi, 1, precond32);
+  off += S.L.total;
+  S.total = off;
+  return S;
+}
+
+size_t svd_split_ws_bytes(int64_t m, int64_t n, int distr, bool precond32) {
+  if (distr != PTN_SPLIT_LEFT && distr != PTN_SPLIT_RIGHT) return 0;
+  return split_layout(m, n, distr, precond32).total;
+}
+
+// kept count of the discarded-weight rule (pytenet/bond_ops.py:23-38) on the
+// descending singular values s[0..q): normalise by the 2-norm, square,
+// accumulate from the smallest upwards, keep the values whose running sum
+// exceeds tol; then the cap (the max_bond largest of them; <= 0: no cap)
+static int64_t kept_count(const double* s, int64_t q, double tol, int64_t max_bond) {
+  double w2 = 0.0;
+  for (int64_t i = 0; i < q; ++i) w2 += s[i] * s[i];
+  if (!(w2 > 0.0)) return 0;
+  const double w = sqrt(w2);
+  int64_t k = q;
+  double acc = 0.0;
+  for (int64_t i = q - 1; i >= 0; --i) {   // ascending weights
+    const double pi = (s[i] / w) * (s[i] / w);
+    acc += pi;
+    if (acc > tol) break;
+    k = i;
+  }
+  if (max_bond > 0 && k > max_bond) k = max_bond;
+  return k;
+}
+
+int svd_split_run(int64_t m, int64_t n, const void* theta, int64_t lda, int distr, double tol,
+                  int64_t max_bond, bool precond32, void* a0, void* a1, double* s_host,
+                  int64_t* k_out, int32_t* info_host, void* ws, size_t ws_bytes, hipStream_t st) {
+  if (m <= 0 || n <= 0 || lda < n || !k_out || !s_host) return PTN_ERR_ARG;
+  if (distr != PTN_SPLIT_LEFT && distr != PTN_SPLIT_RIGHT) return PTN_ERR_ARG;
+  const SplitLayout S = split_layout(m, n, distr, precond32);
+  if (ws_bytes < S.total) return PTN_ERR_WORKSPACE;
+  const SvdLayout& L = S.L;
+  const int npairs = L.nblk / 2;
+  const int64_t aseg = (L.qp + AROWS - 1) / AROWS;
+  if (npairs > 65535 || aseg > 65535 || L.q > 65535) return PTN_ERR_ARG;
+  char* wsb = (char*)ws;
+  char* base = wsb + S.inner;
+  int stt;
+  const double2* th = (const double2*)theta;
+  // M^H in adj when the QR input needs it
+  double2* adj = (double2*)(wsb + S.adj);
+  const bool need_adj = (distr == PTN_SPLIT_LEFT) != S.wide;
+  if (need_adj) {
+    k_adjoint<<<dim3((unsigned)((m * n + 255) / 256), 1), 256, 0, st>>>(m, n, th, lda, adj, 0);
+    PTN_LAUNCH_CHECK();
+  }
+  // the matrix whose QR starts the factorisation: M' (tall) directly, or M^H (wide)
+  const double2* qin;
+  int64_t ldq;
+  if (!S.wide) {
+    qin = distr == PTN_SPLIT_LEFT ? adj : th;                // M = Theta^H / Theta
+    ldq = distr == PTN_SPLIT_LEFT ? m : lda;
+  } else {
+    qin = distr == PTN_SPLIT_LEFT ? th : adj;                // M^H = Theta / Theta^H
+    ldq = distr == PTN_SPLIT_LEFT ? lda : m;
+  }
+  QrFact f0;
+  if (S.wide) {
+    // M^H (qM x pM) = Q0 R0; M' = R0^H (pM x pM, row-major) has M's left singular vectors
+    f0 = qr_fact_layout(S.qM, S.pM, S.pM, wsb + S.fact0, 1);
+    if ((stt = qr_factor(f0, qin, ldq, st, 0))) return stt;
+    double2* T0 = (double2*)(wsb + S.t0);
+    k_rh<<<dim3(gs_blocks(S.pM * S.pM), 1), 256, 0, st>>>(S.qM, S.pM, S.pM, S.pM, 0, f0.W,
+                                                          f0.rdiag, T0, f0.bs.W, f0.bs.k);
+    PTN_LAUNCH_CHECK();
+    qin = T0;
+    ldq = S.pM;
+  }
+  double2* X = (double2*)(base + L.x);
+  double2* V = (double2*)(base + L.v);
+  double2* UB = (double2*)(base + L.ub);
+  double* nrm = (double*)(base + L.nrm);
+  int64_t* perm = (int64_t*)(base + L.perm);
+  int* counters = (int*)(base + L.cnt);
+  double* sdev = (double*)(base + L.vb);   // the singular values (V_B is not formed)
+  const int64_t sx = L.q * L.qp;
+  QrFact f = qr_fact_layout(L.p, L.q, L.q, base + L.fact, 1);
+  if ((stt = qr_factor(f, qin, ldq, st, 0))) return stt;
+  QrFact f2 = qr_fact_layout(L.q, L.q, L.q, base + L.fact2, 1);
+  double2* RH = (double2*)(base + L.rh);
+  k_rh<<<dim3(gs_blocks(L.q * L.q), 1), 256, 0, st>>>(L.p, L.q, L.q, L.q, 0, f.W, f.rdiag, RH,
+                                                      f.bs.W, f.bs.k);
+  PTN_LAUNCH_CHECK();
+  if ((stt = qr_factor(f2, RH, L.q, st, L.q * L.q))) return stt;
+  k_rh<<<dim3(gs_blocks(L.q * L.qp), 1), 256, 0, st>>>(L.q, L.q, L.q, L.qp, 1, f2.W, f2.rdiag, X,
+                                                       f2.bs.W, f2.bs.k);
+  PTN_LAUNCH_CHECK();
+  JacobiResult jr;
+  if ((stt = jacobi_core(L, base, X, V, false, st, jr))) return stt;
+  // the singular values carry the power-of-two input scales of the QR(s)
+  double* scl = f.scl;
+  if (S.wide) {
+    scl = (double*)(wsb + S.inner - 16);
+    k_mul_scale<<<1, 64, 0, st>>>(f.scl, f0.scl, scl);
+  }
+  k_colnorms<<<dim3((unsigned)L.q, 1), 256, 0, st>>>(L.q, X, nrm, sx, L.qp);
+  k_rank<<<dim3((unsigned)((L.q + 255) / 256), 1), 256, 0, st>>>(L.q, nrm, perm, sdev,
+                                                                  counters + 1, L.qp, L.q, scl);
+  PTN_LAUNCH_CHECK();
+  // the singular values and the non-finite flag on the host (the kept count
+  // sizes the outputs)
+  int* host_rot = pinned_counter();
+  if (!host_rot) return PTN_ERR_HIP;
+  PTN_HIP_CHECK(hipMemcpyAsync(s_host, sdev, (size_t)L.q * sizeof(double), hipMemcpyDeviceToHost,
+                               st));
+  PTN_HIP_CHECK(hipMemcpyAsync(host_rot, counters + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  PTN_HIP_CHECK(hipStreamSynchronize(st));
+  if (info_host) {
+    info_host[0] = jr.sweeps;
+    info_host[1] = jr.converged ? 0 : 1;
+    info_host[2] = *host_rot;
+    info_host[3] = jr.sweeps32;
+  }
+  if (*host_rot) { *k_out = 0; return PTN_ERR_NONFINITE; }
+  const int64_t k = kept_count(s_host, L.q, tol, max_bond);
+  *k_out = k;
+  if (k == 0) return PTN_OK;
+  // U_M(:, :k) = Q1 [U_X(:, perm[:k]); 0]   (pM x k column-major; f.scl does not enter U)
+  k_embed_u_perm<<<gs_blocks(L.p * k), 256, 0, st>>>(L.p, L.q, k, X, nrm, perm, UB);
+  PTN_LAUNCH_CHECK();
+  if ((stt = qr_apply_q(f, UB, L.p, k, false, st, 0))) return stt;
+  if (distr == PTN_SPLIT_RIGHT) {
+    // a0 = U_k (m x k row-major); a1 = U_k^H Theta (k x n)
+    k_colmajor_out<<<gs_blocks(m * k), 256, 0, st>>>(m, k, UB, (double2*)a0, 0);
+    PTN_LAUNCH_CHECK();
+    ZGemm g = {k, n, m, UB, m, 1, 1, theta, lda, 1, 0, a1, n, 1, 0, 1, 0, 0, 0};
+    return zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, st);
+  }
+  // "left": V_k = U_M(:, :k) (n x k column-major); a1 = V_k^H (k x n row-major:
+  // the conjugate of the column-major storage), a0 = Theta V_k (m x k)
+  k_conj_copy<<<gs_blocks(n * k), 256, 0, st>>>(n * k, UB, (double2*)a1);
+  PTN_LAUNCH_CHECK();
+  ZGemm g = {m, k, n, theta, lda, 1, 0, UB, 1, n, 0, a0, k, 1, 0, 1, 0, 0, 0};
+  return zgemm(g, PTN_FP64, PTN_T_C128, PTN_T_C128, PTN_T_C128, nullptr, 0, st);
 }
 
 }  // namespace ptn

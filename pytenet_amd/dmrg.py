@@ -82,6 +82,7 @@ def dmrg_twosite(hamiltonian: MPO, psi: MPS, numsweeps: int, numiter_lanczos: in
         for i in range(nsites - 2):
             a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
             en, a2 = ctx.site_eigh(a2, lblocks[i], h2[i], rblocks[i + 1])
+            ctx.flush()   # a failed local solve raises here, before its split
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "right", tol=tol_split,
                 max_bond=max_bond, svd_precond32=pre32)
@@ -89,6 +90,7 @@ def dmrg_twosite(hamiltonian: MPO, psi: MPS, numsweeps: int, numiter_lanczos: in
         for i in reversed(range(nsites - 1)):
             a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
             en, a2 = ctx.site_eigh(a2, lblocks[i], h2[i], rblocks[i + 1])
+            ctx.flush()   # a failed local solve raises here, before its split
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=tol_split,
                 max_bond=max_bond, svd_precond32=pre32)

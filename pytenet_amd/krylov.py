@@ -7,19 +7,20 @@ Two layers:
     the final combination run as gfx950 kernels with FP64 scalars in device
     memory, so a local solve needs no host round trip;
   * the functions below keep the reference's callback API
-    (``afunc(x) -> A x``) for arbitrary operators. Their vectors live on the
-    GPU; `afunc` receives and returns the kind of array the caller passed
-    (NumPy arrays are exchanged with the host around each call).
+    (``afunc(x) -> A x``) for arbitrary Hermitian operators. Their Krylov
+    vectors live on the GPU; `afunc` receives and returns the kind of array
+    the caller passed (NumPy arrays are exchanged with the host around each
+    call). Arnoldi (the non-Hermitian branch) is not on the sweep path and is
+    not provided.
 """
 
 import warnings
 import numpy as np
 import torch
-from scipy.linalg import expm as _dense_expm
 
 from . import backend
 
-__all__ = ["lanczos_iteration", "arnoldi_iteration", "eigh_krylov", "expm_krylov", "eigh_tridiag"]
+__all__ = ["lanczos_iteration", "eigh_krylov", "expm_krylov", "eigh_tridiag"]
 
 
 def _apply(afunc, x, host):
@@ -59,38 +60,6 @@ def lanczos_iteration(afunc, vstart, numiter):
     return alpha, beta, (backend.to_host(vt) if host else vt)
 
 
-def arnoldi_iteration(afunc, vstart, numiter):
-    """Matrix-free Arnoldi (krylov.py:60-107); returns (hess, v)."""
-    host = isinstance(vstart, np.ndarray) or not isinstance(vstart, torch.Tensor)
-    v0 = backend.as_dev(vstart).reshape(-1)
-    n = v0.numel()
-    nrmv = float(torch.linalg.vector_norm(v0).item())
-    assert nrmv > 0
-    hess = np.zeros((numiter, numiter), dtype=complex)
-    v = torch.zeros((numiter, n), dtype=torch.complex128, device=v0.device)
-    v[0] = v0 / nrmv
-    for j in range(numiter - 1):
-        w = _apply(afunc, v[j], host)
-        for k in range(j + 1):
-            hess[k, j] = complex(torch.vdot(v[k], w).item())
-            w = w - hess[k, j] * v[k]
-        hess[j + 1, j] = float(torch.linalg.vector_norm(w).item())
-        if abs(hess[j + 1, j]) < 100 * n * np.finfo(float).eps:
-            warnings.warn(f"H[{j+1}, {j}] ~= 0 encountered during Arnoldi iteration.",
-                          RuntimeWarning)
-            m = j + 1
-            vt = v[:m, :].T
-            return hess[:m, :m], (backend.to_host(vt) if host else vt)
-        v[j + 1] = w / hess[j + 1, j].real
-    j = numiter - 1
-    w = _apply(afunc, v[j], host)
-    for k in range(j + 1):
-        hess[k, j] = complex(torch.vdot(v[k], w).item())
-        w = w - hess[k, j] * v[k]
-    vt = v.T
-    return hess, (backend.to_host(vt) if host else vt)
-
-
 def eigh_tridiag(d, e):
     """Eigendecomposition of the small symmetric tridiagonal matrix (krylov.py:142-150).
     The sweep drivers use the device QL solver instead; this helper serves the
@@ -114,11 +83,13 @@ def eigh_krylov(afunc, vstart, numiter, numeig):
 
 
 def expm_krylov(afunc, vec, dt, numiter, hermitian=False):
-    """Krylov approximation of expm(dt*A) vec (krylov.py:122-139)."""
+    """Krylov approximation of expm(dt*A) vec for Hermitian A (krylov.py:122-136).
+    The Arnoldi branch of the reference (hermitian=False, krylov.py:137-139)
+    is not on the sweep path (the drivers always pass hermitian=True) and is
+    not provided."""
+    if not hermitian:
+        raise NotImplementedError("expm_krylov: only the Hermitian (Lanczos) branch is provided")
     nrm = float(np.linalg.norm(backend.to_host(vec)))
-    if hermitian:
-        alpha, beta, v = lanczos_iteration(afunc, vec, numiter)
-        w, u = eigh_tridiag(alpha, beta)
-        return _combine(v, u @ (nrm * np.exp(dt * w) * u[0]))
-    hess, v = arnoldi_iteration(afunc, vec, numiter)
-    return _combine(v, nrm * _dense_expm(dt * hess)[:, 0])
+    alpha, beta, v = lanczos_iteration(afunc, vec, numiter)
+    w, u = eigh_tridiag(alpha, beta)
+    return _combine(v, u @ (nrm * np.exp(dt * w) * u[0]))

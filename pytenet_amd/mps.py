@@ -258,6 +258,14 @@ def mps_split_tensor_svd(a, qsite0, qsite1, qbonds_outer, svd_distr: str, tol=0,
     b0, b2 = a.shape[0], a.shape[2]
     q0 = qnumber_flatten([qbonds_outer[0], qsite0])
     q1 = qnumber_flatten([-qsite1, qbonds_outer[1]])
+    if svd_distr != "sqrt" and not np.any(q0) and not np.any(q1):
+        # one sector: the truncating split computes only the isometric factor's
+        # singular vectors and projects onto them (same kept subspace and weights)
+        a0, a1, s = backend.svd_split(a.reshape(b0 * d0, d1 * b2), svd_distr, tol, max_bond,
+                                      svd_precond32)
+        k = a0.shape[1]
+        q = np.zeros(k, dtype=np.asarray(q0).dtype)
+        return (_like(a0.reshape(b0, d0, k), host), _like(a1.reshape(k, d1, b2), host), q)
     u, s, v, q = block_sparse_svd(a.reshape(b0 * d0, d1 * b2), q0, q1, svd_precond32)
     idx = cap_bond_indices(s, retained_bond_indices(s, tol), max_bond)
     k = len(idx)

@@ -101,6 +101,7 @@ class TwoSiteTDVP:
         for i in range(n - 2):
             a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
             a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -0.5 * dt)
+            ctx.flush()   # a failed local solve raises here, before its split
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, (psi.qbonds[i], psi.qbonds[i + 2]), "right", tol=self.tol,
                 max_bond=self.max_bond, svd_precond32=self.pre32)
@@ -110,6 +111,7 @@ class TwoSiteTDVP:
         i = n - 2
         a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
         a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -dt)
+        ctx.flush()   # a failed local solve raises here, before its split
         psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
             a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=self.tol,
             max_bond=self.max_bond, svd_precond32=self.pre32)
@@ -119,6 +121,7 @@ class TwoSiteTDVP:
                                          0.5 * dt)
             a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
             a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -0.5 * dt)
+            ctx.flush()   # a failed local solve raises here, before its split
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=self.tol,
                 max_bond=self.max_bond, svd_precond32=self.pre32)

@@ -39,3 +39,35 @@ def test_version_and_status_strings():
     assert lib.ptn_status_string(_lib.PTN_ERR_WORKSPACE) == b"workspace too small"
     # workspace queries are pure host arithmetic
     assert lib.ptn_heff_workspace(_lib.PTN_FP64, 16, 2, 16, 5, 5) >= 2 * 16 * 2 * 5 * 16 * 16
+
+
+def test_zero_width_bonds_are_valid_shapes():
+    """Zero-width virtual bonds (a split that keeps no singular value,
+    pytenet/bond_ops.py:28-29) are accepted by the contraction entry points;
+    with an empty output nothing is launched, so this runs without a GPU.
+    Negative sizes and mode mixes beyond d*M = 32 are rejected."""
+    lib = _lib.load()
+    nul = ctypes.c_void_p(0)
+    fp64 = _lib.PTN_FP64
+    ws_h = lib.ptn_heff_workspace(fp64, 0, 2, 6, 5, 5)
+    buf = ctypes.create_string_buffer(max(int(ws_h), 1))
+    ws = ctypes.cast(buf, ctypes.c_void_p)
+    for Dl, Dr in ((0, 6), (6, 0), (0, 0)):
+        n = lib.ptn_heff_workspace(fp64, Dl, 2, Dr, 5, 5)
+        b = ctypes.create_string_buffer(max(int(n), 1))
+        assert lib.ptn_heff_apply(fp64, Dl, 2, Dr, 5, 5, nul, nul, nul, nul, nul,
+                                  ctypes.cast(b, ctypes.c_void_p), n, nul) == _lib.PTN_OK
+        n = lib.ptn_bond_workspace(fp64, Dl, Dr, 5)
+        b = ctypes.create_string_buffer(max(int(n), 1))
+        assert lib.ptn_bond_apply(fp64, Dl, Dr, 5, nul, nul, nul, nul,
+                                  ctypes.cast(b, ctypes.c_void_p), n, nul) == _lib.PTN_OK
+    n = lib.ptn_env_workspace(fp64, 0, 2, 6, 5, 5)
+    b = ctypes.create_string_buffer(max(int(n), 1))
+    assert lib.ptn_env_right(fp64, 0, 2, 6, 5, 5, nul, nul, nul, nul, nul,
+                             ctypes.cast(b, ctypes.c_void_p), n, nul) == _lib.PTN_OK
+    assert lib.ptn_heff_apply(fp64, -1, 2, 6, 5, 5, nul, nul, nul, nul, nul, ws, ws_h,
+                              nul) == _lib.PTN_ERR_ARG
+    n = lib.ptn_heff_workspace(fp64, 4, 8, 4, 5, 5)
+    b = ctypes.create_string_buffer(max(int(n), 1))
+    assert lib.ptn_heff_apply(fp64, 4, 8, 4, 5, 5, nul, nul, nul, nul, nul,
+                              ctypes.cast(b, ctypes.c_void_p), n, nul) == _lib.PTN_ERR_ARG

@@ -526,3 +526,62 @@ def test_live_launch_timing_counts_products():
     backend.matmul(A, B)
     backend.timing_begin()
     assert backend.timing_end() == []
+
+
+def test_callback_krylov_api_matches_oracle():
+    """The callback Krylov API (lanczos_iteration / eigh_krylov / expm_krylov
+    of pytenet/krylov.py:12-57,110-136) with a user matvec: equal to the
+    oracle on the same operator, exact on the full space (numiter = n) against
+    scipy's expm / numpy's eigh, for NumPy and device inputs."""
+    from scipy.linalg import expm
+    rng = np.random.default_rng(17)
+    n = 12
+    a = orc.crandn((n, n), rng)
+    a = 0.5 * (a + a.conj().T)
+    v = orc.crandn(n, rng)
+    # Lanczos without reorthogonalisation amplifies rounding differences in the
+    # late vectors of a full-space run, hence the looser bound there
+    for numiter, vtol in ((5, 1e-12), (n, 1e-8)):
+        al, be, V = ptn.lanczos_iteration(lambda x: a @ x, v, numiter)
+        ral, rbe, rV = orc.lanczos_iteration(lambda x: a @ x, v, numiter)
+        assert np.allclose(al, ral, rtol=vtol, atol=vtol) and np.allclose(be, rbe, rtol=vtol)
+        assert np.abs(V - rV).max() < vtol
+    y = ptn.expm_krylov(lambda x: a @ x, v, -0.3j, n, hermitian=True)
+    assert np.abs(y - expm(-0.3j * a) @ v).max() < 1e-11
+    w, u = ptn.eigh_krylov(lambda x: a @ x, v, n, 1)
+    assert abs(w[0] - np.linalg.eigvalsh(a)[0]) < 1e-11
+    assert abs(abs(np.vdot(u[:, 0], np.linalg.eigh(a)[1][:, 0])) - 1) < 1e-10
+    ad = dev(a)
+    yd = ptn.expm_krylov(lambda x: ad @ x, dev(v), -0.3j, n, hermitian=True)
+    assert isinstance(yd, torch.Tensor) and np.abs(backend.to_host(yd) - expm(-0.3j * a) @ v).max() < 1e-11
+    with pytest.raises(NotImplementedError):
+        ptn.expm_krylov(lambda x: a @ x, v, 0.1, 4, hermitian=False)
+
+
+def test_zero_width_bond_after_a_zero_split_matches_reference_semantics():
+    """A split of an all-zero two-site tensor keeps no singular value
+    (pytenet/bond_ops.py:28-29): the new bond has width 0, the environment
+    step of the empty tensor gives the empty / zero blocks NumPy's tensordot
+    gives (as the oracle), and a local Krylov step on the empty tensor raises
+    AssertionError as the reference's expm_krylov does (krylov.py:30)."""
+    rng = np.random.default_rng(8)
+    qs = np.zeros(2, dtype=int)
+    theta = np.zeros((4, 4, 6), dtype=complex)
+    a0, a1, q = ptn.mps_split_tensor_svd(theta, qs, qs, (np.zeros(4, int), np.zeros(6, int)),
+                                          "right", tol=1e-10)
+    r0, r1, rq = orc.mps_split_tensor_svd(theta, qs, qs, (np.zeros(4, int), np.zeros(6, int)),
+                                          "right", tol=1e-10)
+    assert a0.shape == r0.shape == (4, 2, 0) and a1.shape == r1.shape == (0, 2, 6)
+    assert len(q) == len(rq) == 0
+    w = heisenberg_xxz_1d_mpo(4, 1.0, 0.8, -0.1).a[1]
+    l = orc.crandn((4, w.shape[0], 4), rng)
+    got = backend.env_left(a0, a0, w, l)
+    ref = orc.contraction_operator_step_left(r0, r0, w, l)
+    assert tuple(got.shape) == ref.shape == (0, w.shape[3], 0)
+    rr = orc.crandn((6, w.shape[3], 6), rng)
+    got = backend.env_right(a1, a1, w, rr)
+    ref = orc.contraction_operator_step_right(r1, r1, w, rr)
+    assert tuple(got.shape) == ref.shape and np.abs(backend.to_host(got)).max() == 0.0
+    with pytest.raises(AssertionError):
+        backend.krylov_local(_lib.PTN_OP_HEFF, a1, dev(np.zeros((0, w.shape[0], 0))), dev(w), dev(rr),
+                             4, "expm", 0.1j)
