@@ -139,6 +139,15 @@ typedef struct {
 
 size_t ptn_krylov_workspace(const ptn_local_op* op, int numiter);
 
+/* hipGraph capture of the local solves (no counterpart in the reference):
+ * with graphs on (the default), a solve whose vectors have at most 2^20
+ * elements (all problems of a batch together) is captured once per shape
+ * bucket and replayed as one graph launch with the new pointers; breakdowns
+ * are predicated on the device. on < 0 only queries. Returns the previous
+ * setting. ptn_graph_launches() counts the graph launches of the process. */
+int ptn_set_graphs(int on);
+int64_t ptn_graph_launches(void);
+
 /* expm(dt*H) v by `numiter` Lanczos steps, replaces pytenet/krylov.py:122-136
  * (Hermitian branch) with pytenet/krylov.py:12-57 and :142-150. dt is complex.
  * alpha_beta (device double[2*numiter], optional): alpha then beta. */

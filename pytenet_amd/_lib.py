@@ -101,6 +101,8 @@ SIGNATURES = [
     ("ptn_svd_split_workspace", c_sz, [c_i64, c_i64, c_int, c_int]),
     ("ptn_svd_split", c_int, [c_i64, c_i64, c_p, c_i64, c_int, c_dbl, c_i64, c_int, c_p, c_p,
                               c_p, ctypes.POINTER(c_i64), c_p, c_p, c_sz, c_p]),
+    ("ptn_set_graphs", c_int, [c_int]),
+    ("ptn_graph_launches", c_i64, []),
     ("ptn_timing_begin", c_int, []),
     ("ptn_timing_end", c_int, [ctypes.POINTER(KernelTiming), c_int, ctypes.POINTER(c_int)]),
 ]

@@ -316,6 +316,17 @@ def timing_end(max_rows=64):
     return out
 
 
+def set_graphs(on=True):
+    """Switch the hipGraph capture of short local solves on or off (process-wide);
+    returns the previous setting."""
+    return bool(_lib.load().ptn_set_graphs(int(bool(on))))
+
+
+def graph_launches():
+    """Number of captured-solve graph launches so far in this process."""
+    return int(_lib.load().ptn_graph_launches())
+
+
 def set_autotune(on=True):
     """Switch the tile autotuner of the complex products on or off (process-wide;
     the first product of each shape then times the tile shapes once). Returns

@@ -14,6 +14,8 @@ size_t svd_split_ws_bytes(int64_t m, int64_t n, int distr, bool precond32);
 int svd_split_run(int64_t m, int64_t n, const void* theta, int64_t lda, int distr, double tol,
                   int64_t max_bond, bool precond32, void* a0, void* a1, double* s_host,
                   int64_t* k_out, int32_t* info_host, void* ws, size_t ws_bytes, hipStream_t st);
+bool krylov_graphs(int on);
+int64_t krylov_graph_launches();
 size_t qr_ws_bytes(int64_t m, int64_t n, int64_t batch);
 int qr_run(int64_t m, int64_t n, const void* a, int64_t lda, void* q, int64_t ldq, void* r,
            int64_t ldr, void* ws, size_t ws_bytes, hipStream_t s, int64_t batch);
@@ -231,6 +233,10 @@ int ptn_env_right_batched(int precision, int64_t batch, int64_t Dl, int64_t d, i
   return env_step_b(0, precision, batch, Dl, d, Dr, Ml, Mr, a, b, w, r, out, ws, ws_bytes,
                     (hipStream_t)stream);
 }
+
+int ptn_set_graphs(int on) { return krylov_graphs(on) ? 1 : 0; }
+
+int64_t ptn_graph_launches(void) { return krylov_graph_launches(); }
 
 int ptn_set_autotune(int on) {
   const int prev = g_autotune ? 1 : 0;

@@ -504,7 +504,12 @@ inline int launch_typed(const ZGemm& z, hipStream_t s, void* mws = nullptr, size
   // outweighs its lower efficiency per flop (U(1) two-site DMRG at L=128,
   // D=2048: 76 s against 82 s per sweep)
   if (mws && z.batch == 1) return launch_cfg<Tc, TA, TB, TC, 2, 2>(z, s, mws, mws_bytes);
-  if (!z.accumulate && autotune_enabled()) return launch_tuned<Tc, TA, TB, TC>(z, s);
+  if (!z.accumulate && autotune_enabled()) {
+    // tuning synchronises the stream, which a stream capture does not allow
+    hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cst) == hipSuccess && cst == hipStreamCaptureStatusNone)
+      return launch_tuned<Tc, TA, TB, TC>(z, s);
+  }
   return launch_heuristic<Tc, TA, TB, TC>(z, s, mws, mws_bytes);
 }
 

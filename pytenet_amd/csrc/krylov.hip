@@ -9,6 +9,9 @@
 // round trip. A breakdown or a non-finite value stops the recurrence through a
 // flag that later kernels read; the host reads the status words afterwards.
 #include <float.h>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 #include "common.h"
 #include "ops.h"
 
@@ -426,9 +429,167 @@ size_t krylov_ws_bytes(const ptn_local_op* op, int numiter, int64_t batch) {
 // `batch` independent local problems of identical shape: l, w, r, v and out
 // stacked contiguously along a leading batch axis; info / energy /
 // alpha_beta per problem (4, 1, 2 numiter entries)
+static int krylov_enqueue(const ptn_local_op* op, int mode, const void* v, double dt_re,
+                          double dt_im, int numiter, void* out, double* energy, int32_t* info,
+                          double* alpha_beta, void* ws, size_t ws_bytes, hipStream_t s,
+                          int64_t batch);
+
+// ---------------------------------------------------------------------------
+// hipGraph capture of the local solves. A solve whose vectors are short is
+// bound by kernel-launch latency (numiter x ~7 launches); its whole launch
+// sequence (state init, every Lanczos iteration with its matvec, the
+// tridiagonal eigensolver, the combination) is captured into a graph on a
+// private stream and launched as one graph on the caller's stream. The
+// executable graphs are cached per shape bucket (operator kind and shapes,
+// precision, numiter, mode, batch, device) and updated in place with the
+// new call's pointers (hipGraphExecUpdate; the topology of one bucket never
+// changes). Iterations after a breakdown are predicated off on the device
+// (the "stopped" word), so one graph serves every outcome. Each bucket keeps a
+// ring of GRAPH_RING executables; an executable is updated only after its
+// previous launch has completed (an event per slot). Long vectors (MFMA-bound
+// matvecs of milliseconds) are launched directly.
+namespace {
+bool g_graphs_on = true;
+constexpr int64_t GRAPH_MAX_N = 1 << 20;   // vector length up to which solves are captured
+constexpr int GRAPH_RING = 3;
+constexpr size_t GRAPH_MAX_BUCKETS = 512;
+struct GKey {
+  int dev, kind, prec, mode, numiter;
+  int64_t Dl, d, Dr, Ml, Mr, batch;
+  bool operator==(const GKey& o) const {
+    return dev == o.dev && kind == o.kind && prec == o.prec && mode == o.mode &&
+           numiter == o.numiter && Dl == o.Dl && d == o.d && Dr == o.Dr && Ml == o.Ml &&
+           Mr == o.Mr && batch == o.batch;
+  }
+};
+struct GHash {
+  size_t operator()(const GKey& k) const {
+    size_t h = (size_t)k.dev * 1000003u ^ (size_t)k.kind * 10007u ^ (size_t)k.prec * 101u ^
+               (size_t)k.mode * 7u ^ (size_t)k.numiter;
+    for (int64_t x : {k.Dl, k.d, k.Dr, k.Ml, k.Mr, k.batch}) h = h * 1315423911u ^ (size_t)x;
+    return h;
+  }
+};
+struct GEntry {
+  hipGraphExec_t exec[GRAPH_RING] = {};
+  hipEvent_t done[GRAPH_RING] = {};
+  int next = 0;
+};
+std::mutex g_gmu;
+std::unordered_map<GKey, GEntry, GHash> g_graph_cache;
+std::map<int, hipStream_t> g_capture_streams;
+int64_t g_graph_launches = 0;
+
+void drop_entry(GEntry& e) {
+  for (int i = 0; i < GRAPH_RING; ++i) {
+    if (e.done[i]) { (void)hipEventSynchronize(e.done[i]); (void)hipEventDestroy(e.done[i]); }
+    if (e.exec[i]) (void)hipGraphExecDestroy(e.exec[i]);
+    e.exec[i] = nullptr;
+    e.done[i] = nullptr;
+  }
+}
+}  // namespace
+
+bool krylov_graphs(int on) {
+  std::lock_guard<std::mutex> lk(g_gmu);
+  const bool prev = g_graphs_on;
+  if (on >= 0) g_graphs_on = on != 0;
+  return prev;
+}
+
+int64_t krylov_graph_launches() {
+  std::lock_guard<std::mutex> lk(g_gmu);
+  return g_graph_launches;
+}
+
 int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, double dt_im,
                int numiter, void* out, double* energy, int32_t* info, double* alpha_beta,
                void* ws, size_t ws_bytes, hipStream_t s, int64_t batch) {
+  if (!op || numiter < 1 || numiter > LZ_MAX || batch < 1 || batch > 65535) return PTN_ERR_ARG;
+  const int64_t n = op->kind == PTN_OP_HEFF ? op->Dl * op->d * op->Dr : op->Dl * op->Dr;
+  bool use_graph;
+  {
+    std::lock_guard<std::mutex> lk(g_gmu);
+    use_graph = g_graphs_on && n * batch <= GRAPH_MAX_N;
+  }
+  hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+  if (use_graph && (hipStreamIsCapturing(s, &cst) != hipSuccess || cst != hipStreamCaptureStatusNone))
+    use_graph = false;   // the caller captures (or the query failed): launch into its capture
+  if (!use_graph)
+    return krylov_enqueue(op, mode, v, dt_re, dt_im, numiter, out, energy, info, alpha_beta, ws,
+                          ws_bytes, s, batch);
+  int dev = 0;
+  PTN_HIP_CHECK(hipGetDevice(&dev));
+  hipStream_t cap;
+  {
+    std::lock_guard<std::mutex> lk(g_gmu);
+    auto it = g_capture_streams.find(dev);
+    if (it == g_capture_streams.end()) {
+      hipStream_t ns;
+      PTN_HIP_CHECK(hipStreamCreateWithFlags(&ns, hipStreamNonBlocking));
+      it = g_capture_streams.emplace(dev, ns).first;
+    }
+    cap = it->second;
+  }
+  PTN_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+  const int st = krylov_enqueue(op, mode, v, dt_re, dt_im, numiter, out, energy, info, alpha_beta,
+                                ws, ws_bytes, cap, batch);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(cap, &g);
+  if (st) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (ec != hipSuccess) { ptn_set_last_hip_error(ec); return PTN_ERR_HIP; }
+  const GKey key{dev, op->kind, op->precision, mode, numiter,
+                 op->Dl, op->d, op->Dr, op->Ml, op->Mr, batch};
+  std::lock_guard<std::mutex> lk(g_gmu);
+  if (g_graph_cache.size() >= GRAPH_MAX_BUCKETS && !g_graph_cache.count(key)) {
+    for (auto& kv : g_graph_cache) drop_entry(kv.second);
+    g_graph_cache.clear();
+  }
+  GEntry& E = g_graph_cache[key];
+  const int slot = E.next;
+  E.next = (slot + 1) % GRAPH_RING;
+  hipError_t e = hipSuccess;
+  if (E.exec[slot]) {
+    // the slot's previous launch must be complete before its parameters change
+    if ((e = hipEventSynchronize(E.done[slot])) != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      ptn_set_last_hip_error(e);
+      return PTN_ERR_HIP;
+    }
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res;
+    if (hipGraphExecUpdate(E.exec[slot], g, &err_node, &res) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(E.exec[slot]);
+      E.exec[slot] = nullptr;
+    }
+  }
+  if (!E.exec[slot] && (e = hipGraphInstantiate(&E.exec[slot], g, nullptr, nullptr, 0)) != hipSuccess) {
+    E.exec[slot] = nullptr;
+    (void)hipGraphDestroy(g);
+    ptn_set_last_hip_error(e);
+    return PTN_ERR_HIP;
+  }
+  (void)hipGraphDestroy(g);
+  if (!E.done[slot] && (e = hipEventCreateWithFlags(&E.done[slot], hipEventDisableTiming)) != hipSuccess) {
+    ptn_set_last_hip_error(e);
+    return PTN_ERR_HIP;
+  }
+  if ((e = hipGraphLaunch(E.exec[slot], s)) != hipSuccess || (e = hipEventRecord(E.done[slot], s)) != hipSuccess) {
+    ptn_set_last_hip_error(e);
+    return PTN_ERR_HIP;
+  }
+  ++g_graph_launches;
+  return PTN_OK;
+}
+
+static int krylov_enqueue(const ptn_local_op* op, int mode, const void* v, double dt_re,
+                          double dt_im, int numiter, void* out, double* energy, int32_t* info,
+                          double* alpha_beta, void* ws, size_t ws_bytes, hipStream_t s,
+                          int64_t batch) {
   if (!op || numiter < 1 || numiter > LZ_MAX || batch < 1 || batch > 65535) return PTN_ERR_ARG;
   if (op->kind != PTN_OP_HEFF && op->kind != PTN_OP_BOND) return PTN_ERR_ARG;
   const int bprec = op->precision & ~PTN_BLOCK_SPARSE;

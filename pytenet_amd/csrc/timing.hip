@@ -32,6 +32,12 @@ constexpr size_t MAX_RECS = 1u << 18;
 
 int timing_pre(hipStream_t s) {
   if (!g_on) return -1;
+  // launches recorded into a graph (stream capture) are not timed per launch
+  hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cst) != hipSuccess || cst != hipStreamCaptureStatusNone) {
+    (void)hipGetLastError();
+    return -1;
+  }
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g_on) return -1;
   const size_t idx = g_recs.size();

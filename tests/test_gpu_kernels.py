@@ -581,7 +581,62 @@ def test_zero_width_bond_after_a_zero_split_matches_reference_semantics():
     rr = orc.crandn((6, w.shape[3], 6), rng)
     got = backend.env_right(a1, a1, w, rr)
     ref = orc.contraction_operator_step_right(r1, r1, w, rr)
-    assert tuple(got.shape) == ref.shape and np.abs(backend.to_host(got)).max() == 0.0
+    assert tuple(got.shape) == ref.shape == (0, w.shape[0], 0)
+    # an empty sum: the left step through the zero-width bond gives zeros
+    l0 = np.zeros((0, w.shape[0], 0), dtype=complex)
+    got = backend.env_left(a1, a1, w, l0)
+    ref = orc.contraction_operator_step_left(r1, r1, w, l0)
+    assert tuple(got.shape) == ref.shape == (6, w.shape[3], 6)
+    assert np.abs(backend.to_host(got)).max() == 0.0 and np.abs(ref).max() == 0.0
     with pytest.raises(AssertionError):
         backend.krylov_local(_lib.PTN_OP_HEFF, a1, dev(np.zeros((0, w.shape[0], 0))), dev(w), dev(rr),
                              4, "expm", 0.1j)
+
+
+def test_graph_captured_solves_equal_direct_launches():
+    """hipGraph capture of short local solves (ptn_set_graphs): a sequence of
+    solves of one shape bucket with different operands and start vectors,
+    interleaved with eager work and synchronisations, replays the cached graph
+    with updated pointers; every replay equals the directly launched solve bit
+    for bit, including a solve that breaks down (predicated iterations) and
+    eigh solves, and the oracle to 1e-12."""
+    prev = backend.set_graphs(True)
+    try:
+        n0 = backend.graph_launches()
+        results = []
+        for seed in range(6):
+            a, w, l, r = _heff_problem(100 + seed, Dl=10, d=4, Dr=12, M=3)
+            la, wa, ra, va = dev(l), dev(w), dev(r), dev(a)
+            g_out, _, g_info = backend.krylov_local(_lib.PTN_OP_HEFF, va, la, wa, ra, 6, "expm",
+                                                    -0.2j)
+            g_eig, g_en, _ = backend.krylov_local(_lib.PTN_OP_HEFF, va, la, wa, ra, 6, "eigh")
+            # eager work and a synchronisation between the replays
+            backend.heff_apply(va, wa, la, ra)
+            torch.cuda.synchronize()
+            results.append((seed, a, w, l, r, backend.to_host(g_out), backend.to_host(g_info),
+                            backend.to_host(g_eig), float(g_en.item())))
+        assert backend.graph_launches() - n0 == 12
+        # invariant-subspace breakdown through the same capture path
+        wb = np.zeros((1, 4, 4, 1), dtype=complex)
+        wb[0, :, :, 0] = np.diag([1.0, 2.0, 3.0, 4.0])
+        one = np.ones((1, 1, 1))
+        ab = np.array([1.0, 1.0, 0.0, 0.0], dtype=complex).reshape(1, 4, 1)
+        gb, _, gbi = backend.krylov_local(_lib.PTN_OP_HEFF, ab, one, wb, one, 4, "expm", -0.3j)
+        gb, gbi = backend.to_host(gb), backend.to_host(gbi)
+        backend.set_graphs(False)
+        n1 = backend.graph_launches()
+        for seed, a, w, l, r, out, info, eig, en in results:
+            d_out, _, d_info = backend.krylov_local(_lib.PTN_OP_HEFF, a, l, w, r, 6, "expm", -0.2j)
+            d_eig, d_en, _ = backend.krylov_local(_lib.PTN_OP_HEFF, a, l, w, r, 6, "eigh")
+            assert np.array_equal(out, backend.to_host(d_out)), seed
+            assert np.array_equal(info, backend.to_host(d_info)), seed
+            assert np.array_equal(eig, backend.to_host(d_eig)) and en == float(d_en.item()), seed
+            ref = orc.expm_krylov(lambda x: orc.apply_local_hamiltonian(x.reshape(a.shape), w, l, r)
+                                  .reshape(-1), a.reshape(-1), -0.2j, 6)
+            assert relerr(out.reshape(-1), ref) < 1e-12
+        db, _, dbi = backend.krylov_local(_lib.PTN_OP_HEFF, ab, one, wb, one, 4, "expm", -0.3j)
+        assert np.array_equal(gb, backend.to_host(db)) and np.array_equal(gbi, backend.to_host(dbi))
+        assert gbi[0] == 2 and gbi[1] == 2
+        assert backend.graph_launches() == n1
+    finally:
+        backend.set_graphs(prev)
