@@ -756,6 +756,21 @@ static int jacobi_core(const SvdLayout& L, char* base, double2*& X, double2* V, 
   int stt;
   int* host_rot = pinned_counter();
   if (!host_rot) return PTN_ERR_HIP;
+  // algorithmic work per round (for the live launch timing): Gram of every
+  // block pair over the q rows (8 flops per complex multiply-add), one
+  // rotation sweep of JP(JP-1)/2 rotations per pair (each updating two rows
+  // and two columns of the pair's Gram matrix and two columns of J: 12 JP
+  // complex multiply-adds), the pair's columns times J in X (and V); bytes in
+  // complex128 elements (the FP32 phase moves half): the pair's columns read
+  // (and written back by the apply), partial Gram matrices written / read
+  struct RoundWork { double gram_flop, gram_bytes, solve_flop, solve_bytes, apply_flop, apply_bytes; };
+  const double np_b = (double)npairs * (double)L.batch;
+  const RoundWork W = {np_b * 8.0 * (double)L.q * JP * JP,
+                       np_b * 16.0 * ((double)L.q * JP + (double)L.nseg * JP * JP),
+                       np_b * 8.0 * 12.0 * JP * (JP * (JP - 1) / 2),
+                       np_b * 16.0 * ((double)L.nseg + 1) * JP * JP,
+                       np_b * 8.0 * (double)L.q * JP * JP,
+                       np_b * 16.0 * 2.0 * (double)L.q * JP};
   k_set_ints<<<1, 64, 0, st>>>(counters, 2, 0);
   if (L.pre32) {
     float2* X32 = (float2*)(base + L.x32);
@@ -773,12 +788,18 @@ static int jacobi_core(const SvdLayout& L, char* base, double2*& X, double2* V, 
     while (res.sweeps32 < max_sweeps32) {
       k_set_ints<<<1, 64, 0, st>>>(counters, 1, 0);
       for (int r = 0; r < L.nblk - 1; ++r) {
+        int slot = timing_pre(st);
         k_jac_gram<float><<<dim3((unsigned)L.nseg, (unsigned)npairs, NB), 256, 0, st>>>(
             L.q, L.nblk, r, X32, Gp32, (int)L.nseg, sx);
+        timing_post(slot, st, "k_jac_gram<float>", W.gram_flop, W.gram_bytes / 2);
+        slot = timing_pre(st);
         k_jac_solve<float><<<dim3((unsigned)npairs, NB), 256, 0, st>>>(
             L.nblk, Gp32, (int)L.nseg, tol32, big32, Jg32, rflag, counters);
+        timing_post(slot, st, "k_jac_solve<float>", W.solve_flop, W.solve_bytes / 2);
+        slot = timing_pre(st);
         k_jac_apply<float><<<dim3((unsigned)aseg, (unsigned)npairs, 2 * NB), 256, 0, st>>>(
             L.q, L.qp, L.nblk, r, X32, V32, Jg32, rflag, 2);
+        timing_post(slot, st, "k_jac_apply<float>", 2 * W.apply_flop, W.apply_bytes);
       }
       PTN_LAUNCH_CHECK();
       PTN_HIP_CHECK(hipMemcpyAsync(host_rot, counters, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -826,12 +847,18 @@ static int jacobi_core(const SvdLayout& L, char* base, double2*& X, double2* V, 
   while (!res.converged && res.sweeps < max_sweeps) {
     k_set_ints<<<1, 64, 0, st>>>(counters, 1, 0);
     for (int r = 0; r < L.nblk - 1; ++r) {
+      int slot = timing_pre(st);
       k_jac_gram<double><<<dim3((unsigned)L.nseg, (unsigned)npairs, NB), 256, 0, st>>>(
           L.q, L.nblk, r, X, Gp, (int)L.nseg, sx);
+      timing_post(slot, st, "k_jac_gram<double>", W.gram_flop, W.gram_bytes);
+      slot = timing_pre(st);
       k_jac_solve<double><<<dim3((unsigned)npairs, NB), 256, 0, st>>>(L.nblk, Gp, (int)L.nseg, tol,
                                                                       big, Jg, rflag, counters);
+      timing_post(slot, st, "k_jac_solve<double>", W.solve_flop, W.solve_bytes);
+      slot = timing_pre(st);
       k_jac_apply<double><<<dim3((unsigned)aseg, (unsigned)npairs, nmat * NB), 256, 0, st>>>(
           L.q, L.qp, L.nblk, r, X, V, Jg, rflag, nmat);
+      timing_post(slot, st, "k_jac_apply<double>", nmat * W.apply_flop, nmat * W.apply_bytes);
     }
     PTN_LAUNCH_CHECK();
     PTN_HIP_CHECK(hipMemcpyAsync(host_rot, counters, sizeof(int), hipMemcpyDeviceToHost, st));
