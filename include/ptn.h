@@ -155,6 +155,22 @@ int ptn_krylov_expm(const ptn_local_op* op, const void* v, double dt_re, double 
                     int numiter, void* out, int32_t* info, double* alpha_beta,
                     void* ws, size_t ws_bytes, void* stream);
 
+/* Allocation-free variant of ptn_krylov_expm. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as ptn_krylov_expm without the workspace: the
+ * kernels use a fixed device scratch array of PTN_NOALLOC_SCRATCH_BYTES per
+ * GPU (held until the process ends); problems whose ptn_krylov_workspace()
+ * exceeds it are rejected with PTN_ERR_WORKSPACE. The call launches its
+ * kernels directly (no graph, no launch timing, no tile autotuning) and
+ * allocates nothing. Calls that overlap on one GPU share
+ * the scratch array and race; serialise them (one stream, or the caller's
+ * own ordering). */
+#define PTN_NOALLOC_SCRATCH_BYTES (64u << 20)
+int ptn_krylov_expm_noalloc(const ptn_local_op* op, const void* v, double dt_re, double dt_im,
+                            int numiter, void* out, int32_t* info, double* alpha_beta,
+                            void* stream);
+
 /* lowest Ritz pair, replaces pytenet/krylov.py:110-119 with numeig = 1:
  * out = Ritz vector, *energy (device double) = lowest Ritz value. */
 int ptn_krylov_eigh(const ptn_local_op* op, const void* v, int numiter, void* out,
@@ -266,13 +282,15 @@ size_t ptn_svd_workspace_precond32(int64_t batch, int64_t m, int64_t n);
 int ptn_svd_precond32(int64_t batch, int64_t m, int64_t n, const void* a, void* u, double* s,
                       void* vh, int32_t* info, void* ws, size_t ws_bytes, void* stream);
 
-/* ---- Live launch timing of the complex products (no counterpart in the
- * reference, which has no device code): between ptn_timing_begin() and
- * ptn_timing_end(), every complex-product launch is bracketed by two HIP
- * events on the stream it runs on. ptn_timing_end() waits for them and
- * writes, per kernel instance (largest total time first, at most max_out
- * rows), the launch count, the summed device time and the summed algorithmic
- * flops (8 per complex multiply-add) and bytes (each operand once). */
+/* ---- Live launch timing (no counterpart in the reference, which has no
+ * device code): between ptn_timing_begin() and ptn_timing_end(), every launch
+ * of a complex product and of the three Jacobi kernels of the SVD (Gram,
+ * rotation solve, apply) is bracketed by two HIP events on the stream it runs
+ * on (launches recorded into a graph are not timed). ptn_timing_end() waits
+ * for them and writes, per kernel instance (largest total time first, at most
+ * max_out rows), the launch count, the summed device time and the summed
+ * algorithmic flops (8 per complex multiply-add) and bytes (each operand
+ * once). */
 typedef struct {
   char name[160];    /* e.g. "zgemm_kernel<float, c64, c64, c128, 4, 8, true, true>" */
   int64_t launches;

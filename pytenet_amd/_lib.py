@@ -28,6 +28,8 @@ PTN_OP_BOND = 1
 PTN_SPLIT_LEFT = 0
 PTN_SPLIT_RIGHT = 1
 
+PTN_NOALLOC_SCRATCH_BYTES = 64 << 20
+
 c_i64 = ctypes.c_int64
 c_p = ctypes.c_void_p
 c_sz = ctypes.c_size_t
@@ -101,6 +103,8 @@ SIGNATURES = [
     ("ptn_svd_split_workspace", c_sz, [c_i64, c_i64, c_int, c_int]),
     ("ptn_svd_split", c_int, [c_i64, c_i64, c_p, c_i64, c_int, c_dbl, c_i64, c_int, c_p, c_p,
                               c_p, ctypes.POINTER(c_i64), c_p, c_p, c_sz, c_p]),
+    ("ptn_krylov_expm_noalloc", c_int, [ctypes.POINTER(LocalOp), c_p, c_dbl, c_dbl, c_int, c_p,
+                                        c_p, c_p, c_p]),
     ("ptn_set_graphs", c_int, [c_int]),
     ("ptn_graph_launches", c_i64, []),
     ("ptn_timing_begin", c_int, []),

@@ -14,6 +14,8 @@ size_t svd_split_ws_bytes(int64_t m, int64_t n, int distr, bool precond32);
 int svd_split_run(int64_t m, int64_t n, const void* theta, int64_t lda, int distr, double tol,
                   int64_t max_bond, bool precond32, void* a0, void* a1, double* s_host,
                   int64_t* k_out, int32_t* info_host, void* ws, size_t ws_bytes, hipStream_t st);
+int krylov_expm_noalloc(const ptn_local_op* op, const void* v, double dt_re, double dt_im,
+                        int numiter, void* out, int32_t* info, double* alpha_beta, hipStream_t s);
 bool krylov_graphs(int on);
 int64_t krylov_graph_launches();
 size_t qr_ws_bytes(int64_t m, int64_t n, int64_t batch);
@@ -28,6 +30,10 @@ static thread_local hipError_t g_last_hip = hipSuccess;
 static bool g_autotune = false;
 bool autotune_enabled() { return g_autotune; }
 void ptn_set_last_hip_error(hipError_t e) { g_last_hip = e; }
+bool& noalloc_mode() {
+  static thread_local bool f = false;
+  return f;
+}
 SkipFlag& skip_flag() {
   static thread_local SkipFlag f;
   return f;
@@ -267,6 +273,13 @@ int ptn_krylov_expm(const ptn_local_op* op, const void* v, double dt_re, double 
                     void* stream) {
   return krylov_run(op, 0, v, dt_re, dt_im, numiter, out, nullptr, info, alpha_beta, ws, ws_bytes,
                     (hipStream_t)stream, 1);
+}
+
+int ptn_krylov_expm_noalloc(const ptn_local_op* op, const void* v, double dt_re, double dt_im,
+                            int numiter, void* out, int32_t* info, double* alpha_beta,
+                            void* stream) {
+  return krylov_expm_noalloc(op, v, dt_re, dt_im, numiter, out, info, alpha_beta,
+                             (hipStream_t)stream);
 }
 
 int ptn_krylov_eigh(const ptn_local_op* op, const void* v, int numiter, void* out, double* energy,

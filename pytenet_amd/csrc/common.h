@@ -66,6 +66,15 @@ struct SkipGuard {
   ~SkipGuard() { skip_flag() = SkipFlag{}; }
 };
 
+// set (on the calling host thread) while an allocation-free entry point runs:
+// the launch timing and the tile autotuner (which create and wait on events)
+// stand aside
+bool& noalloc_mode();
+struct NoAllocGuard {
+  NoAllocGuard() { noalloc_mode() = true; }
+  ~NoAllocGuard() { noalloc_mode() = false; }
+};
+
 template <typename T> struct type_tag;
 template <> struct type_tag<double> { static constexpr const char* s = "double"; };
 template <> struct type_tag<float> { static constexpr const char* s = "float"; };

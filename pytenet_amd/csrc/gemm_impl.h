@@ -504,7 +504,7 @@ inline int launch_typed(const ZGemm& z, hipStream_t s, void* mws = nullptr, size
   // outweighs its lower efficiency per flop (U(1) two-site DMRG at L=128,
   // D=2048: 76 s against 82 s per sweep)
   if (mws && z.batch == 1) return launch_cfg<Tc, TA, TB, TC, 2, 2>(z, s, mws, mws_bytes);
-  if (!z.accumulate && autotune_enabled()) {
+  if (!z.accumulate && autotune_enabled() && !noalloc_mode()) {
     // tuning synchronises the stream, which a stream capture does not allow
     hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cst) == hipSuccess && cst == hipStreamCaptureStatusNone)

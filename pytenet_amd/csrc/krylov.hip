@@ -669,4 +669,52 @@ static int krylov_enqueue(const ptn_local_op* op, int mode, const void* v, doubl
   return PTN_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free local solve (ptn_krylov_expm_noalloc): the same kernels as
+// ptn_krylov_expm, launched directly (no graph), with the workspace in a
+// file-scope device array instead of caller or library allocations. Each GPU
+// holds its own copy of the array (for the life of the process); its address
+// on a device is taken once with hipGetSymbolAddress while that device is
+// current and kept per device: the array's name in host code is only a
+// host-side stand-in and is never passed to a kernel. Nothing in the
+// workspace needs to start at zero (the state block is initialised by the
+// first kernel of every solve, every partial sum is written before it is
+// read). Calls that overlap on one GPU share the array and race.
+__device__ __align__(256) unsigned char scratch[PTN_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;
+}  // namespace
+
+static int scratch_address(void** out) {
+  int dev = 0;
+  PTN_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      ptn_set_last_hip_error(e != hipSuccess ? e : hipErrorInvalidSymbol);
+      return PTN_ERR_HIP;
+    }
+    it = g_scratch_ptr.emplace(dev, p).first;
+  }
+  *out = it->second;
+  return PTN_OK;
+}
+
+int krylov_expm_noalloc(const ptn_local_op* op, const void* v, double dt_re, double dt_im,
+                        int numiter, void* out, int32_t* info, double* alpha_beta, hipStream_t s) {
+  if (!op || numiter < 1 || numiter > LZ_MAX) return PTN_ERR_ARG;
+  if (krylov_ws_bytes(op, numiter, 1) > (size_t)PTN_NOALLOC_SCRATCH_BYTES) return PTN_ERR_WORKSPACE;
+  void* ws = nullptr;
+  int st = scratch_address(&ws);
+  if (st) return st;
+  NoAllocGuard guard;
+  return krylov_enqueue(op, 0, v, dt_re, dt_im, numiter, out, nullptr, info, alpha_beta, ws,
+                        (size_t)PTN_NOALLOC_SCRATCH_BYTES, s, 1);
+}
+
 }  // namespace ptn

@@ -31,7 +31,7 @@ constexpr size_t MAX_RECS = 1u << 18;
 }  // namespace
 
 int timing_pre(hipStream_t s) {
-  if (!g_on) return -1;
+  if (!g_on || noalloc_mode()) return -1;
   // launches recorded into a graph (stream capture) are not timed per launch
   hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &cst) != hipSuccess || cst != hipStreamCaptureStatusNone) {
