@@ -405,10 +405,13 @@ def workload_dmrg2_u1(args):
 
 def workload_chains(args):
     """256 transverse-field Ising chains (L=32, J=1, h=0, g uniform in
-    [0.5, 1.5], seeded), 64 each at D = 32, 64, 128, 256 (seeded random MPS),
-    imaginary-time single-site TDVP dt=0.05, 20 steps, numiter 10, evolved
-    together by tdvp_singlesite_batched in FP32 and in FP64; the oracle is
-    timed on one chain per D for one step."""
+    [0.5, 1.5], seeded), each starting from a random product state padded to
+    its own bond dimension D_i drawn uniformly from {32, 64, 128, 256}
+    (seeded), imaginary-time single-site TDVP dt=0.05, 20 steps, numiter 10,
+    evolved together by tdvp_singlesite_batched (buckets of equal D on their
+    own streams, advancing one site step at a time together; tile autotuner
+    per bucket) with complex64 contractions and FP64 Lanczos, and in all-FP64;
+    the oracle is timed on one chain per D for one step."""
     import copy as cp
     import torch
     import pytenet_amd as ptn
@@ -420,19 +423,20 @@ def workload_chains(args):
     for k in range(nch):
         g = rng.uniform(0.5, 1.5)
         H = ptn.ising_1d_mpo(L, 1.0, 0.0, g)
-        D = Ds[k % len(Ds)]
-        bonds = [min(D, 2**min(i, 40), 2**min(L - i, 40)) for i in range(L + 1)]
-        psi = ptn.MPS(H.qsite, [np.zeros(b, dtype=int) for b in bonds], fill="random", rng=rng)
+        psi = ptn.MPS.padded_product_state(H.qsite, L, int(rng.choice(Ds)), rng)
         psi.to_device()
         Hs.append(H)
         psis.append(psi)
-    out = {"workload": f"{nch} Ising chains L={L}, D in {Ds}, single-site TDVP dt=0.05, "
-                       f"{steps} steps, numiter {nit}, batched", "unit": "site updates/s"}
+    counts = {D: sum(1 for p in psis if max(p.bond_dims) == D) for D in Ds}
+    out = {"workload": f"{nch} Ising chains L={L}, padded product states, D_i uniform in {Ds} "
+                       f"(counts {counts}), single-site TDVP dt=0.05, {steps} steps, numiter {nit}, "
+                       "batched", "unit": "site updates/s"}
     upd = nch * steps * 2 * (L - 1)
     ens = {}
     for prec in ("fp32", "fp64"):
         ps = [cp.deepcopy(p) for p in psis]
         torch.cuda.synchronize()
+        g0 = ptn.backend.graph_launches()
         t0 = time.perf_counter()
         ptn.tdvp_singlesite_batched(Hs, ps, 0.05, steps, numiter_lanczos=nit, precision=prec)
         torch.cuda.synchronize()
@@ -441,13 +445,21 @@ def workload_chains(args):
                               for p, H in zip(ps, Hs)])
         out[f"{prec}_s"] = round(dt, 2)
         out[f"{prec}_site_updates_per_s"] = round(upd / dt, 1)
-        print(f"[bench] chains {prec} done", file=sys.stderr, flush=True)
+        out[f"{prec}_graph_launches"] = ptn.backend.graph_launches() - g0
+        print(f"[bench] chains {prec} done ({dt:.1f} s)", file=sys.stderr, flush=True)
     err = np.abs(ens["fp32"] - ens["fp64"]) / np.abs(ens["fp64"])
     out["value"] = out["fp32_site_updates_per_s"]
     out["energy_rel_err_fp32_vs_fp64_max"] = float(err.max())
     out["energy_rel_err_fp32_vs_fp64_median"] = float(np.median(err))
+    out["energy_rel_err_fp32_vs_fp64_quantiles"] = [float(np.quantile(err, q))
+                                                     for q in (0.5, 0.9, 0.99)]
+    worst = int(np.argmax(err))
+    out["worst_chain"] = {"D": int(max(psis[worst].bond_dims)),
+                          "energy_fp64": float(ens["fp64"][worst]),
+                          "energy_fp32": float(ens["fp32"][worst])}
     t_cpu = 0.0
-    for k, D in enumerate(Ds):
+    for D in Ds:
+        k = next(k for k in range(nch) if max(psis[k].bond_dims) == D)
         p = orc.MPS(psis[k].qsite, psis[k].qbonds)
         p.a = [a.cpu().numpy() for a in psis[k].a]
         t0 = time.perf_counter()

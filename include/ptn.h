@@ -62,7 +62,9 @@ extern "C" {
 
 /* Krylov status words (int32[4] in device memory):
  *   [0] number of Lanczos vectors used (numiter, or j+1 after a breakdown at step j)
- *   [1] j+1 if beta[j] < 100*n*eps stopped the iteration at step j, else 0
+ *   [1] j+1 if beta[j] < 100*n*eps stopped the iteration at step j, else 0; with
+ *       PTN_FP32 / PTN_BF16X3 matvecs also if beta[j] < rel * |A v_j| (rel = 128 eps_fp32
+ *       / 2^-12): a residual below the matvec's rounding noise is treated as a breakdown
  *   [2] bit 0: a norm or inner product was not finite;
  *       bit 1: the tridiagonal QL iteration did not converge (the result is unusable)
  *   [3] 1 if the start vector has norm zero (the reference asserts, krylov.py:30), else 0

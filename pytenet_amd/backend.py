@@ -57,20 +57,23 @@ def ptr(t):
 
 
 class _Workspace:
-    """Grow-only scratch buffer per device, reused across calls (stream-ordered)."""
+    """Grow-only scratch buffer per (device, stream), reused across calls: calls
+    on one stream are ordered, calls on different streams (the concurrent
+    buckets of the many-chain driver) get separate buffers."""
 
     def __init__(self):
         self.buf = {}
 
     def get(self, nbytes):
         dev = torch.cuda.current_device()
-        cur = self.buf.get(dev)
+        key = (dev, torch.cuda.current_stream().cuda_stream)
+        cur = self.buf.get(key)
         if cur is None or cur.numel() < nbytes:
             nb = max(int(nbytes), 1 << 20)
             if cur is not None:
                 nb = max(nb, int(cur.numel() * 1.25))
             cur = torch.empty(nb, dtype=torch.uint8, device=torch.device("cuda", dev))
-            self.buf[dev] = cur
+            self.buf[key] = cur
         return cur
 
 

@@ -177,12 +177,18 @@ __global__ __launch_bounds__(256) void k_alpha_update(int64_t n, double2* __rest
   block_store_partial(s, part_out);
 }
 
-// beta_j = |w|; breakdown test; v_{j+1} = w / beta_j
+// beta_j = |w|; breakdown test; v_{j+1} = w / beta_j. The test is the
+// reference's absolute one (beta_j < 100 n eps) and, for matvecs in reduced
+// precision (rel > 0), also beta_j < rel |A v_j| with |A v_j| =
+// sqrt(alpha_j^2 + beta_{j-1}^2 + beta_j^2): a residual below the rounding
+// noise of a complex64 / split-bf16 matvec is that noise, not a new Krylov
+// direction (a start vector in an invariant subspace otherwise continues with
+// normalised noise).
 __global__ __launch_bounds__(256) void k_beta_next(int64_t n, const double2* __restrict__ w,
                                                    int64_t ws_, double2* __restrict__ vnext,
                                                    int64_t vs, int j,
                                                    const double* __restrict__ part_in, int rb,
-                                                   double thresh, LzState* st) {
+                                                   double thresh, double rel, LzState* st) {
   st += blockIdx.y;
   if (st->stopped) return;
   w += blockIdx.y * ws_;
@@ -190,7 +196,9 @@ __global__ __launch_bounds__(256) void k_beta_next(int64_t n, const double2* __r
   part_in += blockIdx.y * rb;
   const double b = sqrt(block_sum_partials(part_in, rb));
   const bool nf = !isfinite(b);
-  const bool brk = !nf && (b < thresh);
+  const double a = st->alpha[j], bm = j > 0 ? st->beta[j - 1] : 0.0;
+  const double thr = fmax(thresh, rel * sqrt(a * a + bm * bm + b * b));
+  const bool brk = !nf && (b < thr);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     if (nf) { st->nonfinite = 1; st->stopped = 1; st->nvalid = j + 1; }
     else if (brk) { st->breakdown = j + 1; st->stopped = 1; st->nvalid = j + 1; }
@@ -611,6 +619,8 @@ static int krylov_enqueue(const ptn_local_op* op, int mode, const void* v, doubl
   const int rb = red_width(n);
   const int64_t js = batch * n;   // stride between V_j and V_{j+1}
   const double thresh = 100.0 * (double)n * DBL_EPSILON;
+  // relative breakdown level of reduced-precision matvecs (0: FP64, the reference's rule)
+  const double rel = bprec == PTN_FP32 ? 128.0 * FLT_EPSILON : (bprec == PTN_BF16X3 ? 1.0 / 4096 : 0.0);
 
   // environment blocks in the matvec's operand precision, converted once per solve
   const void* L = op->l;
@@ -652,7 +662,7 @@ static int krylov_enqueue(const ptn_local_op* op, int mode, const void* v, doubl
     k_alpha_update<<<dim3(rb, B), 256, 0, s>>>(n, w, n, vj, j > 0 ? vj - js : vj, n, j, pa, pb,
                                                 rb, st);
     k_beta_next<<<dim3((unsigned)gblocks, B), 256, 0, s>>>(n, w, n, vj + js, n, j, pb, rb,
-                                                            thresh, st);
+                                                            thresh, rel, st);
     PTN_LAUNCH_CHECK();
   }
   {

@@ -97,6 +97,27 @@ class MPS:
         fill = "random" if dtype in (complex, "complex") else "random real"
         return cls(qsite, qbonds, fill=fill, rng=rng)
 
+    @classmethod
+    def padded_product_state(cls, qsite, nsites: int, max_vdim: int,
+                             rng: np.random.Generator = None):
+        """Random product state embedded in an MPS of bond dimensions
+        min(max_vdim, d^i, d^(L-i)) (zero quantum numbers): site i holds one
+        complex normal vector of length d, normalised, at a[i][0, :, 0]; every
+        other entry is zero. The single-site sweeps of such a state reach the
+        whole bond dimension through the QR completion of its rank-deficient
+        tensors. Draws one complex vector per site in site order."""
+        qsite = np.asarray(qsite)
+        d = len(qsite)
+        if rng is None:
+            rng = np.random.default_rng()
+        bonds = [min(max_vdim, d**min(i, nsites - i, 40)) for i in range(nsites + 1)]
+        psi = cls(np.zeros_like(qsite), [np.zeros(b, dtype=int) for b in bonds], fill=0.0)
+        for i in range(nsites):
+            v = crandn(d, rng)
+            psi.a[i] = np.zeros((bonds[i], d, bonds[i + 1]), dtype=complex)
+            psi.a[i][0, :, 0] = v / np.linalg.norm(v)
+        return psi
+
     @property
     def nsites(self) -> int:
         return len(self.a)

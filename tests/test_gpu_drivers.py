@@ -14,6 +14,7 @@ import pytest
 import oracle as orc
 from conftest import load_fixture, mps_arrays, mpo_arrays
 import pytenet_amd as ptn
+from pytenet_amd import backend
 
 pytestmark = pytest.mark.gpu
 
@@ -212,6 +213,7 @@ def test_tdvp_checkpoint_resume(tmp_path):
     bonds = [min(8, 2**i, 2**(L - i)) for i in range(L + 1)]
     psi = ptn.MPS(H.qsite, [np.zeros(b, dtype=int) for b in bonds], fill="random", rng=rng)
     psi.orthonormalize("left")
+    psi0 = copy.deepcopy(psi)
     ref = copy.deepcopy(psi)
     ptn.tdvp_twosite(H, ref, -0.05j, 2, numiter_lanczos=8, tol_split=0)
     run = ptn.TwoSiteTDVP(H, psi, -0.05j, 8, 0)
@@ -227,3 +229,12 @@ def test_tdvp_checkpoint_resume(tmp_path):
     got = run2.psi
     f = abs(ptn.mps_vdot(ref, got))**2 / (ptn.mps_norm(ref)**2 * ptn.mps_norm(got)**2)
     assert abs(1 - f) < 1e-12
+    # and the resumed GPU run against the oracle's uninterrupted two steps
+    orc_psi = orc.MPS(psi0.qsite, [q.copy() for q in psi0.qbonds], fill="postpone")
+    orc_psi.a = [backend.to_host(a).copy() for a in psi0.a]
+    orc.tdvp_twosite(orc.MPO(H.qsite, H.qbonds, H.a), orc_psi, -0.05j, 2, numiter_lanczos=8,
+                     tol_split=0)
+    x, y = got.to_vector(), orc_psi.to_vector()
+    x = backend.to_host(x)
+    y = y * np.linalg.norm(x) / np.linalg.norm(y)    # the resumed run restarts from norm one
+    assert np.abs(_phase_aligned(x, y) - y).max() < 1e-10 * np.linalg.norm(y)

@@ -640,3 +640,29 @@ def test_graph_captured_solves_equal_direct_launches():
         assert backend.graph_launches() == n1
     finally:
         backend.set_graphs(prev)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_low_precision_lanczos_breaks_down_on_invariant_subspace(precision):
+    """A start vector in a 2-dimensional invariant subspace of a block-diagonal
+    H_eff: in FP64 the residual after two steps is at rounding level and the
+    reference's rule stops the iteration; with complex64 / split-bf16 matvecs
+    the residual is the matvec's rounding noise, which the precision-aware
+    rule also treats as a breakdown (instead of continuing with normalised
+    noise), so the result stays within the mode's accuracy of the FP64 one."""
+    rng = np.random.default_rng(21)
+    blk = [orc.crandn((2, 2), rng) for _ in range(2)]
+    blk = [0.5 * (b + b.conj().T) for b in blk]
+    w = np.zeros((1, 4, 4, 1), dtype=complex)
+    w[0, :2, :2, 0], w[0, 2:, 2:, 0] = blk
+    one = np.ones((1, 1, 1))
+    a = np.zeros((1, 4, 1), dtype=complex)
+    a[0, :2, 0] = orc.crandn(2, rng)
+    o64, _, i64 = backend.krylov_local(_lib.PTN_OP_HEFF, a, one, w, one, 4, "expm", -0.7j)
+    olp, _, ilp = backend.krylov_local(_lib.PTN_OP_HEFF, a, one, w, one, 4, "expm", -0.7j,
+                                       precision)
+    i64, ilp = backend.to_host(i64), backend.to_host(ilp)
+    assert i64[0] == 2 and i64[1] == 2
+    assert ilp[0] == 2 and ilp[1] == 2
+    tol = 1e-6 if precision == "fp32" else 1e-3
+    assert relerr(olp, backend.to_host(o64)) < tol

@@ -65,3 +65,30 @@ def test_krylov_status_words_map_to_the_reference_errors():
     # every row of a batch is checked
     with pytest.raises(np.linalg.LinAlgError):
         backend.report_krylov_status(np.array([[8, 0, 0, 0], [8, 0, 2, 0]], dtype=np.int32))
+
+
+def test_padded_product_state_is_the_product_of_its_site_vectors():
+    """MPS.padded_product_state: tapered bonds min(D, d^i, d^(L-i)), zeros
+    outside a[i][0, :, 0], unit site vectors, and a state vector equal to the
+    Kronecker product of the site vectors (computed on the host)."""
+    import pytenet_amd as ptn
+    psi = ptn.MPS.padded_product_state([0, 0], 7, 4, np.random.default_rng(3))
+    assert psi.bond_dims == [1, 2, 4, 4, 4, 4, 2, 1]
+    vecs = []
+    for a in psi.a:
+        v = a[0, :, 0].copy()
+        assert abs(np.linalg.norm(v) - 1) < 1e-15
+        b = a.copy()
+        b[0, :, 0] = 0
+        assert not np.any(b)
+        vecs.append(v)
+    full = vecs[0]
+    for v in vecs[1:]:
+        full = np.kron(full, v)
+    t = psi.a[0]
+    for a in psi.a[1:]:
+        t = np.tensordot(t, a, axes=(-1, 0))
+    assert np.allclose(t.reshape(-1), full, atol=1e-15)
+    # seeded: the same draws give the same state
+    again = ptn.MPS.padded_product_state([0, 0], 7, 4, np.random.default_rng(3))
+    assert all(np.array_equal(x, y) for x, y in zip(psi.a, again.a))
