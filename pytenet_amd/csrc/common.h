@@ -93,7 +93,13 @@ struct ZGemm {
   int accumulate;      // C += op(A) op(B) when 1
   int64_t batch;       // number of independent problems
   int64_t bz_a, bz_b, bz_c;
+  // optional scratch for the split-bf16 operand planes of large products
+  // (bf16x3_split_bytes); without it the split happens in the product kernel
+  void* sws = nullptr;
+  size_t sws_bytes = 0;
 };
+// scratch bytes of the pre-split split-bf16 product of an M x N x K (batched) product
+size_t bf16x3_split_bytes(int64_t M, int64_t N, int64_t K, int64_t batch);
 
 // element type codes for the GEMM / mix operands
 enum { PTN_T_C128 = 0, PTN_T_C64 = 1 };

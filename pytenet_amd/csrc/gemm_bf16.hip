@@ -17,6 +17,8 @@
 // complex column of B, expanded on the fly: real column 2n takes
 // (Re B, -Im B), real column 2n+1 takes (Im B, Re B) per complex k. The 16x16
 // FP32 result has row 4*(lane/16) + v and column lane % 16.
+#include <algorithm>
+#include <string>
 #include "gemm_impl.h"
 
 namespace ptn {
@@ -240,11 +242,365 @@ inline int launch_cfg_bf16(const ZGemm& z, hipStream_t s, void* mws, size_t mws_
   return PTN_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Pre-split form for large products. The two operands are split once, in a
+// pass of their own, into bf16 hi / lo planes of the real embedding:
+//   A_r (Mp x K2p, row-major): A_r[m][2k] = Re A(m, k), A_r[m][2k+1] = Im A(m, k)
+//   B_e (N2p x K2p, row-major, i.e. K-contiguous columns of the real B):
+//     B_e[2n][2k] = Re B(k, n),   B_e[2n][2k+1] = -Im B(k, n)
+//     B_e[2n+1][2k] = Im B(k, n), B_e[2n+1][2k+1] = Re B(k, n)
+// (conjugation flips the signs of the Im terms), zero-padded to the tile
+// multiples (Mp, N2p: 256; K2p: 32), so that C_r = A_r B_e^T gives
+// C_r[m][2n] = Re C(m, n) and C_r[m][2n+1] = Im C(m, n). The product itself
+// is then a plain bf16 GEMM with FP32 accumulation and three MFMAs per pair of
+// operand fragments (hi hi + hi lo + lo hi): v_mfma_f32_32x32x16_bf16, 256 x
+// 256 real tiles, 8 waves (2 x 4) of 128 x 64, 32-deep k-stages double
+// buffered in LDS (4 planes x 256 rows x 40 bf16 per stage; the 80-byte row
+// pitch keeps the 16-byte fragment reads of a wave on distinct banks).
+constexpr int V2_BM = 256, V2_BN = 256, V2_BK = 32, V2_LD = 40;
+
+inline int64_t v2_round(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// k-chunks of a single product whose tiles alone do not fill the chip (each
+// workgroup of the product holds a whole CU): 1, 2 or 4, keeping at least
+// 256 real k per chunk
+inline int v2_splitk(int64_t M, int64_t N, int64_t K, int64_t batch) {
+  if (batch != 1) return 1;
+  const int64_t tiles = ((M + V2_BM - 1) / V2_BM) * ((2 * N + V2_BN - 1) / V2_BN);
+  const int64_t K2p = v2_round(2 * K, V2_BK);
+  int sk = 1;
+  while (sk < 4 && tiles * sk < 256 && K2p / (2 * sk) >= 256) sk *= 2;
+  return sk;
+}
+
+// bytes of the split planes of one product (per batch problem: times batch),
+// plus the FP32 partial sums of a k-chunked product
+inline size_t v2_split_bytes(int64_t M, int64_t N, int64_t K, int64_t batch) {
+  const int64_t Mp = v2_round(M, V2_BM), N2p = v2_round(2 * N, V2_BN), K2p = v2_round(2 * K, V2_BK);
+  const int sk = v2_splitk(M, N, K, batch);
+  const size_t part = sk > 1 ? (size_t)sk * (size_t)Mp * (size_t)N2p * sizeof(float) + 256 : 0;
+  return (size_t)batch * (size_t)(Mp + N2p) * (size_t)K2p * 2 * sizeof(__bf16) + 512 + part;
+}
+
+__device__ __forceinline__ void split1(float x, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
+// A (M x K complex, element (m, k) at A[m a_m + k a_k], conjugated if conj)
+// -> A_r hi / lo planes. One workgroup per 64 x 32 complex block, staged
+// through LDS so that both the reads (along the operand's contiguous index)
+// and the writes (along k) are coalesced; problem blockIdx.z at A + z bz.
+template <typename TA>
+__global__ __launch_bounds__(256) void k_split_a(int64_t M, int64_t K, const TA* __restrict__ A,
+                                                 int64_t a_m, int64_t a_k, int conj, int64_t bz,
+                                                 int64_t K2p, int64_t pz, __bf16* __restrict__ hi,
+                                                 __bf16* __restrict__ lo) {
+  __shared__ float2 t[64][33];
+  const int64_t m0 = (int64_t)blockIdx.y * 64, k0 = (int64_t)blockIdx.x * 32;
+  A += blockIdx.z * bz;
+  hi += blockIdx.z * pz;
+  lo += blockIdx.z * pz;
+  const bool kc = (a_k == 1) || (a_m != 1);
+  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+    const int r = kc ? e / 32 : e % 64, c = kc ? e % 32 : e / 64;
+    const int64_t m = m0 + r, k = k0 + c;
+    float re = 0.f, im = 0.f;
+    if (m < M && k < K) load_c<float>(A + m * a_m + k * a_k, re, im);
+    t[r][c] = make_float2(re, conj ? -im : im);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+    const int r = e / 32, c = e % 32;
+    const int64_t k2 = 2 * (k0 + c);
+    if (k2 >= K2p) continue;
+    const float2 v = t[r][c];
+    __bf16 h0, l0, h1, l1;
+    split1(v.x, h0, l0);
+    split1(v.y, h1, l1);
+    const int64_t o = (m0 + r) * K2p + k2;
+    hi[o] = h0; hi[o + 1] = h1;
+    lo[o] = l0; lo[o + 1] = l1;
+  }
+}
+
+// B (K x N complex, element (k, n) at B[k b_k + n b_n]) -> B_e hi / lo planes
+template <typename TB>
+__global__ __launch_bounds__(256) void k_split_b(int64_t K, int64_t N, const TB* __restrict__ B,
+                                                 int64_t b_k, int64_t b_n, int conj, int64_t bz,
+                                                 int64_t K2p, int64_t pz, __bf16* __restrict__ hi,
+                                                 __bf16* __restrict__ lo) {
+  __shared__ float2 t[64][33];
+  const int64_t n0 = (int64_t)blockIdx.y * 64, k0 = (int64_t)blockIdx.x * 32;
+  B += blockIdx.z * bz;
+  hi += blockIdx.z * pz;
+  lo += blockIdx.z * pz;
+  const bool kc = (b_k == 1) || (b_n != 1);
+  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+    const int r = kc ? e / 32 : e % 64, c = kc ? e % 32 : e / 64;
+    const int64_t n = n0 + r, k = k0 + c;
+    float re = 0.f, im = 0.f;
+    if (n < N && k < K) load_c<float>(B + k * b_k + n * b_n, re, im);
+    t[r][c] = make_float2(re, conj ? -im : im);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+    const int r = e / 32, c = e % 32;
+    const int64_t k2 = 2 * (k0 + c);
+    if (k2 >= K2p) continue;
+    const float2 v = t[r][c];
+    __bf16 hr, lr, hi_, li_, hmi, lmi;
+    split1(v.x, hr, lr);
+    split1(v.y, hi_, li_);
+    split1(-v.y, hmi, lmi);
+    const int64_t o0 = (2 * (n0 + r)) * K2p + k2, o1 = o0 + K2p;
+    hi[o0] = hr; hi[o0 + 1] = hmi; hi[o1] = hi_; hi[o1 + 1] = hr;
+    lo[o0] = lr; lo[o0 + 1] = lmi; lo[o1] = li_; lo[o1 + 1] = lr;
+  }
+}
+
+typedef float f16v_t __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+struct V2Args {
+  const __bf16 *ahi, *alo, *bhi, *blo;
+  int64_t K2p, pza, pzb;      // k extent, batch strides of the A and B planes
+  int64_t kchunk;             // real k per chunk of a k-chunked product (blockIdx.z = chunk)
+  float* part;                // FP32 partial sums (Mp x N2p per chunk), or null
+  int64_t N2p;
+  int64_t M, N;               // complex output extent
+  int tiles_m, tiles_n;
+  void* C;
+  int64_t c_m, c_n, bz_c;
+  int accumulate;
+  const int* skip;
+  int64_t skip_stride;
+};
+
+template <typename TC>
+__global__ __launch_bounds__(512) void zgemm_bf16x3_v2(V2Args g) {
+  __shared__ __align__(16) __bf16 sm[2][4][V2_BM][V2_LD];
+  if (g.skip && g.skip[blockIdx.z * g.skip_stride]) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 2, wn = wave & 3;
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, g.tiles_m, g.tiles_n, tm, tn);
+  const int64_t m0 = (int64_t)tm * V2_BM, c0 = (int64_t)tn * V2_BN;   // c0: real column
+  // blockIdx.z: the problem of a batch, or the k-chunk of a chunked product
+  const int64_t z = g.part ? 0 : blockIdx.z;
+  const int64_t kbeg = g.part ? (int64_t)blockIdx.z * g.kchunk : 0;
+  const __bf16* src[4] = {g.ahi + z * g.pza + m0 * g.K2p + kbeg, g.alo + z * g.pza + m0 * g.K2p + kbeg,
+                          g.bhi + z * g.pzb + c0 * g.K2p + kbeg, g.blo + z * g.pzb + c0 * g.K2p + kbeg};
+  // global -> register staging: per plane 256 rows x 32 bf16 = 1024 chunks of
+  // 16 bytes, two per thread
+  u32x4_t pre[4][2];
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int c = tid + 512 * u, row = c >> 2, part = c & 3;
+        pre[p][u] = *(const u32x4_t*)(src[p] + (int64_t)row * g.K2p + k0 + 8 * part);
+      }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int c = tid + 512 * u, row = c >> 2, part = c & 3;
+        *(u32x4_t*)&sm[buf][p][row][8 * part] = pre[p][u];
+      }
+  };
+  f16v_t acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+  // real k of this workgroup: the whole K2p, or its chunk (the last one may be shorter)
+  const int64_t klen = g.part ? (g.K2p - kbeg < g.kchunk ? g.K2p - kbeg : g.kchunk) : g.K2p;
+  const int nst = (int)(klen / V2_BK);
+  const int r32 = lane & 31, kg = 8 * (lane >> 5);
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const int buf = st & 1;
+    const bool more = st + 1 < nst;
+    if (more) gload((int64_t)(st + 1) * V2_BK);
+#pragma unroll
+    for (int ks = 0; ks < V2_BK / 16; ++ks) {
+      const int kk = 16 * ks + kg;
+      bf16x8_t ah[4], al[4], bh[2], bl[2];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = 128 * wm + 32 * i + r32;
+        ah[i] = *(const bf16x8_t*)&sm[buf][0][row][kk];
+        al[i] = *(const bf16x8_t*)&sm[buf][1][row][kk];
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int col = 64 * wn + 32 * j + r32;
+        bh[j] = *(const bf16x8_t*)&sm[buf][2][col][kk];
+        bl[j] = *(const bf16x8_t*)&sm[buf][3][col][kk];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    if (more) lstore(buf ^ 1);
+    __syncthreads();
+  }
+  // epilogue: element v of a 32 x 32 result sits at column lane % 32 and row
+  // 8 (v / 4) + 4 (lane / 32) + v % 4; real column 2n is Re, 2n + 1 Im, held by
+  // the lane pair (2l, 2l + 1); the even lane writes the complex value
+  if (g.part) {
+    // k-chunk partial sums, real layout (summed by k_v2_reduce)
+    float* P = g.part + (int64_t)blockIdx.z * (int64_t)g.tiles_m * V2_BM * g.N2p;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int64_t c = c0 + 64 * wn + 32 * j + r32;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int64_t m = m0 + 128 * wm + 32 * i + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+          P[m * g.N2p + c] = acc[i][j][v];
+        }
+      }
+    return;
+  }
+  TC* C = (TC*)g.C + z * g.bz_c;
+  const int pb = lane & 1;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int64_t n = (c0 + 64 * wn + 32 * j + r32) >> 1;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const float mine = acc[i][j][v];
+        const float other = __shfl_xor(mine, 1);
+        const int64_t m = m0 + 128 * wm + 32 * i + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (!pb && m < g.M && n < g.N) {
+          double re = mine, im = other;
+          TC* p = C + m * g.c_m + n * g.c_n;
+          if (g.accumulate) {
+            double pr, pi;
+            load_c<double>(p, pr, pi);
+            re += pr; im += pi;
+          }
+          store_c(p, re, im);
+        }
+      }
+    }
+}
+
+// C (complex, strides c_m, c_n; += when accumulating) = sum over the sk
+// k-chunk partials P[s] (Mp x N2p real) in a fixed order
+template <typename TC>
+__global__ void k_v2_reduce(int64_t M, int64_t N, int64_t Mp, int64_t N2p, int sk,
+                            const float* __restrict__ P, TC* __restrict__ C, int64_t c_m,
+                            int64_t c_n, int accumulate) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < M * N;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = t / N, n = t % N;
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < sk; ++s) {
+      const float2 v = *(const float2*)(P + (int64_t)s * Mp * N2p + m * N2p + 2 * n);
+      re += v.x;
+      im += v.y;
+    }
+    TC* p = C + m * c_m + n * c_n;
+    if (accumulate) {
+      double pr, pi;
+      load_c<double>(p, pr, pi);
+      re += pr; im += pi;
+    }
+    store_c(p, re, im);
+  }
+}
+
+// the pre-split path: split passes into `sws`, then the v2 product
+template <typename TA, typename TB, typename TC>
+inline int launch_bf16_v2(const ZGemm& z, hipStream_t s) {
+  const int64_t Mp = v2_round(z.M, V2_BM), N2p = v2_round(2 * z.N, V2_BN),
+                K2p = v2_round(2 * z.K, V2_BK);
+  const int64_t pza = Mp * K2p, pzb = N2p * K2p;
+  char* base = (char*)(((uintptr_t)z.sws + 255) & ~(uintptr_t)255);
+  __bf16* ahi = (__bf16*)base;
+  __bf16* alo = ahi + z.batch * pza;
+  __bf16* bhi = alo + z.batch * pza;
+  __bf16* blo = bhi + z.batch * pzb;
+  const unsigned bt = (unsigned)z.batch;
+  const unsigned kblk = (unsigned)((K2p + 63) / 64);   // 32 complex k per block
+  k_split_a<TA><<<dim3(kblk, (unsigned)(Mp / 64), bt), 256, 0, s>>>(
+      z.M, z.K, (const TA*)z.A, z.a_m, z.a_k, z.conjA, z.bz_a, K2p, pza, ahi, alo);
+  k_split_b<TB><<<dim3(kblk, (unsigned)(N2p / 128), bt), 256, 0, s>>>(
+      z.K, z.N, (const TB*)z.B, z.b_k, z.b_n, z.conjB, z.bz_b, K2p, pzb, bhi, blo);
+  PTN_LAUNCH_CHECK();
+  const int sk = v2_splitk(z.M, z.N, z.K, z.batch);
+  V2Args g;
+  g.ahi = ahi; g.alo = alo; g.bhi = bhi; g.blo = blo;
+  g.K2p = K2p; g.pza = pza; g.pzb = pzb;
+  g.N2p = N2p;
+  g.kchunk = v2_round((K2p + sk - 1) / sk, V2_BK);
+  g.part = nullptr;
+  if (sk > 1)
+    g.part = (float*)(((uintptr_t)(blo + z.batch * pzb) + 255) & ~(uintptr_t)255);
+  g.M = z.M; g.N = z.N;
+  g.tiles_m = (int)(Mp / V2_BM);
+  g.tiles_n = (int)(N2p / V2_BN);
+  g.C = z.C; g.c_m = z.c_m; g.c_n = z.c_n; g.bz_c = z.bz_c;
+  g.accumulate = z.accumulate;
+  g.skip = skip_flag().p;
+  g.skip_stride = skip_flag().stride;
+  // the chunks cover K2p: every chunk but the last is kchunk deep
+  const unsigned nz = sk > 1 ? (unsigned)((K2p + g.kchunk - 1) / g.kchunk) : bt;
+  dim3 grid((unsigned)(g.tiles_m * g.tiles_n), 1, nz);
+  const int slot = timing_pre(s);
+  zgemm_bf16x3_v2<TC><<<grid, 512, 0, s>>>(g);
+  if (sk > 1)
+    k_v2_reduce<TC><<<(unsigned)std::min<int64_t>((z.M * z.N + 255) / 256, 4096), 256, 0, s>>>(
+        z.M, z.N, Mp, N2p, (int)nz, g.part, (TC*)z.C, z.c_m, z.c_n, z.accumulate);
+  PTN_LAUNCH_CHECK();
+  if (slot >= 0) {
+    static const char* name = nullptr;
+    if (!name) {
+      static std::string nm = std::string("zgemm_bf16x3_v2<") + type_tag<TC>::s + ">";
+      name = nm.c_str();
+    }
+    timing_post(slot, s, name, gemm_flop(z), gemm_bytes<TA, TB, TC>(z));
+  }
+  return PTN_OK;
+}
+
+// the pre-split product for shapes that fill the chip with 256 x 256 tiles and
+// have room for the split planes in the caller's workspace
+inline bool use_bf16_v2(const ZGemm& z, void* mws) {
+  if (mws || !z.sws) return false;
+  const int64_t tiles = ((z.M + V2_BM - 1) / V2_BM) * ((2 * z.N + V2_BN - 1) / V2_BN) * z.batch;
+  if (tiles * v2_splitk(z.M, z.N, z.K, z.batch) < 128 || z.K < 16 || z.batch > 65535) return false;
+  return z.sws_bytes >= v2_split_bytes(z.M, z.N, z.K, z.batch);
+}
+
 template <typename TA, typename TB, typename TC>
 inline int launch_typed_bf16(const ZGemm& z, hipStream_t s, void* mws, size_t mws_bytes) {
+  if (use_bf16_v2(z, mws)) return launch_bf16_v2<TA, TB, TC>(z, s);
   int64_t big = ((z.M + 127) / 128) * ((z.N + 63) / 64) * z.batch;
   if (big >= 512) return launch_cfg_bf16<TA, TB, TC, 4, 4>(z, s, mws, mws_bytes);
   return launch_cfg_bf16<TA, TB, TC, 2, 2>(z, s, mws, mws_bytes);
+}
+
+size_t bf16x3_split_bytes(int64_t M, int64_t N, int64_t K, int64_t batch) {
+  return v2_split_bytes(M, N, K, batch);
 }
 
 int zgemm_bf16x3(const ZGemm& z, int ta, int tb, int tc, hipStream_t s, void* mws,

@@ -55,12 +55,23 @@ int convert_c128_c64(int64_t n, const void* x, void* y, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
+// scratch of the pre-split split-bf16 products of a contraction (the larger
+// of its two products; 0 in the other modes)
+static size_t split2(int prec, int64_t batch, int64_t m1, int64_t n1, int64_t k1, int64_t m3,
+                     int64_t n3, int64_t k3) {
+  if (base_prec(prec) != PTN_BF16X3) return 0;
+  const size_t a = bf16x3_split_bytes(m1, n1, k1, batch), b = bf16x3_split_bytes(m3, n3, k3, batch);
+  return align256(a > b ? a : b);
+}
+
 size_t heff_ws_bytes(int prec, const Dims& D, int64_t batch) {
+  const size_t sp = split2(prec, batch, D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl, D.d * D.Dr,
+                           D.Dl * D.Ml);
   prec = base_prec(prec);
   size_t t1 = (size_t)batch * (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
   size_t t2 = (size_t)batch * (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
   return align256(t1) + align256(t2) +
-         mask_bytes2(D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl, D.d * D.Dr, D.Dl * D.Ml) + 256;
+         mask_bytes2(D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl, D.d * D.Dr, D.Dl * D.Ml) + 256 + sp;
 }
 
 size_t env_ws_bytes(int prec, const Dims& D, int64_t batch) {
@@ -72,13 +83,17 @@ size_t env_ws_bytes(int prec, const Dims& D, int64_t batch) {
   size_t wt = (size_t)batch * (size_t)(D.Ml * D.d * D.d * D.Mr) * 16;
   const size_t mr = mask_bytes2(D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl * D.Ml, D.Dl, D.d * D.Dr);
   const size_t ml = mask_bytes2(D.Dl * D.Ml, D.d * D.Dr, D.Dl, D.Dr, D.Mr * D.Dr, D.Dl * D.d);
-  return 2 * align256(a > b ? a : b) + align256(wt) + (mr > ml ? mr : ml) + 256;
+  const size_t sr = split2(prec, batch, D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl * D.Ml, D.Dl, D.d * D.Dr);
+  const size_t sl = split2(prec, batch, D.Dl * D.Ml, D.d * D.Dr, D.Dl, D.Dr, D.Mr * D.Dr, D.Dl * D.d);
+  return 2 * align256(a > b ? a : b) + align256(wt) + (mr > ml ? mr : ml) + 256 +
+         (sr > sl ? sr : sl);
 }
 
 size_t bond_ws_bytes(int prec, int64_t Dl, int64_t Dr, int64_t M, int64_t batch) {
+  const size_t sp = split2(prec, batch, Dl, M * Dr, Dr, Dl, Dr, Dl * M);
   prec = base_prec(prec);
   return align256((size_t)batch * (size_t)Dl * (size_t)(M * Dr) * csize(prec)) +
-         mask_bytes2(Dl, M * Dr, Dr, Dl, Dr, Dl * M) + 256;
+         mask_bytes2(Dl, M * Dr, Dr, Dl, Dr, Dl * M) + 256 + sp;
 }
 
 // zero-width virtual bonds (Dl or Dr = 0, as a split that keeps no singular
@@ -95,7 +110,11 @@ int heff_apply(int prec, const Dims& D, const void* l, int tl, const void* w, co
                const void* a, int ta, void* out, int tout, void* ws, size_t ws_bytes,
                hipStream_t s, int64_t batch) {
   if (!dims_ok(D) || batch < 1) return PTN_ERR_ARG;
-  if (ws_bytes < heff_ws_bytes(prec, D, batch)) return PTN_ERR_WORKSPACE;
+  const size_t wtot = heff_ws_bytes(prec, D, batch);
+  if (ws_bytes < wtot) return PTN_ERR_WORKSPACE;
+  const size_t spb = split2(prec, batch, D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl, D.d * D.Dr,
+                            D.Dl * D.Ml);
+  void* sws = spb ? (char*)ws + (wtot - spb) : nullptr;
   const bool sp = is_sparse(prec) && batch == 1;
   prec = base_prec(prec);
   const int ct = ctype(prec), cp = gemm_compute(prec);
@@ -112,7 +131,7 @@ int heff_apply(int prec, const Dims& D, const void* l, int tl, const void* w, co
   ZGemm g1 = {D.Dl * D.d, D.Mr * D.Dr, D.Dr,
               a, D.Dr, 1, 0,
               r, D.Mr * D.Dr, 1, 0,
-              t1, D.Mr * D.Dr, 1, 0, batch, nA, nR, n1};
+              t1, D.Mr * D.Dr, 1, 0, batch, nA, nR, n1, sws, spb};
   if ((st = zgemm(g1, cp, ta, tr, ct, mws, mb, s))) return st;
   if ((st = mode_mix(D.Dl, D.d * D.Mr, D.Ml * D.d, D.Dr, w, t1, ct, t2, ct, s, batch,
                      D.Ml * D.d * D.d * D.Mr)))
@@ -120,14 +139,18 @@ int heff_apply(int prec, const Dims& D, const void* l, int tl, const void* w, co
   ZGemm g3 = {D.Dl, D.d * D.Dr, D.Dl * D.Ml,
               l, 1, D.Dl, 0,
               t2, D.d * D.Dr, 1, 0,
-              out, D.d * D.Dr, 1, 0, batch, nL, n2, nA};
+              out, D.d * D.Dr, 1, 0, batch, nL, n2, nA, sws, spb};
   return zgemm(g3, cp, tl, ct, tout, mws, mb, s);
 }
 
 int env_right(int prec, const Dims& D, const void* a, const void* b, const void* w, const void* r,
               int tr, void* out, void* ws, size_t ws_bytes, hipStream_t s, int64_t batch) {
   if (!dims_ok(D) || batch < 1) return PTN_ERR_ARG;
-  if (ws_bytes < env_ws_bytes(prec, D, batch)) return PTN_ERR_WORKSPACE;
+  const size_t wtot = env_ws_bytes(prec, D, batch);
+  if (ws_bytes < wtot) return PTN_ERR_WORKSPACE;
+  const size_t spb = split2(prec, batch, D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl * D.Ml, D.Dl,
+                            D.d * D.Dr);
+  void* sws = spb ? (char*)ws + (wtot - spb) : nullptr;
   const bool sp = is_sparse(prec) && batch == 1;
   prec = base_prec(prec);
   const int ct = ctype(prec), cp = gemm_compute(prec);
@@ -146,7 +169,7 @@ int env_right(int prec, const Dims& D, const void* a, const void* b, const void*
   ZGemm g1 = {D.Dl * D.d, D.Mr * D.Dr, D.Dr,
               a, D.Dr, 1, 0,
               r, D.Mr * D.Dr, 1, 0,
-              t1, D.Mr * D.Dr, 1, 0, batch, nA, nR, n1};
+              t1, D.Mr * D.Dr, 1, 0, batch, nA, nR, n1, sws, spb};
   if ((st = zgemm(g1, cp, PTN_T_C128, tr, ct, mws, mb, s))) return st;
   if ((st = mode_mix(D.Dl, D.d * D.Mr, D.Ml * D.d, D.Dr, w, t1, ct, t2, ct, s, batch,
                      D.Ml * D.d * D.d * D.Mr)))
@@ -155,14 +178,18 @@ int env_right(int prec, const Dims& D, const void* a, const void* b, const void*
   ZGemm g3 = {D.Dl * D.Ml, D.Dl, D.d * D.Dr,
               t2, D.d * D.Dr, 1, 0,
               b, 1, D.d * D.Dr, 1,
-              out, D.Dl, 1, 0, batch, n2, nA, nO};
+              out, D.Dl, 1, 0, batch, n2, nA, nO, sws, spb};
   return zgemm(g3, cp, ct, PTN_T_C128, PTN_T_C128, mws, mb, s);
 }
 
 int env_left(int prec, const Dims& D, const void* a, const void* b, const void* w, const void* l,
              int tl, void* out, void* ws, size_t ws_bytes, hipStream_t s, int64_t batch) {
   if (!dims_ok(D) || batch < 1) return PTN_ERR_ARG;
-  if (ws_bytes < env_ws_bytes(prec, D, batch)) return PTN_ERR_WORKSPACE;
+  const size_t wtot = env_ws_bytes(prec, D, batch);
+  if (ws_bytes < wtot) return PTN_ERR_WORKSPACE;
+  const size_t spb = split2(prec, batch, D.Dl * D.Ml, D.d * D.Dr, D.Dl, D.Dr, D.Mr * D.Dr,
+                            D.Dl * D.d);
+  void* sws = spb ? (char*)ws + (wtot - spb) : nullptr;
   const bool sp = is_sparse(prec) && batch == 1;
   prec = base_prec(prec);
   const int ct = ctype(prec), cp = gemm_compute(prec);
@@ -188,7 +215,7 @@ int env_left(int prec, const Dims& D, const void* a, const void* b, const void* 
   ZGemm g1 = {D.Dl * D.Ml, D.d * D.Dr, D.Dl,
               l, D.Dl, 1, 0,
               b, D.d * D.Dr, 1, 1,
-              t1, D.d * D.Dr, 1, 0, batch, nL, nA, n1};
+              t1, D.d * D.Dr, 1, 0, batch, nL, nA, n1, sws, spb};
   if ((st = zgemm(g1, cp, tl, PTN_T_C128, ct, mws, mb, s))) return st;
   // T2[l, (s, m'), rb] = sum_{(m, sb)} W[m, sb, s, m'] T1[l, (m, sb), rb]
   if ((st = mode_mix(D.Dl, D.Ml * D.d, D.d * D.Mr, D.Dr, wt, t1, ct, t2, ct, s, batch, nW)))
@@ -197,7 +224,7 @@ int env_left(int prec, const Dims& D, const void* a, const void* b, const void* 
   ZGemm g3 = {D.Dr, D.Mr * D.Dr, D.Dl * D.d,
               a, 1, D.Dr, 0,
               t2, D.Mr * D.Dr, 1, 0,
-              out, D.Mr * D.Dr, 1, 0, batch, nA, n2, nO};
+              out, D.Mr * D.Dr, 1, 0, batch, nA, n2, nO, sws, spb};
   return zgemm(g3, cp, PTN_T_C128, ct, PTN_T_C128, mws, mb, s);
 }
 
@@ -205,7 +232,10 @@ int bond_apply(int prec, int64_t Dl, int64_t Dr, int64_t M, const void* l, int t
                int tr, const void* c, int tcin, void* out, int tout, void* ws, size_t ws_bytes,
                hipStream_t s, int64_t batch) {
   if (Dl < 0 || Dr < 0 || M <= 0 || batch < 1) return PTN_ERR_ARG;
-  if (ws_bytes < bond_ws_bytes(prec, Dl, Dr, M, batch)) return PTN_ERR_WORKSPACE;
+  const size_t wtot = bond_ws_bytes(prec, Dl, Dr, M, batch);
+  if (ws_bytes < wtot) return PTN_ERR_WORKSPACE;
+  const size_t spb = split2(prec, batch, Dl, M * Dr, Dr, Dl, Dr, Dl * M);
+  void* sws = spb ? (char*)ws + (wtot - spb) : nullptr;
   const bool sp = is_sparse(prec) && batch == 1;
   prec = base_prec(prec);
   const int ct = ctype(prec), cp = gemm_compute(prec);
@@ -218,13 +248,13 @@ int bond_apply(int prec, int64_t Dl, int64_t Dr, int64_t M, const void* l, int t
   ZGemm g1 = {Dl, M * Dr, Dr,
               c, Dr, 1, 0,
               r, M * Dr, 1, 0,
-              t, M * Dr, 1, 0, batch, nC, nR, nT};
+              t, M * Dr, 1, 0, batch, nC, nR, nT, sws, spb};
   if ((st = zgemm(g1, cp, tcin, tr, ct, mws, mb, s))) return st;
   // out[lb, rb] = sum_{(l, m)} L[(l, m), lb] T[(l, m), rb]
   ZGemm g3 = {Dl, Dr, Dl * M,
               l, 1, Dl, 0,
               t, Dr, 1, 0,
-              out, Dr, 1, 0, batch, nL, nT, nC};
+              out, Dr, 1, 0, batch, nL, nT, nC, sws, spb};
   return zgemm(g3, cp, tl, ct, tout, mws, mb, s);
 }
 

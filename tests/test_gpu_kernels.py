@@ -666,3 +666,28 @@ def test_low_precision_lanczos_breaks_down_on_invariant_subspace(precision):
     assert ilp[0] == 2 and ilp[1] == 2
     tol = 1e-6 if precision == "fp32" else 1e-3
     assert relerr(olp, backend.to_host(o64)) < tol
+
+
+@pytest.mark.parametrize("Dl,Dr", [(515, 509), (512, 512)])
+def test_presplit_bf16x3_products_match_oracle(Dl, Dr):
+    """The pre-split split-bf16 products (operands split once into bf16 hi / lo
+    planes, the real embedding of B built in that pass, 256 x 256 MFMA tiles)
+    at ragged and aligned shapes large enough to select them: H_eff, both
+    environment steps and the bond operator against the oracle at the split-bf16
+    bound; the launched kernels include the pre-split product."""
+    a, w, l, r = _heff_problem(Dl + Dr, Dl=Dl, d=4, Dr=Dr, M=5)
+    backend.timing_begin()
+    got = backend.heff_apply(a, w, l, r, "bf16x3")
+    names = [t["name"] for t in backend.timing_end()]
+    assert any(n.startswith("zgemm_bf16x3_v2<") for n in names), names
+    assert relerr(got, orc.apply_local_hamiltonian(a, w, l, r)) < BF16X3_TOL
+    a2 = a[:, :2, :] + 0.0
+    w2 = w[:, :2, :2, :] + 0.0
+    assert relerr(backend.env_left(a2, a2, w2, l, "bf16x3"),
+                  orc.contraction_operator_step_left(a2, a2, w2, l)) < BF16X3_TOL
+    assert relerr(backend.env_right(a2, a2, w2, r, "bf16x3"),
+                  orc.contraction_operator_step_right(a2, a2, w2, r)) < BF16X3_TOL
+    rng = np.random.default_rng(4)
+    c = orc.crandn((Dl, Dr), rng)
+    assert relerr(backend.bond_apply(c, l, r, "bf16x3"),
+                  orc.apply_local_bond_contraction(c, l, r)) < BF16X3_TOL

@@ -24,7 +24,7 @@ TOL = {"fp64": 1e-12, "fp32": 2e-5, "bf16x3": 2e-4}
 # product-kernel name prefixes of the large tiles the launch rule selects at D = 1024
 BIG_TILE = {"fp64": lambda n: n.startswith("zgemm_kernel<double,") and ", 4, 4, " in n,
             "fp32": lambda n: n.startswith("zgemm_kernel<float,") and ", 4, 8, " in n,
-            "bf16x3": lambda n: n.startswith("zgemm_bf16x3_kernel<") and ", 4, 4, " in n}
+            "bf16x3": lambda n: n.startswith("zgemm_bf16x3_v2<")}
 
 
 def relerr(got, ref):
