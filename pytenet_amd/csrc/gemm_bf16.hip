@@ -380,7 +380,9 @@ struct V2Args {
 template <typename TC>
 __global__ __launch_bounds__(512) void zgemm_bf16x3_v2(V2Args g) {
   __shared__ __align__(16) __bf16 sm[2][4][V2_BM][V2_LD];
-  if (g.skip && g.skip[blockIdx.z * g.skip_stride]) return;
+  // predication by the problem's flag (blockIdx.z is the k-chunk of a chunked
+  // product, which has a single problem)
+  if (g.skip && g.skip[(g.part ? 0 : blockIdx.z) * g.skip_stride]) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 2, wn = wave & 3;
   int tm, tn;
@@ -508,7 +510,8 @@ __global__ __launch_bounds__(512) void zgemm_bf16x3_v2(V2Args g) {
 template <typename TC>
 __global__ void k_v2_reduce(int64_t M, int64_t N, int64_t Mp, int64_t N2p, int sk,
                             const float* __restrict__ P, TC* __restrict__ C, int64_t c_m,
-                            int64_t c_n, int accumulate) {
+                            int64_t c_n, int accumulate, const int* __restrict__ skip) {
+  if (skip && skip[0]) return;   // the single problem is predicated off
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < M * N;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t m = t / N, n = t % N;
@@ -569,7 +572,7 @@ inline int launch_bf16_v2(const ZGemm& z, hipStream_t s) {
   zgemm_bf16x3_v2<TC><<<grid, 512, 0, s>>>(g);
   if (sk > 1)
     k_v2_reduce<TC><<<(unsigned)std::min<int64_t>((z.M * z.N + 255) / 256, 4096), 256, 0, s>>>(
-        z.M, z.N, Mp, N2p, (int)nz, g.part, (TC*)z.C, z.c_m, z.c_n, z.accumulate);
+        z.M, z.N, Mp, N2p, (int)nz, g.part, (TC*)z.C, z.c_m, z.c_n, z.accumulate, g.skip);
   PTN_LAUNCH_CHECK();
   if (slot >= 0) {
     static const char* name = nullptr;

@@ -6,8 +6,10 @@
 // of the launch. ptn_timing_end waits for the events and sums, per kernel
 // instance, the launch count, the device time and the flops/bytes. Event
 // records cost a few microseconds of host time per launch and do not
-// synchronise anything, so a sweep can be timed with it switched on.
+// synchronise anything, so a sweep can be timed with it switched on (beyond
+// 32768 pending launches the oldest are read, long after they completed).
 #include <algorithm>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <string>
@@ -21,13 +23,57 @@ namespace {
 struct Rec {
   const char* name;
   double flop, bytes;
+  hipEvent_t e0, e1;
+  int seq;
+};
+struct Agg {
+  int64_t n = 0;
+  double ms = 0, flop = 0, bytes = 0;
 };
 std::mutex g_mu;
 bool g_on = false;
-int64_t g_dropped = 0;
-std::vector<hipEvent_t> g_events;   // two per record, reused across timing windows
-std::vector<Rec> g_recs;
-constexpr size_t MAX_RECS = 1u << 18;
+int g_err = PTN_OK;
+int g_seq = 0;
+std::vector<hipEvent_t> g_free;      // recycled events
+std::deque<Rec> g_pending;           // launches whose events are not read yet
+std::map<std::string, Agg> g_agg;    // folded records of the current window
+// pending records beyond this many are folded (their events waited for and
+// read, oldest first), so a long window keeps a bounded number of events
+constexpr size_t MAX_PENDING = 1u << 15;
+
+hipEvent_t take_event() {
+  if (!g_free.empty()) {
+    hipEvent_t e = g_free.back();
+    g_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// read the oldest pending record into the aggregates (caller holds g_mu)
+void fold_front() {
+  Rec r = g_pending.front();
+  g_pending.pop_front();
+  if (r.name && r.e1) {
+    hipError_t e = hipEventSynchronize(r.e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, r.e0, r.e1);
+    if (e != hipSuccess) {
+      ptn_set_last_hip_error(e);
+      g_err = PTN_ERR_HIP;
+    } else {
+      Agg& a = g_agg[r.name];
+      a.n += 1;
+      a.ms += ms;
+      a.flop += r.flop;
+      a.bytes += r.bytes;
+    }
+  }
+  if (r.e0) g_free.push_back(r.e0);
+  if (r.e1) g_free.push_back(r.e1);
+}
 }  // namespace
 
 int timing_pre(hipStream_t s) {
@@ -40,28 +86,30 @@ int timing_pre(hipStream_t s) {
   }
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g_on) return -1;
-  const size_t idx = g_recs.size();
-  if (idx >= MAX_RECS) { ++g_dropped; return -1; }
-  while (g_events.size() < 2 * (idx + 1)) {
-    hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) { ++g_dropped; return -1; }
-    g_events.push_back(e);
+  while (g_pending.size() >= MAX_PENDING) fold_front();
+  Rec r{nullptr, 0.0, 0.0, take_event(), take_event(), g_seq};
+  if (!r.e0 || !r.e1 || hipEventRecord(r.e0, s) != hipSuccess) {
+    if (r.e0) g_free.push_back(r.e0);
+    if (r.e1) g_free.push_back(r.e1);
+    return -1;
   }
-  if (hipEventRecord(g_events[2 * idx], s) != hipSuccess) { ++g_dropped; return -1; }
-  g_recs.push_back({nullptr, 0.0, 0.0});
-  return (int)idx;
+  g_seq = (g_seq + 1) & 0x3fffffff;
+  g_pending.push_back(r);
+  return r.seq;
 }
 
 void timing_post(int slot, hipStream_t s, const char* name, double flop, double bytes) {
   if (slot < 0) return;
   std::lock_guard<std::mutex> lk(g_mu);
-  if ((size_t)slot >= g_recs.size()) return;
-  if (hipEventRecord(g_events[2 * (size_t)slot + 1], s) != hipSuccess) {
-    g_recs[slot].name = nullptr;   // unusable pair: left out of the sums
-    ++g_dropped;
+  // the record of this launch: normally the newest (searched from the back)
+  for (auto it = g_pending.rbegin(); it != g_pending.rend(); ++it) {
+    if (it->seq != slot) continue;
+    if (hipEventRecord(it->e1, s) != hipSuccess) return;   // left out of the sums
+    it->name = name;
+    it->flop = flop;
+    it->bytes = bytes;
     return;
   }
-  g_recs[slot] = {name, flop, bytes};
 }
 
 }  // namespace ptn
@@ -72,36 +120,19 @@ extern "C" {
 
 int ptn_timing_begin(void) {
   std::lock_guard<std::mutex> lk(g_mu);
-  g_recs.clear();
-  g_dropped = 0;
+  while (!g_pending.empty()) fold_front();
+  g_agg.clear();
+  g_err = PTN_OK;
   g_on = true;
   return PTN_OK;
 }
 
 int ptn_timing_end(ptn_kernel_timing* out, int max_out, int* n_out) {
-  std::vector<Rec> recs;
-  {
-    std::lock_guard<std::mutex> lk(g_mu);
-    g_on = false;
-    recs.swap(g_recs);
-  }
-  struct Agg { int64_t n = 0; double ms = 0, flop = 0, bytes = 0; };
-  std::map<std::string, Agg> agg;
-  int st = PTN_OK;
-  for (size_t i = 0; i < recs.size(); ++i) {
-    if (!recs[i].name) continue;
-    hipEvent_t e0 = g_events[2 * i], e1 = g_events[2 * i + 1];
-    hipError_t e = hipEventSynchronize(e1);
-    float ms = 0.0f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    if (e != hipSuccess) { ptn_set_last_hip_error(e); st = PTN_ERR_HIP; continue; }
-    Agg& a = agg[recs[i].name];
-    a.n += 1;
-    a.ms += ms;
-    a.flop += recs[i].flop;
-    a.bytes += recs[i].bytes;
-  }
-  std::vector<std::pair<std::string, Agg>> rows(agg.begin(), agg.end());
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_on = false;
+  while (!g_pending.empty()) fold_front();
+  std::vector<std::pair<std::string, Agg>> rows(g_agg.begin(), g_agg.end());
+  g_agg.clear();
   std::sort(rows.begin(), rows.end(),
             [](const std::pair<std::string, Agg>& x, const std::pair<std::string, Agg>& y) {
               return x.second.ms > y.second.ms;
@@ -118,6 +149,8 @@ int ptn_timing_end(ptn_kernel_timing* out, int max_out, int* n_out) {
     o.bytes = r.second.bytes;
   }
   if (n_out) *n_out = n;
+  const int st = g_err;
+  g_err = PTN_OK;
   return st;
 }
 

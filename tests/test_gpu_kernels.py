@@ -691,3 +691,29 @@ def test_presplit_bf16x3_products_match_oracle(Dl, Dr):
     c = orc.crandn((Dl, Dr), rng)
     assert relerr(backend.bond_apply(c, l, r, "bf16x3"),
                   orc.apply_local_bond_contraction(c, l, r)) < BF16X3_TOL
+
+
+def test_presplit_bf16x3_inside_lanczos_matches_fp64():
+    """A local solve whose matvec runs the k-chunked pre-split split-bf16
+    product (one-site H_eff at D = 512: 80 output tiles, two k-chunks) under
+    the Lanczos predication: finite and within the split-bf16 accuracy of the
+    FP64 solve (the predication flag belongs to the problem, not to a k-chunk)."""
+    a, w, l, r = _heff_problem(77, Dl=512, d=2, Dr=512, M=5)
+    a = a / np.linalg.norm(a)
+    prev = backend.set_graphs(False)    # direct launches, so that the launch timing sees them
+    try:
+        backend.timing_begin()
+        o16, _, i16 = backend.krylov_local(_lib.PTN_OP_HEFF, a, l, w, r, 6, "expm", -0.02j,
+                                           "bf16x3")
+        names = [t["name"] for t in backend.timing_end()]
+    finally:
+        backend.set_graphs(prev)
+    g_o16, _, _ = backend.krylov_local(_lib.PTN_OP_HEFF, a, l, w, r, 6, "expm", -0.02j,
+                                       "bf16x3")
+    assert torch.equal(g_o16, o16)     # the captured graph computes the same
+    o64, _, i64 = backend.krylov_local(_lib.PTN_OP_HEFF, a, l, w, r, 6, "expm", -0.02j)
+    assert any(n.startswith("zgemm_bf16x3_v2<") for n in names), names
+    assert np.all(backend.to_host(i16) == backend.to_host(i64))
+    g16 = backend.to_host(o16)
+    assert np.all(np.isfinite(g16))
+    assert relerr(g16, backend.to_host(o64)) < 1e-3
