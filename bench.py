@@ -178,7 +178,7 @@ def heff_roofline(D, prec, reps=10):
     return ms, heff_flops(D, 4, D, 5, 5) / (ms * 1e-3) / 1e12
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "zgemm_tdvp2_D1024_hbm_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "svd_split_2048_kernels_and_hbm_traffic.txt")
 
 
 def profiled_traffic(kernel):
@@ -208,41 +208,6 @@ def svd_time(D, precond32, reps=2):
         _, _, _, inf = backend.svd(a, return_info=True, precond32=precond32)
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / reps, int(inf[0]), int(inf[3])
-
-
-def jacobi_apply_roofline(D):
-    """Live HBM roofline of the SVD's dominant kernel, k_jac_apply<double>: one
-    FP64 SVD of a seeded random 2D x 2D matrix under torch.profiler (device
-    times of the library's kernels on the stream they run on); the first sweep
-    (2D/16 - 1 launches) rotates every block pair, so each of its launches
-    reads and writes all of X (2D x 2D) and V (2D x 2D): 4 x (2D)^2 x 16 bytes."""
-    import torch
-    from torch.profiler import profile, ProfilerActivity
-    from pytenet_amd import backend
-    dev = backend.device()
-    g = torch.Generator(device=dev).manual_seed(1)
-    a = torch.randn(2 * D, 2 * D, dtype=torch.complex128, device=dev, generator=g)
-    backend.svd(a)
-    torch.cuda.synchronize()
-    with profile(activities=[ProfilerActivity.CUDA]) as prof:
-        backend.svd(a)
-        torch.cuda.synchronize()
-    ev = [e for e in prof.events() if "k_jac_apply<double>" in e.name]
-    ev.sort(key=lambda e: e.time_range.start)
-    rounds = 2 * D // 16 - 1
-    first = ev[:rounds]
-    if len(first) < rounds:
-        return None
-    dur = [getattr(e, "device_time_total", 0.0) or e.cuda_time_total for e in first]
-    avg_us = float(np.mean(dur))
-    nbytes = 4.0 * (2 * D)**2 * 16
-    ach = nbytes / (avg_us * 1e-6) / 1e9
-    return {"kernel": "k_jac_apply<double>: Jacobi rotation of the X and V column pairs, "
-                      "first sweep of a %d x %d SVD" % (2 * D, 2 * D),
-            "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_launch": nbytes, "avg_us": round(avg_us, 2), "launches": len(first),
-            "source": "torch.profiler device times of the first-sweep launches"}
 
 
 def cpu_baseline(D, L, numiter, budget_s=30.0):
@@ -472,24 +437,70 @@ def workload_chains(args):
     return out
 
 
-def _safe(fn, *a):
-    try:
-        return fn(*a)
-    except Exception as exc:   # optional measurement: report the failure instead
-        return {"error": f"{type(exc).__name__}: {exc}"}
+def _graded_matrix(n, decades, seed):
+    """n x n complex matrix with singular values 10^(-decades i / (n-1)), built
+    on the GPU from two seeded random unitaries (the library's QR and product)."""
+    import torch
+    from pytenet_amd import backend
+    dev = backend.device()
+    g = torch.Generator(device=dev).manual_seed(seed)
+    u, _ = backend.qr(torch.randn(n, n, dtype=torch.complex128, device=dev, generator=g))
+    v, _ = backend.qr(torch.randn(n, n, dtype=torch.complex128, device=dev, generator=g))
+    s = 10.0 ** (-decades * torch.arange(n, device=dev, dtype=torch.float64) / (n - 1))
+    # u diag(s) v^H: the adjoint through the product's conjugation flag on v's transpose
+    return backend.matmul((u * s.to(torch.complex128)).contiguous(), v.transpose(0, 1),
+                          conjB=True)
 
 
 def workload_svd(args):
-    """The bond SVD of the two-site split at D: a seeded random 2D x 2D complex
-    matrix, plain FP64 Jacobi and FP32-preconditioned Jacobi (two timed calls
-    each after one untimed)."""
+    """The bond split of the two-site TDVP at D: the truncating split
+    (ptn_svd_split, cap D, tol 1e-10) of a 2D x 2D complex matrix with a
+    spectrum graded over nine decades and of a seeded random one, in FP64 and
+    with FP32 Jacobi preconditioning, plus the full SVD (two timed calls each
+    after one untimed). With --copy-calibration, a plain 512 MiB device copy
+    first (the reference point of HBM byte counters in a profile run)."""
+    import torch
+    from pytenet_amd import backend
     D = args.D or 1024
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    out = {"workload": f"bond split of a {2 * D} x {2 * D} complex matrix", "unit": "s"}
+    g = torch.Generator(device=backend.device()).manual_seed(1)
+    mats = {"graded": _graded_matrix(2 * D, 9.0, 7),
+            "random": torch.randn(2 * D, 2 * D, dtype=torch.complex128, device="cuda", generator=g)}
+    timed = {}
+    for name, a in mats.items():
+        for pre in (False, True):
+            key = f"{name}_{'precond32' if pre else 'fp64'}"
+            backend.svd_split(a, "right", 1e-10, D, pre)
+            torch.cuda.synchronize()
+            backend.timing_begin()
+            t0 = time.perf_counter()
+            for _ in range(2):
+                backend.svd_split(a, "right", 1e-10, D, pre)
+            torch.cuda.synchronize()
+            out[f"split_{key}_s"] = round((time.perf_counter() - t0) / 2, 4)
+            for t in backend.timing_end():
+                agg = timed.setdefault(t["name"], {"launches": 0, "total_ms": 0.0, "bytes": 0.0,
+                                                   "flop": 0.0})
+                for f in ("launches", "total_ms", "bytes", "flop"):
+                    agg[f] += t[f]
+    out["timed_kernels"] = [{"name": k, "launches": v["launches"],
+                             "avg_us": round(1e3 * v["total_ms"] / v["launches"], 2),
+                             "bytes_per_launch": v["bytes"] / v["launches"],
+                             "flop_per_launch": v["flop"] / v["launches"]}
+                            for k, v in sorted(timed.items(), key=lambda kv: -kv[1]["total_ms"])[:8]]
+    out["value"] = out["split_graded_precond32_s"]
     t64, sw64, _ = svd_time(D, False)
     t32, sw64p, sw32p = svd_time(D, True)
-    return {"workload": f"SVD of a {2 * D} x {2 * D} complex matrix (seeded random)",
-            "unit": "s", "value": round(t32, 4), "fp64_s": round(t64, 4), "fp64_sweeps": sw64,
-            "precond32_s": round(t32, 4), "precond32_sweeps_fp32_fp64": [sw32p, sw64p],
-            "apply_roofline": _safe(jacobi_apply_roofline, D)}
+    out.update({"full_svd_random_fp64_s": round(t64, 4), "full_svd_random_fp64_sweeps": sw64,
+                "full_svd_random_precond32_s": round(t32, 4),
+                "full_svd_random_precond32_sweeps_fp32_fp64": [sw32p, sw64p]})
+    return out
 
 
 WORKLOADS = {"dmrg1": workload_dmrg1, "dmrg2-u1": workload_dmrg2_u1, "chains": workload_chains,
@@ -520,6 +531,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--strict-svd", action="store_true",
                     help="with --extras: also time the mixed sweep with the plain FP64 Jacobi SVD")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="with --workload svd: a plain 512 MiB device copy first (counter reference)")
     ap.add_argument("--extras", action="store_true",
                     help="after the JSON line, time the bond SVD alone and print it to stderr")
     ap.add_argument("--time-budget", type=float, default=420.0,

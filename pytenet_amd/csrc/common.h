@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdint.h>
 #include "../../include/ptn.h"
 
@@ -46,11 +47,25 @@ __device__ __forceinline__ void store_c(float2* p, double re, double im) {
 
 void ptn_set_last_hip_error(hipError_t e);
 
-// live launch timing (timing.hip): timing_pre records a start event on s and
-// returns a slot (-1 when timing is off); timing_post records the end event
-// with the launch's kernel name and algorithmic flops / bytes
-int timing_pre(hipStream_t s);
-void timing_post(int slot, hipStream_t s, const char* name, double flop, double bytes);
+// live launch timing (timing.hip): timing_acquire hands out a start / stop
+// event pair for the next launch on s (slot -1 and no events when timing is
+// off or s is capturing); timing_commit attaches the kernel name and the
+// launch's algorithmic flops / bytes. tlaunch launches a kernel with the pair
+// attached to its dispatch (hipExtLaunchKernelGGL), or plainly.
+int timing_acquire(hipStream_t s, hipEvent_t* e0, hipEvent_t* e1);
+void timing_commit(int slot, const char* name, double flop, double bytes);
+template <typename F, typename... Args>
+inline void tlaunch(const char* name, double flop, double bytes, F kernel, dim3 grid, dim3 block,
+                    hipStream_t s, Args... args) {
+  hipEvent_t e0, e1;
+  const int slot = timing_acquire(s, &e0, &e1);
+  if (slot >= 0) {
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, args...);
+    timing_commit(slot, name, flop, bytes);
+  } else {
+    kernel<<<grid, block, 0, s>>>(args...);
+  }
+}
 
 // Device-side predication of the matvec products: while set (on the calling
 // host thread), the product and mode-mix kernels of batch problem z return at

@@ -225,20 +225,18 @@ inline int launch_cfg_bf16(const ZGemm& z, hipStream_t s, void* mws, size_t mws_
   dim3 grid((unsigned)nwg, 1, (unsigned)z.batch);
   const bool ak = (z.a_k == 1) || (z.a_m != 1);
   const bool bn = (z.b_n == 1) || (z.b_k != 1);
-  const int slot = timing_pre(s);
+  const char* nm = gemm_kernel_name<float, TA, TB, TC, TM, TN>(ak, bn, "zgemm_bf16x3_kernel");
+  const double fl = gemm_flop(z), by = gemm_bytes<TA, TB, TC>(z);
   if (ak && bn)
-    zgemm_bf16x3_kernel<TA, TB, TC, TM, TN, true, true><<<grid, 256, 0, s>>>(g);
+    tlaunch(nm, fl, by, zgemm_bf16x3_kernel<TA, TB, TC, TM, TN, true, true>, grid, dim3(256), s, g);
   else if (ak && !bn)
-    zgemm_bf16x3_kernel<TA, TB, TC, TM, TN, true, false><<<grid, 256, 0, s>>>(g);
+    tlaunch(nm, fl, by, zgemm_bf16x3_kernel<TA, TB, TC, TM, TN, true, false>, grid, dim3(256), s, g);
   else if (!ak && bn)
-    zgemm_bf16x3_kernel<TA, TB, TC, TM, TN, false, true><<<grid, 256, 0, s>>>(g);
+    tlaunch(nm, fl, by, zgemm_bf16x3_kernel<TA, TB, TC, TM, TN, false, true>, grid, dim3(256), s, g);
   else
-    zgemm_bf16x3_kernel<TA, TB, TC, TM, TN, false, false><<<grid, 256, 0, s>>>(g);
+    tlaunch(nm, fl, by, zgemm_bf16x3_kernel<TA, TB, TC, TM, TN, false, false>, grid, dim3(256), s,
+            g);
   PTN_LAUNCH_CHECK();
-  if (slot >= 0)
-    timing_post(slot, s,
-                gemm_kernel_name<float, TA, TB, TC, TM, TN>(ak, bn, "zgemm_bf16x3_kernel"),
-                gemm_flop(z), gemm_bytes<TA, TB, TC>(z));
   return PTN_OK;
 }
 
@@ -568,20 +566,13 @@ inline int launch_bf16_v2(const ZGemm& z, hipStream_t s) {
   // the chunks cover K2p: every chunk but the last is kchunk deep
   const unsigned nz = sk > 1 ? (unsigned)((K2p + g.kchunk - 1) / g.kchunk) : bt;
   dim3 grid((unsigned)(g.tiles_m * g.tiles_n), 1, nz);
-  const int slot = timing_pre(s);
-  zgemm_bf16x3_v2<TC><<<grid, 512, 0, s>>>(g);
+  static const std::string v2name = std::string("zgemm_bf16x3_v2<") + type_tag<TC>::s + ">";
+  tlaunch(v2name.c_str(), gemm_flop(z), gemm_bytes<TA, TB, TC>(z), zgemm_bf16x3_v2<TC>, grid,
+          dim3(512), s, g);
   if (sk > 1)
     k_v2_reduce<TC><<<(unsigned)std::min<int64_t>((z.M * z.N + 255) / 256, 4096), 256, 0, s>>>(
         z.M, z.N, Mp, N2p, (int)nz, g.part, (TC*)z.C, z.c_m, z.c_n, z.accumulate, g.skip);
   PTN_LAUNCH_CHECK();
-  if (slot >= 0) {
-    static const char* name = nullptr;
-    if (!name) {
-      static std::string nm = std::string("zgemm_bf16x3_v2<") + type_tag<TC>::s + ">";
-      name = nm.c_str();
-    }
-    timing_post(slot, s, name, gemm_flop(z), gemm_bytes<TA, TB, TC>(z));
-  }
   return PTN_OK;
 }
 

@@ -397,19 +397,17 @@ inline int launch_cfg(const ZGemm& z, hipStream_t s, void* mws = nullptr, size_t
   // a K-contiguous A when its k stride is 1 (or K == 1), else M-contiguous
   const bool ak = (z.a_k == 1) || (z.a_m != 1);
   const bool bn = (z.b_n == 1) || (z.b_k != 1);
-  const int slot = timing_pre(s);
+  const char* nm = gemm_kernel_name<Tc, TA, TB, TC, TM, TN>(ak, bn);
+  const double fl = gemm_flop(z), by = gemm_bytes<TA, TB, TC>(z);
   if (ak && bn)
-    zgemm_kernel<Tc, TA, TB, TC, TM, TN, true, true><<<grid, 256, 0, s>>>(g);
+    tlaunch(nm, fl, by, zgemm_kernel<Tc, TA, TB, TC, TM, TN, true, true>, grid, dim3(256), s, g);
   else if (ak && !bn)
-    zgemm_kernel<Tc, TA, TB, TC, TM, TN, true, false><<<grid, 256, 0, s>>>(g);
+    tlaunch(nm, fl, by, zgemm_kernel<Tc, TA, TB, TC, TM, TN, true, false>, grid, dim3(256), s, g);
   else if (!ak && bn)
-    zgemm_kernel<Tc, TA, TB, TC, TM, TN, false, true><<<grid, 256, 0, s>>>(g);
+    tlaunch(nm, fl, by, zgemm_kernel<Tc, TA, TB, TC, TM, TN, false, true>, grid, dim3(256), s, g);
   else
-    zgemm_kernel<Tc, TA, TB, TC, TM, TN, false, false><<<grid, 256, 0, s>>>(g);
+    tlaunch(nm, fl, by, zgemm_kernel<Tc, TA, TB, TC, TM, TN, false, false>, grid, dim3(256), s, g);
   PTN_LAUNCH_CHECK();
-  if (slot >= 0)
-    timing_post(slot, s, gemm_kernel_name<Tc, TA, TB, TC, TM, TN>(ak, bn), gemm_flop(z),
-                gemm_bytes<TA, TB, TC>(z));
   return PTN_OK;
 }
 

@@ -788,18 +788,15 @@ static int jacobi_core(const SvdLayout& L, char* base, double2*& X, double2* V, 
     while (res.sweeps32 < max_sweeps32) {
       k_set_ints<<<1, 64, 0, st>>>(counters, 1, 0);
       for (int r = 0; r < L.nblk - 1; ++r) {
-        int slot = timing_pre(st);
-        k_jac_gram<float><<<dim3((unsigned)L.nseg, (unsigned)npairs, NB), 256, 0, st>>>(
-            L.q, L.nblk, r, X32, Gp32, (int)L.nseg, sx);
-        timing_post(slot, st, "k_jac_gram<float>", W.gram_flop, W.gram_bytes / 2);
-        slot = timing_pre(st);
-        k_jac_solve<float><<<dim3((unsigned)npairs, NB), 256, 0, st>>>(
-            L.nblk, Gp32, (int)L.nseg, tol32, big32, Jg32, rflag, counters);
-        timing_post(slot, st, "k_jac_solve<float>", W.solve_flop, W.solve_bytes / 2);
-        slot = timing_pre(st);
-        k_jac_apply<float><<<dim3((unsigned)aseg, (unsigned)npairs, 2 * NB), 256, 0, st>>>(
-            L.q, L.qp, L.nblk, r, X32, V32, Jg32, rflag, 2);
-        timing_post(slot, st, "k_jac_apply<float>", 2 * W.apply_flop, W.apply_bytes);
+        tlaunch("k_jac_gram<float>", W.gram_flop, W.gram_bytes / 2, k_jac_gram<float>,
+                dim3((unsigned)L.nseg, (unsigned)npairs, NB), dim3(256), st, (int64_t)L.q, L.nblk,
+                r, (const float2*)X32, Gp32, (int)L.nseg, sx);
+        tlaunch("k_jac_solve<float>", W.solve_flop, W.solve_bytes / 2, k_jac_solve<float>,
+                dim3((unsigned)npairs, NB), dim3(256), st, L.nblk, (const float2*)Gp32,
+                (int)L.nseg, tol32, big32, Jg32, rflag, counters);
+        tlaunch("k_jac_apply<float>", 2 * W.apply_flop, W.apply_bytes, k_jac_apply<float>,
+                dim3((unsigned)aseg, (unsigned)npairs, 2 * NB), dim3(256), st, (int64_t)L.q,
+                (int64_t)L.qp, L.nblk, r, X32, V32, (const float2*)Jg32, (const int*)rflag, 2);
       }
       PTN_LAUNCH_CHECK();
       PTN_HIP_CHECK(hipMemcpyAsync(host_rot, counters, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -847,18 +844,16 @@ static int jacobi_core(const SvdLayout& L, char* base, double2*& X, double2* V, 
   while (!res.converged && res.sweeps < max_sweeps) {
     k_set_ints<<<1, 64, 0, st>>>(counters, 1, 0);
     for (int r = 0; r < L.nblk - 1; ++r) {
-      int slot = timing_pre(st);
-      k_jac_gram<double><<<dim3((unsigned)L.nseg, (unsigned)npairs, NB), 256, 0, st>>>(
-          L.q, L.nblk, r, X, Gp, (int)L.nseg, sx);
-      timing_post(slot, st, "k_jac_gram<double>", W.gram_flop, W.gram_bytes);
-      slot = timing_pre(st);
-      k_jac_solve<double><<<dim3((unsigned)npairs, NB), 256, 0, st>>>(L.nblk, Gp, (int)L.nseg, tol,
-                                                                      big, Jg, rflag, counters);
-      timing_post(slot, st, "k_jac_solve<double>", W.solve_flop, W.solve_bytes);
-      slot = timing_pre(st);
-      k_jac_apply<double><<<dim3((unsigned)aseg, (unsigned)npairs, nmat * NB), 256, 0, st>>>(
-          L.q, L.qp, L.nblk, r, X, V, Jg, rflag, nmat);
-      timing_post(slot, st, "k_jac_apply<double>", nmat * W.apply_flop, nmat * W.apply_bytes);
+      tlaunch("k_jac_gram<double>", W.gram_flop, W.gram_bytes, k_jac_gram<double>,
+              dim3((unsigned)L.nseg, (unsigned)npairs, NB), dim3(256), st, (int64_t)L.q, L.nblk, r,
+              (const double2*)X, Gp, (int)L.nseg, sx);
+      tlaunch("k_jac_solve<double>", W.solve_flop, W.solve_bytes, k_jac_solve<double>,
+              dim3((unsigned)npairs, NB), dim3(256), st, L.nblk, (const double2*)Gp, (int)L.nseg,
+              tol, big, Jg, rflag, counters);
+      tlaunch("k_jac_apply<double>", nmat * W.apply_flop, nmat * W.apply_bytes,
+              k_jac_apply<double>, dim3((unsigned)aseg, (unsigned)npairs, nmat * NB), dim3(256), st,
+              (int64_t)L.q, (int64_t)L.qp, L.nblk, r, X, V, (const double2*)Jg, (const int*)rflag,
+              nmat);
     }
     PTN_LAUNCH_CHECK();
     PTN_HIP_CHECK(hipMemcpyAsync(host_rot, counters, sizeof(int), hipMemcpyDeviceToHost, st));

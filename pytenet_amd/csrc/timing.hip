@@ -1,9 +1,10 @@
-// Live launch timing of the complex products (ptn_timing_begin / ptn_timing_end).
+// Live launch timing of the complex products and the Jacobi SVD kernels
+// (ptn_timing_begin / ptn_timing_end).
 //
-// While timing is on, every complex-product kernel launch is bracketed by two
-// HIP events recorded on the stream it is launched on (the device clock of
-// that stream, nothing else), together with the algorithmic flops and bytes
-// of the launch. ptn_timing_end waits for the events and sums, per kernel
+// While timing is on, every such kernel is launched with a start and a stop
+// HIP event attached to its own dispatch (hipExtLaunchKernel), so the events
+// carry the kernel's begin and end on the device clock of its stream, together
+// with the algorithmic flops and bytes of the launch. ptn_timing_end waits for the events and sums, per kernel
 // instance, the launch count, the device time and the flops/bytes. Event
 // records cost a few microseconds of host time per launch and do not
 // synchronise anything, so a sweep can be timed with it switched on (beyond
@@ -76,7 +77,8 @@ void fold_front() {
 }
 }  // namespace
 
-int timing_pre(hipStream_t s) {
+int timing_acquire(hipStream_t s, hipEvent_t* e0, hipEvent_t* e1) {
+  *e0 = *e1 = nullptr;
   if (!g_on || noalloc_mode()) return -1;
   // launches recorded into a graph (stream capture) are not timed per launch
   hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
@@ -88,23 +90,24 @@ int timing_pre(hipStream_t s) {
   if (!g_on) return -1;
   while (g_pending.size() >= MAX_PENDING) fold_front();
   Rec r{nullptr, 0.0, 0.0, take_event(), take_event(), g_seq};
-  if (!r.e0 || !r.e1 || hipEventRecord(r.e0, s) != hipSuccess) {
+  if (!r.e0 || !r.e1) {
     if (r.e0) g_free.push_back(r.e0);
     if (r.e1) g_free.push_back(r.e1);
     return -1;
   }
   g_seq = (g_seq + 1) & 0x3fffffff;
   g_pending.push_back(r);
+  *e0 = r.e0;
+  *e1 = r.e1;
   return r.seq;
 }
 
-void timing_post(int slot, hipStream_t s, const char* name, double flop, double bytes) {
+void timing_commit(int slot, const char* name, double flop, double bytes) {
   if (slot < 0) return;
   std::lock_guard<std::mutex> lk(g_mu);
   // the record of this launch: normally the newest (searched from the back)
   for (auto it = g_pending.rbegin(); it != g_pending.rend(); ++it) {
     if (it->seq != slot) continue;
-    if (hipEventRecord(it->e1, s) != hipSuccess) return;   // left out of the sums
     it->name = name;
     it->flop = flop;
     it->bytes = bytes;
