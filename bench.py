@@ -503,8 +503,64 @@ def workload_svd(args):
     return out
 
 
-WORKLOADS = {"dmrg1": workload_dmrg1, "dmrg2-u1": workload_dmrg2_u1, "chains": workload_chains,
-             "svd": workload_svd}
+def workload_growth(args):
+    """The precision study of the two-site TDVP workload as specified: XXZ
+    L=100 (J=1, Delta=0.8, h=-0.1), seeded random MPS with bond 16,
+    left-orthonormalised, dt=-0.05j, 40 steps, numiter 8, tol_split 1e-10, the
+    bond cap D (default 256; the bonds grow from 16 towards it), run in FP64,
+    FP32 and split-bf16 contractions. Reports, per mode, the energy drift
+    |E(t) - E(0)| / |E(0)| after every step (real-time evolution conserves the
+    energy), and against the FP64 run the final |1 - fidelity| and relative
+    energy difference, plus the bond profile and wall time."""
+    import torch
+    import pytenet_amd as ptn
+    L, D, steps = args.L or 100, args.D or 256, args.sweeps or 40
+    nit = args.numiter or 8
+    H = ptn.heisenberg_xxz_1d_mpo(L, 1.0, 0.8, -0.1).zero_qnumbers()
+    bonds = [min(16, 2**min(i, 40), 2**min(L - i, 40)) for i in range(L + 1)]
+    psi0 = ptn.MPS(H.qsite, [np.zeros(b, dtype=int) for b in bonds], fill="random",
+                   rng=np.random.default_rng(args.seed))
+    psi0.to_device()
+    psi0.orthonormalize(mode="left")
+    e0 = ptn.mpo_average(psi0, H).real
+    out = {"workload": f"two-site TDVP, XXZ L={L}, bond 16 -> cap {D}, dt=-0.05j, {steps} steps, "
+                       f"numiter {nit}, tol 1e-10", "unit": "s", "E0": e0}
+    finals = {}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        psi = copy.deepcopy(psi0)
+        run = ptn.TwoSiteTDVP(H, psi, -0.05j, nit, 1e-10, precision=prec, max_bond=D)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run.setup()
+        drift = []
+        for k in range(steps):
+            run.sweep()
+            n = ptn.mps_norm(psi)
+            drift.append(abs(ptn.mpo_average(psi, H).real / n**2 - e0) / abs(e0))
+            if k % 5 == 4:
+                print(f"[bench] growth {prec}: step {k + 1}/{steps}, bond max {max(psi.bond_dims)}, "
+                      f"drift {drift[-1]:.2e}", file=sys.stderr, flush=True)
+        run.finish()
+        torch.cuda.synchronize()
+        out[f"{prec}_s"] = round(time.perf_counter() - t0, 2)
+        out[f"{prec}_energy_drift"] = [float(f"{x:.3e}") for x in drift[::4]] + [float(f"{drift[-1]:.3e}")]
+        out[f"{prec}_bond_max"] = int(max(psi.bond_dims))
+        finals[prec] = psi
+        print(f"[bench] growth {prec} done ({out[f'{prec}_s']} s)", file=sys.stderr, flush=True)
+    ref = finals["fp64"]
+    nr = ptn.mps_norm(ref)
+    er = ptn.mpo_average(ref, H).real / nr**2
+    for prec in ("fp32", "bf16x3"):
+        p = finals[prec]
+        npr = ptn.mps_norm(p)
+        out[f"{prec}_infidelity_vs_fp64"] = abs(1 - abs(ptn.mps_vdot(ref, p))**2 / (nr**2 * npr**2))
+        out[f"{prec}_energy_rel_err_vs_fp64"] = abs(ptn.mpo_average(p, H).real / npr**2 - er) / abs(er)
+    out["value"] = out["fp32_s"]
+    return out
+
+
+WORKLOADS = {"growth": workload_growth, "dmrg1": workload_dmrg1, "dmrg2-u1": workload_dmrg2_u1,
+             "chains": workload_chains, "svd": workload_svd}
 
 
 def main():
