@@ -137,7 +137,39 @@ typedef struct {
   const void* l;            /* L (Dl, Ml, Dl) */
   const void* w;            /* W (Ml, d, d, Mr); NULL for the bond operator */
   const void* r;            /* R (Dr, Mr, Dr) */
+  /* Optional U(1) quantum numbers (host int64 arrays; NULL = none; zero-
+   * initialise the struct when not used): left / right virtual bond (Dl / Dr
+   * labels), physical (d; two sites: the flattened pair sums,
+   * pytenet/block_sparse_util.py:40-44) and MPO virtual bonds (Ml / Mr), with
+   * the reference's sign conventions (A: q_l + q_s = q_r, pytenet/mps.py:71;
+   * W[m, s', s, m']: q_ml + q_s' = q_s + q_mr, pytenet/mpo.py:71,154; L, R:
+   * q + q_m = q', pytenet/tdvp.py:62). With PTN_BLOCK_SPARSE, PTN_FP64 or PTN_FP32, an
+   * H_eff operator, batch 1 and sector-contiguous virtual bonds (every label
+   * in one run of indices, as the reference's QR and SVD splits leave them),
+   * the two products of H_eff run as one grouped launch each over the
+   * admissible sector blocks only (the block descriptors are built on the
+   * first call with a given set of labels and cached; that first call copies
+   * them to the device synchronously). Otherwise the tile-mask products run.
+   * Results are the same either way: the grouped sums run over exactly the
+   * nonzero terms. The labels are read during the call only. */
+  const int64_t* q_l;
+  const int64_t* q_s;
+  const int64_t* q_r;
+  const int64_t* q_ml;
+  const int64_t* q_mr;
 } ptn_local_op;
+
+/* One application of a local operator (the matvec of the local solves):
+ * out = H_eff a (PTN_OP_HEFF) or the bond operator on a (PTN_OP_BOND), with
+ * the sector-grouped products when `op` carries quantum numbers as above.
+ * Workspace: ptn_heff_workspace / ptn_bond_workspace of the operator's
+ * precision and shapes. Replaces the matvec closures pytenet/tdvp.py:223-238
+ * and pytenet/dmrg.py:181-189 (one call of apply_local_hamiltonian,
+ * pytenet/chain_ops.py:237-279, or apply_local_bond_contraction,
+ * pytenet/chain_ops.py:282-317). *grouped (optional, host) receives 1 if the
+ * grouped products ran, else 0. */
+int ptn_local_apply(const ptn_local_op* op, const void* a, void* out, int32_t* grouped,
+                    void* ws, size_t ws_bytes, void* stream);
 
 size_t ptn_krylov_workspace(const ptn_local_op* op, int numiter);
 

@@ -41,7 +41,8 @@ class LocalOp(ctypes.Structure):
     """ptn_local_op (include/ptn.h)."""
     _fields_ = [("kind", c_int), ("precision", c_int),
                 ("Dl", c_i64), ("d", c_i64), ("Dr", c_i64), ("Ml", c_i64), ("Mr", c_i64),
-                ("l", c_p), ("w", c_p), ("r", c_p)]
+                ("l", c_p), ("w", c_p), ("r", c_p),
+                ("q_l", c_p), ("q_s", c_p), ("q_r", c_p), ("q_ml", c_p), ("q_mr", c_p)]
 
 
 class KernelTiming(ctypes.Structure):
@@ -60,6 +61,7 @@ SIGNATURES = [
                                c_p, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
     ("ptn_bond_workspace", c_sz, [c_int, c_i64, c_i64, c_i64]),
     ("ptn_bond_apply", c_int, [c_int, c_i64, c_i64, c_i64, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
+    ("ptn_local_apply", c_int, [ctypes.POINTER(LocalOp), c_p, c_p, c_p, c_p, c_sz, c_p]),
     ("ptn_env_workspace", c_sz, [c_int, c_i64, c_i64, c_i64, c_i64, c_i64]),
     ("ptn_env_left", c_int, [c_int, c_i64, c_i64, c_i64, c_i64, c_i64,
                              c_p, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),

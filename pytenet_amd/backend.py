@@ -124,6 +124,27 @@ def heff_apply(a, w, l, r, precision="fp64", block_sparse=False):
     return out
 
 
+def heff_apply_sectors(a, w, l, r, qnums, precision="fp64"):
+    """H_eff applied once through ptn_local_apply with U(1) quantum numbers
+    qnums = (q_l, q_s, q_r, q_ml, q_mr): the sector-grouped products when
+    they apply (sector-contiguous virtual bonds, FP64 / FP32), else the
+    tile-mask products. Returns (out, grouped)."""
+    a, w, l, r = as_dev(a), as_dev(w), as_dev(l), as_dev(r)
+    Dl, d, Dr = a.shape
+    Ml, Mr = w.shape[0], w.shape[3]
+    _shape_check(w.shape == (Ml, d, d, Mr) and l.shape == (Dl, Ml, Dl) and r.shape == (Dr, Mr, Dr),
+                 "apply_local_hamiltonian", a, w, l, r)
+    op, keep = _local_op(_lib.PTN_OP_HEFF, precision, l, w, r, Dl, d, Dr, True, qnums)
+    lib = _lib.load()
+    out = torch.empty((Dl, d, Dr), dtype=torch.complex128, device=a.device)
+    ws = workspace(lib.ptn_heff_workspace(op.precision, Dl, d, Dr, Ml, Mr))
+    grouped = ctypes.c_int32(0)
+    _check(lib.ptn_local_apply(ctypes.byref(op), ptr(a), ptr(out), ctypes.addressof(grouped),
+                               ptr(ws), ws.numel(), stream()), "ptn_local_apply")
+    del keep
+    return out, bool(grouped.value)
+
+
 def bond_apply(c, l, r, precision="fp64", block_sparse=False):
     """out[lb,rb] = sum L[l,m,lb] C[l,r] R[r,m,rb]."""
     c, l, r = as_dev(c), as_dev(l), as_dev(r)
@@ -210,7 +231,9 @@ def merge_pair(a0, a1):
 # ---------------------------------------------------------------------------
 # Krylov solvers (K8-K10)
 
-def _local_op(kind, precision, l, w, r, Dl, d, Dr, block_sparse=False):
+def _local_op(kind, precision, l, w, r, Dl, d, Dr, block_sparse=False, qnums=None):
+    """ptn_local_op of a local operator, and the host label arrays it points to
+    (keep them alive for the call). qnums = (q_l, q_s, q_r, q_ml, q_mr) or None."""
     op = _lib.LocalOp()
     op.kind = kind
     op.precision = precision_code(precision, block_sparse)
@@ -220,15 +243,23 @@ def _local_op(kind, precision, l, w, r, Dl, d, Dr, block_sparse=False):
     op.l = l.data_ptr()
     op.w = w.data_ptr() if w is not None else None
     op.r = r.data_ptr()
-    return op
+    keep = None
+    if qnums is not None and block_sparse and kind == _lib.PTN_OP_HEFF:
+        keep = [np.ascontiguousarray(np.asarray(q, dtype=np.int64).reshape(-1)) for q in qnums]
+        want = (Dl, d, Dr, op.Ml, op.Mr)
+        assert tuple(len(q) for q in keep) == want, "quantum numbers do not match the shapes"
+        op.q_l, op.q_s, op.q_r, op.q_ml, op.q_mr = (q.ctypes.data for q in keep)
+    return op, keep
 
 
 def krylov_local(kind, v, l, w, r, numiter, mode, dt=0.0, precision="fp64", info=None,
-                 block_sparse=False):
+                 block_sparse=False, qnums=None):
     """Lanczos on the local operator (PTN_OP_HEFF or PTN_OP_BOND):
     mode "expm" -> expm(dt*H) v; mode "eigh" -> (lowest Ritz vector, energy tensor).
     `info` (int32[4] device tensor) receives the status words; returns
-    (out, energy or None, info)."""
+    (out, energy or None, info). With block_sparse and the U(1) labels
+    qnums = (q_l, q_s, q_r, q_ml, q_mr) of an H_eff operator the matvecs use
+    the sector-grouped products where they apply."""
     v, l, r = as_dev(v), as_dev(l), as_dev(r)
     if w is not None:
         w = as_dev(w)
@@ -239,7 +270,7 @@ def krylov_local(kind, v, l, w, r, numiter, mode, dt=0.0, precision="fp64", info
         Dl, d, Dr = v.shape
     else:
         (Dl, Dr), d = v.shape, 1
-    op = _local_op(kind, precision, l, w, r, Dl, d, Dr, block_sparse)
+    op, keep = _local_op(kind, precision, l, w, r, Dl, d, Dr, block_sparse, qnums)
     lib = _lib.load()
     nb = lib.ptn_krylov_workspace(ctypes.byref(op), int(numiter))
     ws = workspace(nb)

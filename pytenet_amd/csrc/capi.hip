@@ -169,6 +169,47 @@ int ptn_bond_apply(int precision, int64_t Dl, int64_t Dr, int64_t M, const void*
                     PTN_T_C128, ws, base, s);
 }
 
+int ptn_local_apply(const ptn_local_op* op, const void* a, void* out, int32_t* grouped, void* ws,
+                    size_t ws_bytes, void* stream) {
+  if (grouped) *grouped = 0;
+  if (!op) return PTN_ERR_ARG;
+  if (op->kind == PTN_OP_BOND) {
+    if (op->Ml != op->Mr) return PTN_ERR_ARG;
+    return ptn_bond_apply(op->precision, op->Dl, op->Dr, op->Ml, op->l, op->r, a, out, ws,
+                          ws_bytes, stream);
+  }
+  if (op->kind != PTN_OP_HEFF) return PTN_ERR_ARG;
+  const SectorPlan* sp = nullptr;
+  int st;
+  if ((st = sector_plan_get(op, &sp))) return st;
+  if (!sp)
+    return ptn_heff_apply(op->precision, op->Dl, op->d, op->Dr, op->Ml, op->Mr, op->l, op->w,
+                          op->r, a, out, ws, ws_bytes, stream);
+  // a plan exists only for valid shapes with Dl, Dr >= 1
+  const Dims D = {op->Dl, op->d, op->Dr, op->Ml, op->Mr};
+  if (op->d <= 0 || op->d * op->Mr > 32 || op->Ml * op->d > 32) return PTN_ERR_ARG;
+  if (ws_bytes < ptn_heff_workspace(op->precision, op->Dl, op->d, op->Dr, op->Ml, op->Mr))
+    return PTN_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t base = al256(heff_ws_bytes(op->precision, D));
+  const void* l = op->l;
+  const void* r = op->r;
+  int tl = PTN_T_C128;
+  if (lowp(op->precision)) {
+    char* l64 = (char*)ws + base;
+    char* r64 = l64 + env64_bytes(op->Dl, op->Ml);
+    if ((st = convert_c128_c64(op->Dl * op->Ml * op->Dl, op->l, l64, s))) return st;
+    if ((st = convert_c128_c64(op->Dr * op->Mr * op->Dr, op->r, r64, s))) return st;
+    l = l64;
+    r = r64;
+    tl = PTN_T_C64;
+  }
+  if ((st = heff_apply_sectors(op->precision, D, sp, l, tl, op->w, r, tl, a, out, ws, base, s)))
+    return st;
+  if (grouped) *grouped = 1;
+  return PTN_OK;
+}
+
 static size_t env_workspace_b(int precision, int64_t batch, int64_t Dl, int64_t d, int64_t Dr,
                               int64_t Ml, int64_t Mr) {
   Dims D = {Dl, d, Dr, Ml, Mr};

@@ -1,0 +1,506 @@
+// Sector-grouped H_eff for U(1)-symmetric tensors: the two products of
+// H_eff computed over the admissible quantum-number blocks only, each as ONE
+// launch over all of its blocks.
+//
+// The tensors keep the reference's dense layout with zero blocks
+// (pytenet/mps.py:71, pytenet/block_sparse_util.py:56-64). With quantum
+// numbers ql (left virtual bond), qs (physical), qr (right virtual bond), qml /
+// qmr (MPO virtual bonds) and the sign conventions of the reference's
+// sparsity checks (pytenet/mps.py:71, pytenet/mpo.py:71,154,
+// pytenet/tdvp.py:62) the nonzero entries are
+//   A[l, s, r]          ql[l] + qs[s] = qr[r]
+//   W[m, s', s, m']     qml[m] + qs[s'] = qs[s] + qmr[m']
+//   R[r, m', r']        qr[r] + qmr[m'] = qr[r']
+//   L[l, m, lb]         ql[l] + qml[m] = ql[lb]
+// so that
+//   T1[(l, s), (m', r')] = sum_r A[l, s, r] R[r, m', r'] is nonzero only where
+//     ql[l] + qs[s] + qmr[m'] = qr[r'], and for fixed (s, m') and right
+//     sector Q of r it is the dense block product
+//       rows l in sector Q - qs[s], k = r in sector Q, columns r' in sector Q + qmr[m'];
+//   out[lb, (s', r')] = sum_{(l, m)} L[l, m, lb] T2[(l, m), (s', r')] is nonzero only where
+//     ql[lb] + qs[s'] = qr[r'], and for left sector P of lb and fixed s' it is
+//       rows lb in P, columns r' in sector P + qs[s'], k over the segments
+//       (l in sector P - qml[m], m), one segment per MPO index m.
+// Every output block belongs to exactly one product block (its sector fixes
+// the k-range), so no two workgroups write the same entry and no block needs
+// accumulation. The entries outside the blocks are structurally zero and are
+// written by one fill pass; the sums inside a block run over exactly the
+// reference's nonzero terms. The MPO mode mix between the products stays
+// dense (it is HBM-bound and reads T1 once).
+//
+// Launch: the grid covers the tiles of all blocks; a workgroup finds its block
+// by binary search over the blocks' first tiles (descriptors in device memory,
+// built once per distinct set of quantum numbers and kept in a cache).
+// Requires the two virtual bonds to be sector-contiguous (each label in one
+// run of indices), which the reference's QR and SVD splits produce
+// (pytenet/block_sparse_util.py:106-180, 244-319); otherwise the tile-mask
+// products are used.
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+#include "gemm_impl.h"
+#include "ops.h"
+
+namespace ptn {
+
+constexpr int GMAXSEG = 16;   // k-segments per block (MPO bond dimension at most 16)
+
+struct GBlock {
+  int64_t M, N;                 // block rows / columns (complex elements)
+  int64_t c_off;                // element offset of the block in C
+  int64_t tile0;                // first linear tile of the block in the launch
+  int32_t tiles_n, nseg;
+  int64_t a_off[GMAXSEG], b_off[GMAXSEG], K[GMAXSEG];
+};
+
+struct GLaunch {
+  const GBlock* blocks;
+  int nblocks;
+  int64_t a_m, a_k, b_k, b_n, c_m, c_n;   // strides shared by all blocks
+  const void* A;
+  const void* B;
+  void* C;
+  const int* skip;
+};
+
+// One workgroup = one BM x BN tile of one block; the MFMA tile body is the one
+// of zgemm_kernel (real embedding of the complex product, double-buffered LDS
+// stages), with k running over the block's segments.
+template <typename Tc, typename TA, typename TB, typename TC, int TM, int TN, bool A_KCONTIG,
+          bool B_NCONTIG>
+__global__ __launch_bounds__(256) void zgemm_grouped_kernel(GLaunch g) {
+  typedef TileCfg<TM, TN, StageK<Tc>::value> T;
+  constexpr int BM = T::BM, BN = T::BN, BK = T::BK, NA = T::NA, NB = T::NB;
+  typedef typename acc4<Tc>::type acc_t;
+  __shared__ Tc As[2][2 * BK][BM + 1];
+  __shared__ Tc Bs[2][BK][2 * BN + 2];
+  if (g.skip && g.skip[0]) return;
+  // the block of this workgroup: the last block whose first tile <= blockIdx.x
+  const int64_t tile = blockIdx.x;
+  int lo = 0, hi = g.nblocks - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (g.blocks[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
+  }
+  const GBlock* __restrict__ bk = g.blocks + lo;
+  const int64_t bM = bk->M, bN = bk->N;
+  const int nseg = bk->nseg;
+  const int64_t lt = tile - bk->tile0;
+  const int64_t m0 = (lt / bk->tiles_n) * BM, n0 = (lt % bk->tiles_n) * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave >> 1) * T::WM, wn0 = (wave & 1) * T::WN;
+  const TA* __restrict__ A = (const TA*)g.A;
+  const TB* __restrict__ B = (const TB*)g.B;
+  TC* __restrict__ C = (TC*)g.C + bk->c_off;
+  int nst = 0;
+  for (int sg = 0; sg < nseg; ++sg) nst += (int)((bk->K[sg] + BK - 1) / BK);
+
+  Tc ra[NA][2], rb[NB][2];
+  // stage `it` of the concatenated segments: segment sg, k offset k0 in it
+  auto load_tiles = [&](int it) {
+    int sg = 0;
+    int64_t k0 = (int64_t)it * BK;
+    for (; sg < nseg - 1; ++sg) {
+      const int64_t ks = ((bk->K[sg] + BK - 1) / BK) * BK;
+      if (k0 < ks) break;
+      k0 -= ks;
+    }
+    const TA* Ap = A + bk->a_off[sg];
+    const TB* Bp = B + bk->b_off[sg];
+    const int64_t K = bk->K[sg];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int e = tid + 256 * i;
+      int row, kc;
+      if (A_KCONTIG) { row = e / BK; kc = e % BK; } else { kc = e / BM; row = e % BM; }
+      const int64_t m = m0 + row, k = k0 + kc;
+      Tc re = 0, im = 0;
+      if (m < bM && k < K) load_c<Tc>(Ap + m * g.a_m + k * g.a_k, re, im);
+      ra[i][0] = re; ra[i][1] = im;
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int e = tid + 256 * i;
+      int col, kc;
+      if (B_NCONTIG) { kc = e / BN; col = e % BN; } else { col = e / BK; kc = e % BK; }
+      const int64_t n = n0 + col, k = k0 + kc;
+      Tc re = 0, im = 0;
+      if (n < bN && k < K) load_c<Tc>(Bp + k * g.b_k + n * g.b_n, re, im);
+      rb[i][0] = re; rb[i][1] = im;
+    }
+  };
+  auto store_tiles = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int e = tid + 256 * i;
+      int row, kc;
+      if (A_KCONTIG) { row = e / BK; kc = e % BK; } else { kc = e / BM; row = e % BM; }
+      As[buf][2 * kc][row] = ra[i][0];
+      As[buf][2 * kc + 1][row] = ra[i][1];
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int e = tid + 256 * i;
+      int col, kc;
+      if (B_NCONTIG) { kc = e / BN; col = e % BN; } else { col = e / BK; kc = e % BK; }
+      Bs[buf][kc][2 * col] = rb[i][0];
+      Bs[buf][kc][2 * col + 1] = rb[i][1];
+    }
+  };
+
+  acc_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+  // expanded B operand per lane: real k parity (pa), real column parity (pb)
+  const int kk = lane >> 4;
+  const int pa = kk & 1, pb = lane & 1;
+  const bool take_im = (pa != pb);
+  const Tc im_sign = (pa == 1) ? (Tc)-1 : (Tc)1;
+  int buf = 0;
+  if (nst > 0) {
+    load_tiles(0);
+    store_tiles(0);
+  }
+  __syncthreads();
+  for (int it = 0; it < nst; ++it) {
+    const bool more = it + 1 < nst;
+    if (more) load_tiles(it + 1);
+#pragma unroll
+    for (int ks = 0; ks < BK / 2; ++ks) {
+      Tc av[TM], bv[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) av[i] = As[buf][4 * ks + kk][wm0 + 16 * i + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const Tc* p = &Bs[buf][2 * ks + (kk >> 1)][2 * (wn0 + 8 * j + ((lane & 15) >> 1))];
+        const Tc re = p[0], im = p[1];
+        bv[j] = take_im ? im_sign * im : re;
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = mfma16(av[i], bv[j], acc[i][j]);
+    }
+    if (more) store_tiles(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  // epilogue as zgemm_kernel: the partner lane (lane ^ 1) holds the other part
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      Tc mine[4], other[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        mine[v] = acc[i][j][v];
+        other[v] = __shfl_xor(mine[v], 1);
+      }
+      const int64_t n = n0 + wn0 + 8 * j + ((lane & 15) >> 1);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int v = 2 * pb + h;
+        const int64_t m = m0 + wm0 + 16 * i + acc4<Tc>::row(kk, v);
+        const double re = pb ? (double)other[2 + h] : (double)mine[h];
+        const double im = pb ? (double)mine[2 + h] : (double)other[h];
+        if (m < bM && n < bN) store_c(C + m * g.c_m + n * g.c_n, re, im);
+      }
+    }
+  }
+}
+
+// zero fill of n complex elements (the structurally zero entries of T1 / out)
+template <typename TC>
+__global__ void k_zero_fill(int64_t n, TC* __restrict__ p, const int* __restrict__ skip) {
+  if (skip && skip[0]) return;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    store_c(p + t, 0.0, 0.0);
+}
+
+template <typename TC>
+static int zero_fill(int64_t n, void* p, hipStream_t s) {
+  if (n <= 0) return PTN_OK;
+  int64_t b = (n + 255) / 256;
+  if (b > 4096) b = 4096;
+  tlaunch("k_zero_fill", 0.0, (double)n * sizeof(TC), k_zero_fill<TC>, dim3((unsigned)b),
+          dim3(256), s, n, (TC*)p, skip_flag().p);
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plans: the blocks of both products for one set of quantum numbers
+
+struct Sector { int64_t start, len; };
+
+// runs of equal labels of a bond; false if a label has two runs
+static bool sectors_of(const int64_t* q, int64_t n, std::map<int64_t, Sector>& out) {
+  out.clear();
+  for (int64_t i = 0; i < n;) {
+    int64_t j = i;
+    while (j < n && q[j] == q[i]) ++j;
+    if (out.count(q[i])) return false;
+    out[q[i]] = Sector{i, j - i};
+    i = j;
+  }
+  return true;
+}
+
+struct GProduct {
+  std::vector<GBlock> blocks;
+  int64_t tiles = 0;
+  int big = 0;          // 1: 128 x 64 tiles (TileCfg<4, 4>), 0: 64 x 32 (TileCfg<2, 2>)
+  double flop = 0.0;    // 8 M N K summed over the blocks
+  double bytes = 0.0;   // operand blocks read once, output blocks written once
+};
+
+struct SectorPlan {
+  GProduct p1, p3;
+  GBlock* dev = nullptr;   // p1's blocks, then p3's
+  int device = 0;
+  uint64_t stamp = 0;
+};
+
+// tile shape and first tiles of a product: the 128 x 64 tile when its padding
+// costs at most 15 % more work than the 64 x 32 tile's and it still gives
+// enough workgroups for the chip (>= 512)
+static void finish_product(GProduct& P, size_t csz_a, size_t csz_b, size_t csz_c) {
+  double w_small = 0.0, w_big = 0.0;
+  int64_t t_big = 0;
+  P.flop = 0.0;
+  P.bytes = 0.0;
+  for (const GBlock& b : P.blocks) {
+    double k = 0.0;
+    for (int i = 0; i < b.nseg; ++i) k += (double)b.K[i];
+    P.flop += 8.0 * (double)b.M * (double)b.N * k;
+    P.bytes += k * ((double)b.M * csz_a + (double)b.N * csz_b) + (double)b.M * b.N * csz_c;
+    w_small += (double)((b.M + 63) / 64 * 64) * (double)((b.N + 31) / 32 * 32) * k;
+    w_big += (double)((b.M + 127) / 128 * 128) * (double)((b.N + 63) / 64 * 64) * k;
+    t_big += ((b.M + 127) / 128) * ((b.N + 63) / 64);
+  }
+  P.big = (w_big <= 1.15 * w_small && t_big >= 512) ? 1 : 0;
+  const int64_t BM = P.big ? 128 : 64, BN = P.big ? 64 : 32;
+  int64_t tiles = 0;
+  for (GBlock& b : P.blocks) {
+    b.tiles_n = (int32_t)((b.N + BN - 1) / BN);
+    b.tile0 = tiles;
+    tiles += ((b.M + BM - 1) / BM) * b.tiles_n;
+  }
+  P.tiles = tiles;
+}
+
+// blocks of both products; false when a bond is not sector-contiguous, the MPO
+// bond exceeds the segment limit or the launch would be too large
+static bool build_plan(const Dims& D, const int64_t* ql, const int64_t* qs, const int64_t* qr,
+                       const int64_t* qml, const int64_t* qmr, size_t csz, SectorPlan& sp) {
+  if (D.Ml > GMAXSEG || D.Dl <= 0 || D.Dr <= 0) return false;
+  std::map<int64_t, Sector> sl, sr;
+  if (!sectors_of(ql, D.Dl, sl) || !sectors_of(qr, D.Dr, sr)) return false;
+  // T1[(l, s), (m', r')] = sum_r A[(l, s), r] R[r, (m', r')]
+  for (int64_t s = 0; s < D.d; ++s)
+    for (const auto& kq : sr) {
+      const int64_t Q = kq.first;
+      const auto itl = sl.find(Q - qs[s]);
+      if (itl == sl.end()) continue;
+      for (int64_t mp = 0; mp < D.Mr; ++mp) {
+        const auto itc = sr.find(Q + qmr[mp]);
+        if (itc == sr.end()) continue;
+        GBlock b{};
+        b.M = itl->second.len;
+        b.N = itc->second.len;
+        const int64_t l0 = itl->second.start, r0 = kq.second.start, c0 = itc->second.start;
+        b.c_off = (l0 * D.d + s) * (D.Mr * D.Dr) + mp * D.Dr + c0;
+        b.nseg = 1;
+        b.a_off[0] = (l0 * D.d + s) * D.Dr + r0;
+        b.b_off[0] = r0 * (D.Mr * D.Dr) + mp * D.Dr + c0;
+        b.K[0] = kq.second.len;
+        sp.p1.blocks.push_back(b);
+      }
+    }
+  // out[lb, (s', r')] = sum_{(l, m)} L[(l, m), lb] T2[(l, m), (s', r')]
+  for (const auto& pq : sl) {
+    const int64_t P = pq.first, lb0 = pq.second.start;
+    for (int64_t sp2 = 0; sp2 < D.d; ++sp2) {
+      const auto itc = sr.find(P + qs[sp2]);
+      if (itc == sr.end()) continue;
+      GBlock b{};
+      b.M = pq.second.len;
+      b.N = itc->second.len;
+      const int64_t c0 = itc->second.start;
+      b.c_off = lb0 * (D.d * D.Dr) + sp2 * D.Dr + c0;
+      int ns = 0;
+      for (int64_t m = 0; m < D.Ml; ++m) {
+        const auto itl = sl.find(P - qml[m]);
+        if (itl == sl.end()) continue;
+        const int64_t l0 = itl->second.start;
+        b.a_off[ns] = (l0 * D.Ml + m) * D.Dl + lb0;
+        b.b_off[ns] = (l0 * D.Ml + m) * (D.d * D.Dr) + sp2 * D.Dr + c0;
+        b.K[ns] = itl->second.len;
+        ++ns;
+      }
+      if (ns == 0) continue;   // structurally zero output block
+      b.nseg = ns;
+      sp.p3.blocks.push_back(b);
+    }
+  }
+  finish_product(sp.p1, 16, csz, csz);   // A is complex128, R and T1 in the compute type
+  finish_product(sp.p3, csz, csz, 16);   // out is complex128
+  return sp.p1.tiles > 0 && sp.p3.tiles > 0 && sp.p1.tiles < 0x7fffffff &&
+         sp.p3.tiles < 0x7fffffff;
+}
+
+namespace {
+std::mutex g_plan_mu;
+std::unordered_map<std::string, std::unique_ptr<SectorPlan>> g_plans;   // nullptr: not applicable
+uint64_t g_plan_clock = 0;
+constexpr size_t PLAN_CACHE_MAX = 1024;
+}  // namespace
+
+static void append(std::string& k, const int64_t* p, int64_t n) {
+  k.append((const char*)&n, sizeof(n));
+  k.append((const char*)p, (size_t)n * sizeof(int64_t));
+}
+
+// The sector plan of a local operator with quantum numbers (null when the
+// operator has none, is not H_eff, or the grouped products do not apply).
+// Builds and uploads it on first use (synchronous copy: call it outside
+// stream capture); plans are cached per device and set of quantum numbers.
+int sector_plan_get(const ptn_local_op* op, const SectorPlan** out) {
+  *out = nullptr;
+  if (!op || op->kind != PTN_OP_HEFF || !(op->precision & PTN_BLOCK_SPARSE) || !op->q_l ||
+      !op->q_s || !op->q_r || !op->q_ml || !op->q_mr)
+    return PTN_OK;
+  const int bp = op->precision & ~PTN_BLOCK_SPARSE;
+  if (bp != PTN_FP64 && bp != PTN_FP32) return PTN_OK;   // split-bf16: the tile-mask products
+  if (noalloc_mode()) return PTN_OK;
+  int dev = 0;
+  PTN_HIP_CHECK(hipGetDevice(&dev));
+  std::string key;
+  const int64_t head[7] = {dev, bp, op->Dl, op->d, op->Dr, op->Ml, op->Mr};
+  key.append((const char*)head, sizeof(head));
+  append(key, op->q_l, op->Dl);
+  append(key, op->q_s, op->d);
+  append(key, op->q_r, op->Dr);
+  append(key, op->q_ml, op->Ml);
+  append(key, op->q_mr, op->Mr);
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  auto it = g_plans.find(key);
+  if (it != g_plans.end()) {
+    if (it->second) it->second->stamp = ++g_plan_clock;
+    *out = it->second.get();
+    return PTN_OK;
+  }
+  if (g_plans.size() >= PLAN_CACHE_MAX) {
+    // drop the older half of this device's plans (hipFree waits for the
+    // device, so no launch still reads them)
+    std::vector<std::pair<uint64_t, std::string>> age;
+    for (auto& kv : g_plans)
+      if (!kv.second || kv.second->device == dev)
+        age.emplace_back(kv.second ? kv.second->stamp : 0, kv.first);
+    std::sort(age.begin(), age.end());
+    for (size_t i = 0; i < age.size() / 2; ++i) {
+      auto e = g_plans.find(age[i].second);
+      if (e->second && e->second->dev) (void)hipFree(e->second->dev);
+      g_plans.erase(e);
+    }
+  }
+  std::unique_ptr<SectorPlan> sp(new SectorPlan());
+  const Dims D = {op->Dl, op->d, op->Dr, op->Ml, op->Mr};
+  if (!build_plan(D, op->q_l, op->q_s, op->q_r, op->q_ml, op->q_mr, bp == PTN_FP64 ? 16 : 8,
+                  *sp)) {
+    g_plans.emplace(key, nullptr);
+    return PTN_OK;
+  }
+  std::vector<GBlock> all(sp->p1.blocks);
+  all.insert(all.end(), sp->p3.blocks.begin(), sp->p3.blocks.end());
+  hipError_t e = hipMalloc((void**)&sp->dev, all.size() * sizeof(GBlock));
+  if (e == hipSuccess) e = hipMemcpy(sp->dev, all.data(), all.size() * sizeof(GBlock),
+                                     hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (sp->dev) (void)hipFree(sp->dev);
+    ptn_set_last_hip_error(e);
+    return PTN_ERR_HIP;
+  }
+  sp->device = dev;
+  sp->stamp = ++g_plan_clock;
+  *out = sp.get();
+  g_plans.emplace(key, std::move(sp));
+  return PTN_OK;
+}
+
+template <typename Tc, typename TA, typename TB, typename TC, bool AK, bool BNC>
+static int grouped_launch(const GLaunch& g, const GProduct& P, hipStream_t s) {
+  char name[112];
+  snprintf(name, sizeof(name), "zgemm_grouped_kernel<%s, %s, %s, %s, %d, %d, %s, %s>",
+           type_tag<Tc>::s, type_tag<TA>::s, type_tag<TB>::s, type_tag<TC>::s, P.big ? 4 : 2,
+           P.big ? 4 : 2, AK ? "true" : "false", BNC ? "true" : "false");
+  // launch records keep the name pointer: intern the names
+  static std::mutex mu;
+  static std::unordered_map<std::string, std::string> names;
+  const char* nm;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = names.find(name);
+    if (it == names.end()) it = names.emplace(name, name).first;
+    nm = it->second.c_str();
+  }
+  if (P.big)
+    tlaunch(nm, P.flop, P.bytes, zgemm_grouped_kernel<Tc, TA, TB, TC, 4, 4, AK, BNC>,
+            dim3((unsigned)P.tiles), dim3(256), s, g);
+  else
+    tlaunch(nm, P.flop, P.bytes, zgemm_grouped_kernel<Tc, TA, TB, TC, 2, 2, AK, BNC>,
+            dim3((unsigned)P.tiles), dim3(256), s, g);
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+// H_eff through the grouped products: out = L^T . (W (x) (A . R)) as
+// heff_apply computes it (same operand types: A and out complex128, L and R
+// complex128 in FP64 and complex64 in FP32), T1 and out zero-filled and only
+// the blocks computed. Uses T1 and T2 of the heff_apply workspace.
+int heff_apply_sectors(int prec, const Dims& D, const SectorPlan* sp, const void* l, int tl,
+                       const void* w, const void* r, int tr, const void* a, void* out, void* ws,
+                       size_t ws_bytes, hipStream_t s) {
+  const int bp = prec & ~PTN_BLOCK_SPARSE;
+  if (!sp || (bp != PTN_FP64 && bp != PTN_FP32)) return PTN_ERR_UNSUPPORTED;
+  const int ct = bp == PTN_FP64 ? PTN_T_C128 : PTN_T_C64;
+  if (tl != ct || tr != ct) return PTN_ERR_UNSUPPORTED;
+  if (ws_bytes < heff_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;
+  const size_t csz = bp == PTN_FP64 ? 16 : 8;
+  const int64_t n1 = (D.Dl * D.d) * (D.Mr * D.Dr), nout = D.Dl * D.d * D.Dr;
+  void* t1 = ws;
+  void* t2 = (char*)ws + (((size_t)n1 * csz + 255) & ~(size_t)255);
+  const GBlock* b1 = sp->dev;
+  const GBlock* b3 = sp->dev + sp->p1.blocks.size();
+  const int* skip = skip_flag().p;
+  int st;
+  GLaunch g1{b1, (int)sp->p1.blocks.size(), D.d * D.Dr, 1, D.Mr * D.Dr, 1, D.d * D.Mr * D.Dr, 1,
+             a, r, t1, skip};
+  if (bp == PTN_FP64) {
+    if ((st = zero_fill<double2>(n1, t1, s))) return st;
+    if ((st = grouped_launch<double, double2, double2, double2, true, true>(g1, sp->p1, s)))
+      return st;
+  } else {
+    if ((st = zero_fill<float2>(n1, t1, s))) return st;
+    if ((st = grouped_launch<float, double2, float2, float2, true, true>(g1, sp->p1, s)))
+      return st;
+  }
+  if ((st = mode_mix(D.Dl, D.d * D.Mr, D.Ml * D.d, D.Dr, w, t1, ct, t2, ct, s, 1,
+                     D.Ml * D.d * D.d * D.Mr)))
+    return st;
+  if ((st = zero_fill<double2>(nout, out, s))) return st;
+  GLaunch g3{b3, (int)sp->p3.blocks.size(), 1, D.Ml * D.Dl, D.Ml * D.d * D.Dr, 1, D.d * D.Dr, 1,
+             l, t2, out, skip};
+  if (bp == PTN_FP64)
+    return grouped_launch<double, double2, double2, double2, false, true>(g3, sp->p3, s);
+  return grouped_launch<float, float2, float2, double2, false, true>(g3, sp->p3, s);
+}
+
+}  // namespace ptn

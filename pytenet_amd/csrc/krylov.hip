@@ -440,7 +440,7 @@ size_t krylov_ws_bytes(const ptn_local_op* op, int numiter, int64_t batch) {
 static int krylov_enqueue(const ptn_local_op* op, int mode, const void* v, double dt_re,
                           double dt_im, int numiter, void* out, double* energy, int32_t* info,
                           double* alpha_beta, void* ws, size_t ws_bytes, hipStream_t s,
-                          int64_t batch);
+                          int64_t batch, const SectorPlan* sp);
 
 // ---------------------------------------------------------------------------
 // hipGraph capture of the local solves. A solve whose vectors are short is
@@ -461,6 +461,7 @@ bool g_graphs_on = true;
 constexpr int64_t GRAPH_MAX_N = 1 << 20;   // vector length up to which solves are captured
 constexpr int GRAPH_RING = 3;
 constexpr size_t GRAPH_MAX_BUCKETS = 512;
+constexpr int GRAPH_GROUPED = 1 << 20;   // key bit: matvecs through the sector-grouped products
 struct GKey {
   int dev, kind, prec, mode, numiter;
   int64_t Dl, d, Dr, Ml, Mr, batch;
@@ -521,11 +522,19 @@ int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, do
     use_graph = g_graphs_on && n * batch <= GRAPH_MAX_N;
   }
   hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
-  if (use_graph && (hipStreamIsCapturing(s, &cst) != hipSuccess || cst != hipStreamCaptureStatusNone))
+  if (hipStreamIsCapturing(s, &cst) != hipSuccess) cst = hipStreamCaptureStatusActive;
+  if (cst != hipStreamCaptureStatusNone)
     use_graph = false;   // the caller captures (or the query failed): launch into its capture
+  // the sector plan (quantum-number blocks) is built outside any capture; under
+  // the caller's capture only a cached plan can be used
+  const SectorPlan* sp = nullptr;
+  if (batch == 1 && cst == hipStreamCaptureStatusNone) {
+    const int e = sector_plan_get(op, &sp);
+    if (e) return e;
+  }
   if (!use_graph)
     return krylov_enqueue(op, mode, v, dt_re, dt_im, numiter, out, energy, info, alpha_beta, ws,
-                          ws_bytes, s, batch);
+                          ws_bytes, s, batch, sp);
   int dev = 0;
   PTN_HIP_CHECK(hipGetDevice(&dev));
   hipStream_t cap;
@@ -541,7 +550,7 @@ int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, do
   }
   PTN_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
   const int st = krylov_enqueue(op, mode, v, dt_re, dt_im, numiter, out, energy, info, alpha_beta,
-                                ws, ws_bytes, cap, batch);
+                                ws, ws_bytes, cap, batch, sp);
   hipGraph_t g = nullptr;
   const hipError_t ec = hipStreamEndCapture(cap, &g);
   if (st) {
@@ -549,7 +558,7 @@ int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, do
     return st;
   }
   if (ec != hipSuccess) { ptn_set_last_hip_error(ec); return PTN_ERR_HIP; }
-  const GKey key{dev, op->kind, op->precision, mode, numiter,
+  const GKey key{dev, op->kind, op->precision | (sp ? GRAPH_GROUPED : 0), mode, numiter,
                  op->Dl, op->d, op->Dr, op->Ml, op->Mr, batch};
   std::lock_guard<std::mutex> lk(g_gmu);
   if (g_graph_cache.size() >= GRAPH_MAX_BUCKETS && !g_graph_cache.count(key)) {
@@ -597,7 +606,7 @@ int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, do
 static int krylov_enqueue(const ptn_local_op* op, int mode, const void* v, double dt_re,
                           double dt_im, int numiter, void* out, double* energy, int32_t* info,
                           double* alpha_beta, void* ws, size_t ws_bytes, hipStream_t s,
-                          int64_t batch) {
+                          int64_t batch, const SectorPlan* sp) {
   if (!op || numiter < 1 || numiter > LZ_MAX || batch < 1 || batch > 65535) return PTN_ERR_ARG;
   if (op->kind != PTN_OP_HEFF && op->kind != PTN_OP_BOND) return PTN_ERR_ARG;
   const int bprec = op->precision & ~PTN_BLOCK_SPARSE;
@@ -636,6 +645,8 @@ static int krylov_enqueue(const ptn_local_op* op, int mode, const void* v, doubl
   }
   Dims D = {op->Dl, op->d, op->Dr, op->Ml, op->Mr};
   auto matvec = [&](const double2* x, double2* y) -> int {
+    if (op->kind == PTN_OP_HEFF && sp && batch == 1)
+      return heff_apply_sectors(op->precision, D, sp, L, tl, op->w, R, tr, x, y, mvws, mvbytes, s);
     if (op->kind == PTN_OP_HEFF)
       return heff_apply(op->precision, D, L, tl, op->w, R, tr, x, PTN_T_C128, y, PTN_T_C128,
                         mvws, mvbytes, s, batch);
@@ -723,8 +734,9 @@ int krylov_expm_noalloc(const ptn_local_op* op, const void* v, double dt_re, dou
   int st = scratch_address(&ws);
   if (st) return st;
   NoAllocGuard guard;
+  // no sector plan: building one allocates (the tile-mask products run instead)
   return krylov_enqueue(op, 0, v, dt_re, dt_im, numiter, out, nullptr, info, alpha_beta, ws,
-                        (size_t)PTN_NOALLOC_SCRATCH_BYTES, s, 1);
+                        (size_t)PTN_NOALLOC_SCRATCH_BYTES, s, 1, nullptr);
 }
 
 }  // namespace ptn

@@ -24,4 +24,14 @@ int env_left(int prec, const Dims& D, const void* a, const void* b, const void* 
              int tl, void* out, void* ws, size_t ws_bytes, hipStream_t s, int64_t batch = 1);
 int convert_c128_c64(int64_t n, const void* x, void* y, hipStream_t s);
 
+// sector-grouped H_eff (U(1) quantum numbers, batch 1): the plan of an
+// operator, or null when its grouped products do not apply (no or unsuitable
+// quantum numbers); built and uploaded synchronously on first use, so look it
+// up outside stream capture
+struct SectorPlan;
+int sector_plan_get(const ptn_local_op* op, const SectorPlan** out);
+int heff_apply_sectors(int prec, const Dims& D, const SectorPlan* sp, const void* l, int tl,
+                       const void* w, const void* r, int tr, const void* a, void* out, void* ws,
+                       size_t ws_bytes, hipStream_t s);
+
 }  // namespace ptn

@@ -43,12 +43,12 @@ def dmrg_singlesite(hamiltonian: MPO, psi: MPS, numsweeps: int, numiter_lanczos:
     for n in range(numsweeps):
         en = None
         for i in range(nsites - 1):
-            en, psi.a[i] = ctx.site_eigh(psi.a[i], lblocks[i], W[i], rblocks[i])
+            en, psi.a[i] = ctx.site_eigh(psi.a[i], lblocks[i], W[i], rblocks[i], ctx.q1(psi, i))
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_local_orthonormalize_left_qr(
                 psi.a[i], psi.a[i + 1], psi.qsite, psi.qbonds[i:i + 2])
             lblocks[i + 1] = ctx.env_left(psi.a[i], W[i], lblocks[i])
         for i in reversed(range(1, nsites)):
-            en, psi.a[i] = ctx.site_eigh(psi.a[i], lblocks[i], W[i], rblocks[i])
+            en, psi.a[i] = ctx.site_eigh(psi.a[i], lblocks[i], W[i], rblocks[i], ctx.q1(psi, i))
             psi.a[i], psi.a[i - 1], psi.qbonds[i] = mps_local_orthonormalize_right_qr(
                 psi.a[i], psi.a[i - 1], psi.qsite, psi.qbonds[i:i + 2])
             rblocks[i - 1] = ctx.env_right(psi.a[i], W[i], rblocks[i])
@@ -81,7 +81,7 @@ def dmrg_twosite(hamiltonian: MPO, psi: MPS, numsweeps: int, numiter_lanczos: in
         en = None
         for i in range(nsites - 2):
             a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
-            en, a2 = ctx.site_eigh(a2, lblocks[i], h2[i], rblocks[i + 1])
+            en, a2 = ctx.site_eigh(a2, lblocks[i], h2[i], rblocks[i + 1], ctx.q2(psi, i))
             ctx.flush()   # a failed local solve raises here, before its split
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "right", tol=tol_split,
@@ -89,7 +89,7 @@ def dmrg_twosite(hamiltonian: MPO, psi: MPS, numsweeps: int, numiter_lanczos: in
             lblocks[i + 1] = ctx.env_left(psi.a[i], W[i], lblocks[i])
         for i in reversed(range(nsites - 1)):
             a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
-            en, a2 = ctx.site_eigh(a2, lblocks[i], h2[i], rblocks[i + 1])
+            en, a2 = ctx.site_eigh(a2, lblocks[i], h2[i], rblocks[i + 1], ctx.q2(psi, i))
             ctx.flush()   # a failed local solve raises here, before its split
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=tol_split,

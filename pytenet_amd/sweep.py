@@ -9,7 +9,7 @@ import numpy as np
 import torch
 
 from . import backend, _lib
-from .block_sparse_util import is_qsparse
+from .block_sparse_util import is_qsparse, qnumber_flatten
 
 
 class SweepContext:
@@ -35,10 +35,28 @@ class SweepContext:
     def one(self):
         return torch.ones((1, 1, 1), dtype=torch.complex128, device=self.dev)
 
-    def site_expm(self, a, l, w, r, dt):
-        """expm(dt * H_eff) a (the reference passes -dt to expm_krylov, tdvp.py:227)."""
+    def q1(self, psi, i):
+        """U(1) labels of the one-site H_eff at site i (left bond, physical, right
+        bond, MPO bonds), or None without quantum numbers."""
+        if not self.block_sparse:
+            return None
+        return (psi.qbonds[i], psi.qsite, psi.qbonds[i + 1], self.H.qbonds[i],
+                self.H.qbonds[i + 1])
+
+    def q2(self, psi, i):
+        """U(1) labels of the two-site H_eff on sites (i, i+1): the physical
+        labels are the flattened pair sums of the merged pair (mps.py:528-535)."""
+        if not self.block_sparse:
+            return None
+        return (psi.qbonds[i], qnumber_flatten((psi.qsite, psi.qsite)), psi.qbonds[i + 2],
+                self.H.qbonds[i], self.H.qbonds[i + 2])
+
+    def site_expm(self, a, l, w, r, dt, qnums=None):
+        """expm(dt * H_eff) a (the reference passes -dt to expm_krylov, tdvp.py:227);
+        `qnums` (q1 / q2) lets the matvecs run over the sector blocks only."""
         out, _, info = backend.krylov_local(_lib.PTN_OP_HEFF, a, l, w, r, self.numiter, "expm",
-                                            dt, self.precision, block_sparse=self.block_sparse)
+                                            dt, self.precision, block_sparse=self.block_sparse,
+                                            qnums=qnums)
         self._status.append(info)
         return out
 
@@ -50,11 +68,12 @@ class SweepContext:
         self._status.append(info)
         return out
 
-    def site_eigh(self, a, l, w, r):
+    def site_eigh(self, a, l, w, r, qnums=None):
         """Lowest Ritz pair of H_eff started from a (dmrg.py:181-189);
         the energy stays on the device."""
         out, en, info = backend.krylov_local(_lib.PTN_OP_HEFF, a, l, w, r, self.numiter, "eigh",
-                                             0.0, self.precision, block_sparse=self.block_sparse)
+                                             0.0, self.precision, block_sparse=self.block_sparse,
+                                             qnums=qnums)
         self._status.append(info)
         return en, out
 

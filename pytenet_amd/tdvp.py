@@ -40,7 +40,8 @@ def tdvp_singlesite(hamiltonian: MPO, psi: MPS, dt, numsteps: int, numiter_lancz
     for _ in range(numsteps):
         # left to right
         for i in range(nsites - 1):
-            psi.a[i] = ctx.site_expm(psi.a[i], lblocks[i], W[i], rblocks[i], -0.5 * dt)
+            psi.a[i] = ctx.site_expm(psi.a[i], lblocks[i], W[i], rblocks[i], -0.5 * dt,
+                                     ctx.q1(psi, i))
             s = psi.a[i].shape
             q, c, psi.qbonds[i + 1] = block_sparse_qr(
                 psi.a[i].reshape(s[0] * s[1], s[2]),
@@ -50,7 +51,7 @@ def tdvp_singlesite(hamiltonian: MPO, psi: MPS, dt, numsteps: int, numiter_lancz
             c = ctx.bond_expm(c, lblocks[i + 1], rblocks[i], 0.5 * dt)
             psi.a[i + 1] = absorb_left(c, psi.a[i + 1])
         i = nsites - 1
-        psi.a[i] = ctx.site_expm(psi.a[i], lblocks[i], W[i], rblocks[i], -dt)
+        psi.a[i] = ctx.site_expm(psi.a[i], lblocks[i], W[i], rblocks[i], -dt, ctx.q1(psi, i))
         # right to left
         for i in reversed(range(1, nsites)):
             psi.a[i] = flip(psi.a[i])
@@ -65,7 +66,7 @@ def tdvp_singlesite(hamiltonian: MPO, psi: MPS, dt, numsteps: int, numiter_lancz
             c = ctx.bond_expm(c, lblocks[i], rblocks[i - 1], 0.5 * dt)
             psi.a[i - 1] = absorb_right(psi.a[i - 1], c)
             psi.a[i - 1] = ctx.site_expm(psi.a[i - 1], lblocks[i - 1], W[i - 1], rblocks[i - 1],
-                                         -0.5 * dt)
+                                         -0.5 * dt, ctx.q1(psi, i - 1))
         ctx.flush()
     ctx.finish(psi)
     return nrm
@@ -100,17 +101,17 @@ class TwoSiteTDVP:
         dt, qs, n = self.dt, psi.qsite, self.nsites
         for i in range(n - 2):
             a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
-            a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -0.5 * dt)
+            a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -0.5 * dt, ctx.q2(psi, i))
             ctx.flush()   # a failed local solve raises here, before its split
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, (psi.qbonds[i], psi.qbonds[i + 2]), "right", tol=self.tol,
                 max_bond=self.max_bond, svd_precond32=self.pre32)
             lblocks[i + 1] = ctx.env_left(psi.a[i], W[i], lblocks[i])
             psi.a[i + 1] = ctx.site_expm(psi.a[i + 1], lblocks[i + 1], W[i + 1], rblocks[i + 1],
-                                         0.5 * dt)
+                                         0.5 * dt, ctx.q1(psi, i + 1))
         i = n - 2
         a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
-        a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -dt)
+        a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -dt, ctx.q2(psi, i))
         ctx.flush()   # a failed local solve raises here, before its split
         psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
             a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=self.tol,
@@ -118,9 +119,9 @@ class TwoSiteTDVP:
         rblocks[i] = ctx.env_right(psi.a[i + 1], W[i + 1], rblocks[i + 1])
         for i in reversed(range(n - 2)):
             psi.a[i + 1] = ctx.site_expm(psi.a[i + 1], lblocks[i + 1], W[i + 1], rblocks[i + 1],
-                                         0.5 * dt)
+                                         0.5 * dt, ctx.q1(psi, i + 1))
             a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
-            a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -0.5 * dt)
+            a2 = ctx.site_expm(a2, lblocks[i], h2[i], rblocks[i + 1], -0.5 * dt, ctx.q2(psi, i))
             ctx.flush()   # a failed local solve raises here, before its split
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=self.tol,

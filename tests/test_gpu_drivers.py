@@ -238,3 +238,23 @@ def test_tdvp_checkpoint_resume(tmp_path):
     x = backend.to_host(x)
     y = y * np.linalg.norm(x) / np.linalg.norm(y)    # the resumed run restarts from norm one
     assert np.abs(_phase_aligned(x, y) - y).max() < 1e-10 * np.linalg.norm(y)
+
+
+@pytest.mark.parametrize("L,D,steps", [(10, 16, 2), (20, 512, 1)])
+def test_bf16x3_contractions_keep_fidelity(L, D, steps):
+    """Split-bf16 contractions in the two-site TDVP (FP64 Krylov scalars, QR
+    and SVD): fidelity and energy against the all-FP64 run of the same state.
+    At D = 512 the two-site products take the pre-split split-bf16 path."""
+    rng = np.random.default_rng(43)
+    H = ptn.heisenberg_xxz_1d_mpo(L, 1.0, 0.8, -0.1).zero_qnumbers()
+    bonds = [min(D, 2**min(i, L - i)) for i in range(L + 1)]
+    psi = ptn.MPS(H.qsite, [np.zeros(b, dtype=int) for b in bonds], fill="random", rng=rng)
+    psi.orthonormalize("left")
+    p64, p16 = copy.deepcopy(psi), copy.deepcopy(psi)
+    ptn.tdvp_twosite(H, p64, -0.05j, steps, numiter_lanczos=8, tol_split=1e-10, max_bond=D)
+    ptn.tdvp_twosite(H, p16, -0.05j, steps, numiter_lanczos=8, tol_split=1e-10, max_bond=D,
+                     precision="bf16x3")
+    f = abs(ptn.mps_vdot(p64, p16))**2 / (ptn.mps_norm(p64)**2 * ptn.mps_norm(p16)**2)
+    assert abs(1 - f) < 1e-6
+    e64, e16 = ptn.mpo_average(p64, H).real, ptn.mpo_average(p16, H).real
+    assert abs(e16 - e64) < 1e-4 * abs(e64)
