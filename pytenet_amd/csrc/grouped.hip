@@ -23,10 +23,11 @@
 //       (l in sector P - qml[m], m), one segment per MPO index m.
 // Every output block belongs to exactly one product block (its sector fixes
 // the k-range), so no two workgroups write the same entry and no block needs
-// accumulation. The entries outside the blocks are structurally zero and are
-// written by one fill pass; the sums inside a block run over exactly the
-// reference's nonzero terms. The MPO mode mix between the products stays
-// dense (it is HBM-bound and reads T1 once).
+// accumulation. The entries of the output outside the blocks are structurally
+// zero and are written by one fill pass; the sums inside a block run over
+// exactly the reference's nonzero terms. The MPO mode mix between the products
+// (k_mix_sectors) reads only the T1 entries the first product writes and writes
+// only the T2 entries the second product reads.
 //
 // Launch: the grid covers the tiles of all blocks; a workgroup finds its block
 // by binary search over the blocks' first tiles (descriptors in device memory,
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(256) void zgemm_grouped_kernel(GLaunch g) {
   }
 }
 
-// zero fill of n complex elements (the structurally zero entries of T1 / out)
+// zero fill of n complex elements (the structurally zero entries of the output)
 template <typename TC>
 __global__ void k_zero_fill(int64_t n, TC* __restrict__ p, const int* __restrict__ skip) {
   if (skip && skip[0]) return;
@@ -233,6 +234,72 @@ static int zero_fill(int64_t n, void* p, hipStream_t s) {
           dim3(256), s, n, (TC*)p, skip_flag().p);
   PTN_LAUNCH_CHECK();
   return PTN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Sector-aware MPO mode mix T2[l, o, r'] = sum_i Wm[o, i] T1[l, i, r'] with
+// i = (s, m'), o = (m, s'). For a left sector P of l and a right sector Q of
+// r' (a "unit"), T1[l, i, r'] is nonzero only for the inputs i with
+// qs[s] + qmr[m'] = Q - P whose k-sector P + qs[s] exists (exactly the entries
+// the first grouped product writes), and the second product reads T2[l, o, r']
+// only for the outputs o with qml[m] + qs[s'] = Q - P whose output sector
+// P + qml[m] exists. The mix reads and writes just those: T1 needs no zero
+// fill and T2 no zeros outside what is read. One wave per row l of a unit,
+// the lanes over the unit's columns r' (coalesced).
+constexpr int MIX_LIST = 32;   // d * Mr and Ml * d are at most 32
+
+struct MixUnit {
+  int64_t l0, nl, r0, nr;
+  int64_t tile0;                // first workgroup of the unit (4 rows per workgroup)
+  int32_t ni, no;
+  int8_t il[MIX_LIST], ol[MIX_LIST];
+};
+
+template <typename TX, typename TY>
+__global__ __launch_bounds__(256) void k_mix_sectors(const MixUnit* __restrict__ units, int nunits,
+                                                     int ni_all, int no_all, int64_t nr_all,
+                                                     const double2* __restrict__ wm,
+                                                     const TX* __restrict__ x, TY* __restrict__ y,
+                                                     const int* __restrict__ skip) {
+  __shared__ double2 W[MIX_LIST * MIX_LIST];
+  if (skip && skip[0]) return;
+  for (int t = threadIdx.x; t < ni_all * no_all; t += blockDim.x) W[t] = wm[t];
+  __syncthreads();
+  const int64_t tile = blockIdx.x;
+  int lo = 0, hi = nunits - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (units[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
+  }
+  const MixUnit* __restrict__ u = units + lo;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t row = (tile - u->tile0) * 4 + wave;
+  if (row >= u->nl) return;
+  const int64_t l = u->l0 + row;
+  const int ni = u->ni, no = u->no;
+  const TX* __restrict__ xl = x + l * ni_all * nr_all;
+  TY* __restrict__ yl = y + l * no_all * nr_all;
+  for (int64_t r = u->r0 + lane; r < u->r0 + u->nr; r += 64) {
+    double xr[MIX_LIST], xi[MIX_LIST];
+#pragma unroll
+    for (int k = 0; k < MIX_LIST; ++k) {
+      xr[k] = 0; xi[k] = 0;
+      if (k < ni) load_c<double>(xl + (int64_t)u->il[k] * nr_all + r, xr[k], xi[k]);
+    }
+    for (int j = 0; j < no; ++j) {
+      const int o = u->ol[j];
+      double yr = 0, yi = 0;
+#pragma unroll
+      for (int k = 0; k < MIX_LIST; ++k) {
+        if (k < ni) {
+          const double2 w = W[o * ni_all + u->il[k]];
+          yr += w.x * xr[k] - w.y * xi[k];
+          yi += w.x * xi[k] + w.y * xr[k];
+        }
+      }
+      store_c(yl + (int64_t)o * nr_all + r, yr, yi);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -263,7 +330,11 @@ struct GProduct {
 
 struct SectorPlan {
   GProduct p1, p3;
-  GBlock* dev = nullptr;   // p1's blocks, then p3's
+  std::vector<MixUnit> mix;
+  int64_t mix_tiles = 0;
+  double mix_bytes = 0.0;   // T1 entries read and T2 entries written by the mix (elements)
+  GBlock* dev = nullptr;    // p1's blocks, then p3's, then the mix units
+  MixUnit* dev_mix = nullptr;
   int device = 0;
   uint64_t stamp = 0;
 };
@@ -300,7 +371,9 @@ static void finish_product(GProduct& P, size_t csz_a, size_t csz_b, size_t csz_c
 // bond exceeds the segment limit or the launch would be too large
 static bool build_plan(const Dims& D, const int64_t* ql, const int64_t* qs, const int64_t* qr,
                        const int64_t* qml, const int64_t* qmr, size_t csz, SectorPlan& sp) {
-  if (D.Ml > GMAXSEG || D.Dl <= 0 || D.Dr <= 0) return false;
+  if (D.Ml > GMAXSEG || D.Dl <= 0 || D.Dr <= 0 || D.d <= 0 || D.Mr <= 0 ||
+      D.d * D.Mr > MIX_LIST || D.Ml * D.d > MIX_LIST)
+    return false;
   std::map<int64_t, Sector> sl, sr;
   if (!sectors_of(ql, D.Dl, sl) || !sectors_of(qr, D.Dr, sr)) return false;
   // T1[(l, s), (m', r')] = sum_r A[(l, s), r] R[r, (m', r')]
@@ -350,10 +423,31 @@ static bool build_plan(const Dims& D, const int64_t* ql, const int64_t* qs, cons
       sp.p3.blocks.push_back(b);
     }
   }
+  // mix units: (left sector P, right sector Q) with outputs read by the second product
+  for (const auto& pq : sl)
+    for (const auto& rq : sr) {
+      const int64_t P = pq.first, dq = rq.first - P;
+      MixUnit u{};
+      for (int64_t m = 0; m < D.Ml; ++m)
+        for (int64_t s2 = 0; s2 < D.d; ++s2)
+          if (qml[m] + qs[s2] == dq && sl.count(P + qml[m])) u.ol[u.no++] = (int8_t)(m * D.d + s2);
+      if (u.no == 0) continue;
+      for (int64_t s1 = 0; s1 < D.d; ++s1)
+        for (int64_t mp = 0; mp < D.Mr; ++mp)
+          if (qs[s1] + qmr[mp] == dq && sr.count(P + qs[s1])) u.il[u.ni++] = (int8_t)(s1 * D.Mr + mp);
+      u.l0 = pq.second.start;
+      u.nl = pq.second.len;
+      u.r0 = rq.second.start;
+      u.nr = rq.second.len;
+      u.tile0 = sp.mix_tiles;
+      sp.mix_tiles += (u.nl + 3) / 4;
+      sp.mix_bytes += (double)u.nl * (double)u.nr * (double)(u.ni + u.no);
+      sp.mix.push_back(u);
+    }
   finish_product(sp.p1, 16, csz, csz);   // A is complex128, R and T1 in the compute type
   finish_product(sp.p3, csz, csz, 16);   // out is complex128
   return sp.p1.tiles > 0 && sp.p3.tiles > 0 && sp.p1.tiles < 0x7fffffff &&
-         sp.p3.tiles < 0x7fffffff;
+         sp.p3.tiles < 0x7fffffff && !sp.mix.empty() && sp.mix_tiles < 0x7fffffff;
 }
 
 namespace {
@@ -420,9 +514,13 @@ int sector_plan_get(const ptn_local_op* op, const SectorPlan** out) {
   }
   std::vector<GBlock> all(sp->p1.blocks);
   all.insert(all.end(), sp->p3.blocks.begin(), sp->p3.blocks.end());
-  hipError_t e = hipMalloc((void**)&sp->dev, all.size() * sizeof(GBlock));
+  const size_t bb = (all.size() * sizeof(GBlock) + 255) & ~(size_t)255;
+  const size_t mb = sp->mix.size() * sizeof(MixUnit);
+  hipError_t e = hipMalloc((void**)&sp->dev, bb + mb);
   if (e == hipSuccess) e = hipMemcpy(sp->dev, all.data(), all.size() * sizeof(GBlock),
                                      hipMemcpyHostToDevice);
+  sp->dev_mix = (MixUnit*)((char*)sp->dev + bb);
+  if (e == hipSuccess) e = hipMemcpy(sp->dev_mix, sp->mix.data(), mb, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     if (sp->dev) (void)hipFree(sp->dev);
     ptn_set_last_hip_error(e);
@@ -463,8 +561,8 @@ static int grouped_launch(const GLaunch& g, const GProduct& P, hipStream_t s) {
 
 // H_eff through the grouped products: out = L^T . (W (x) (A . R)) as
 // heff_apply computes it (same operand types: A and out complex128, L and R
-// complex128 in FP64 and complex64 in FP32), T1 and out zero-filled and only
-// the blocks computed. Uses T1 and T2 of the heff_apply workspace.
+// complex128 in FP64 and complex64 in FP32), only the blocks computed and out
+// zero-filled outside them. Uses T1 and T2 of the heff_apply workspace.
 int heff_apply_sectors(int prec, const Dims& D, const SectorPlan* sp, const void* l, int tl,
                        const void* w, const void* r, int tr, const void* a, void* out, void* ws,
                        size_t ws_bytes, hipStream_t s) {
@@ -475,7 +573,7 @@ int heff_apply_sectors(int prec, const Dims& D, const SectorPlan* sp, const void
   if (ws_bytes < heff_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;
   const size_t csz = bp == PTN_FP64 ? 16 : 8;
   const int64_t n1 = (D.Dl * D.d) * (D.Mr * D.Dr), nout = D.Dl * D.d * D.Dr;
-  void* t1 = ws;
+  void* t1 = ws;   // written by the first product only inside its blocks (the mix reads no more)
   void* t2 = (char*)ws + (((size_t)n1 * csz + 255) & ~(size_t)255);
   const GBlock* b1 = sp->dev;
   const GBlock* b3 = sp->dev + sp->p1.blocks.size();
@@ -483,18 +581,22 @@ int heff_apply_sectors(int prec, const Dims& D, const SectorPlan* sp, const void
   int st;
   GLaunch g1{b1, (int)sp->p1.blocks.size(), D.d * D.Dr, 1, D.Mr * D.Dr, 1, D.d * D.Mr * D.Dr, 1,
              a, r, t1, skip};
+  const int ni = (int)(D.d * D.Mr), no = (int)(D.Ml * D.d);
+  const dim3 mgrid((unsigned)sp->mix_tiles);
   if (bp == PTN_FP64) {
-    if ((st = zero_fill<double2>(n1, t1, s))) return st;
     if ((st = grouped_launch<double, double2, double2, double2, true, true>(g1, sp->p1, s)))
       return st;
+    tlaunch("k_mix_sectors<c128, c128>", 0.0, sp->mix_bytes * 16, k_mix_sectors<double2, double2>,
+            mgrid, dim3(256), s, (const MixUnit*)sp->dev_mix, (int)sp->mix.size(), ni, no, D.Dr,
+            (const double2*)w, (const double2*)t1, (double2*)t2, skip);
   } else {
-    if ((st = zero_fill<float2>(n1, t1, s))) return st;
     if ((st = grouped_launch<float, double2, float2, float2, true, true>(g1, sp->p1, s)))
       return st;
+    tlaunch("k_mix_sectors<c64, c64>", 0.0, sp->mix_bytes * 8, k_mix_sectors<float2, float2>,
+            mgrid, dim3(256), s, (const MixUnit*)sp->dev_mix, (int)sp->mix.size(), ni, no, D.Dr,
+            (const double2*)w, (const float2*)t1, (float2*)t2, skip);
   }
-  if ((st = mode_mix(D.Dl, D.d * D.Mr, D.Ml * D.d, D.Dr, w, t1, ct, t2, ct, s, 1,
-                     D.Ml * D.d * D.d * D.Mr)))
-    return st;
+  PTN_LAUNCH_CHECK();
   if ((st = zero_fill<double2>(nout, out, s))) return st;
   GLaunch g3{b3, (int)sp->p3.blocks.size(), 1, D.Ml * D.Dl, D.Ml * D.d * D.Dr, 1, D.d * D.Dr, 1,
              l, t2, out, skip};
