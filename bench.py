@@ -360,7 +360,7 @@ def workload_dmrg2_u1(args):
     finally:
         dm.mps_split_tensor_svd = split
     return {"workload": f"two-site DMRG with U(1) sectors, XXZ L={L} Delta=1, D_max={D}, "
-                        f"{sweeps} sweep(s) incl. setup, FP64, sector-grouped H_eff products",
+                        f"{sweeps} sweep(s) incl. setup, FP64, sector-grouped products",
             "value": round(dt / sweeps, 2), "unit": "s/sweep", "total_s": round(dt, 2),
             "sectors_centre": int(len(u)), "largest_sector": int(cnt.max()),
             "energies": [float(e) for e in np.asarray(en).real],

@@ -146,8 +146,8 @@ typedef struct {
    * q + q_m = q', pytenet/tdvp.py:62). With PTN_BLOCK_SPARSE, PTN_FP64 or PTN_FP32, an
    * H_eff operator, batch 1 and sector-contiguous virtual bonds (every label
    * in one run of indices, as the reference's QR and SVD splits leave them),
-   * the two products of H_eff run as one grouped launch each over the
-   * admissible sector blocks only (the block descriptors are built on the
+   * the two products of H_eff (and of the environment steps, ptn_env_apply)
+   * run as one grouped launch each over the admissible sector blocks only (the block descriptors are built on the
    * first call with a given set of labels and cached; that first call copies
    * them to the device synchronously). Otherwise the tile-mask products run.
    * Results are the same either way: the grouped sums run over exactly the
@@ -170,6 +170,19 @@ typedef struct {
  * grouped products ran, else 0. */
 int ptn_local_apply(const ptn_local_op* op, const void* a, void* out, int32_t* grouped,
                     void* ws, size_t ws_bytes, void* stream);
+
+/* One environment step of the site described by `op` (kind PTN_OP_HEFF: Dl,
+ * d, Dr, Ml, Mr, W, and the block being extended: op->l for PTN_ENV_LEFT,
+ * op->r for PTN_ENV_RIGHT), with the sector-grouped products (and the
+ * labels' conditions) as for ptn_local_apply, otherwise the tile-mask
+ * products; results as ptn_env_left / ptn_env_right (the same arguments a, b,
+ * out and workspace, ptn_env_workspace). Replaces
+ * pytenet/chain_ops.py:60-99 contraction_operator_step_left and
+ * pytenet/chain_ops.py:16-57 contraction_operator_step_right. */
+#define PTN_ENV_LEFT  0
+#define PTN_ENV_RIGHT 1
+int ptn_env_apply(const ptn_local_op* op, int side, const void* a, const void* b, void* out,
+                  int32_t* grouped, void* ws, size_t ws_bytes, void* stream);
 
 size_t ptn_krylov_workspace(const ptn_local_op* op, int numiter);
 

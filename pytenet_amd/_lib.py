@@ -27,6 +27,8 @@ PTN_OP_BOND = 1
 
 PTN_SPLIT_LEFT = 0
 PTN_SPLIT_RIGHT = 1
+PTN_ENV_LEFT = 0
+PTN_ENV_RIGHT = 1
 
 PTN_NOALLOC_SCRATCH_BYTES = 64 << 20
 
@@ -62,6 +64,7 @@ SIGNATURES = [
     ("ptn_bond_workspace", c_sz, [c_int, c_i64, c_i64, c_i64]),
     ("ptn_bond_apply", c_int, [c_int, c_i64, c_i64, c_i64, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
     ("ptn_local_apply", c_int, [ctypes.POINTER(LocalOp), c_p, c_p, c_p, c_p, c_sz, c_p]),
+    ("ptn_env_apply", c_int, [ctypes.POINTER(LocalOp), c_int, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),
     ("ptn_env_workspace", c_sz, [c_int, c_i64, c_i64, c_i64, c_i64, c_i64]),
     ("ptn_env_left", c_int, [c_int, c_i64, c_i64, c_i64, c_i64, c_i64,
                              c_p, c_p, c_p, c_p, c_p, c_p, c_sz, c_p]),

@@ -161,13 +161,33 @@ def bond_apply(c, l, r, precision="fp64", block_sparse=False):
     return out
 
 
-def env_left(a, b, w, l, precision="fp64", block_sparse=False):
+def _env_sectors(side, a, b, w, blk, qnums, precision, shape):
+    """Environment step through ptn_env_apply with U(1) labels (grouped
+    products where they apply); returns (out, grouped)."""
+    Dl, d, Dr = a.shape
+    lib = _lib.load()
+    left = side == _lib.PTN_ENV_LEFT
+    op, keep = _local_op(_lib.PTN_OP_HEFF, precision, blk if left else None, w,
+                         None if left else blk, Dl, d, Dr, True, qnums)
+    out = torch.empty(shape, dtype=torch.complex128, device=a.device)
+    ws = workspace(lib.ptn_env_workspace(op.precision, Dl, d, Dr, op.Ml, op.Mr))
+    grouped = ctypes.c_int32(0)
+    _check(lib.ptn_env_apply(ctypes.byref(op), side, ptr(a), ptr(b), ptr(out),
+                             ctypes.addressof(grouped), ptr(ws), ws.numel(), stream()),
+           "ptn_env_apply")
+    del keep
+    return out, bool(grouped.value)
+
+
+def env_left(a, b, w, l, precision="fp64", block_sparse=False, qnums=None):
     """out[r,m',rb] = sum A[l,s,r] W[m,sb,s,m'] L[l,m,lb] conj(B[lb,sb,rb])."""
     a, b, w, l = as_dev(a), as_dev(b), as_dev(w), as_dev(l)
     Dl, d, Dr = a.shape
     Ml, Mr = w.shape[0], w.shape[3]
     _shape_check(b.shape == a.shape and w.shape == (Ml, d, d, Mr) and l.shape == (Dl, Ml, Dl),
                  "contraction_operator_step_left", a, b, w, l)
+    if block_sparse and qnums is not None:
+        return _env_sectors(_lib.PTN_ENV_LEFT, a, b, w, l, qnums, precision, (Dr, Mr, Dr))[0]
     prec = precision_code(precision, block_sparse)
     lib = _lib.load()
     out = torch.empty((Dr, Mr, Dr), dtype=torch.complex128, device=a.device)
@@ -177,13 +197,15 @@ def env_left(a, b, w, l, precision="fp64", block_sparse=False):
     return out
 
 
-def env_right(a, b, w, r, precision="fp64", block_sparse=False):
+def env_right(a, b, w, r, precision="fp64", block_sparse=False, qnums=None):
     """out[l,m,lb] = sum A[l,s,r] W[m,sb,s,m'] R[r,m',rb] conj(B[lb,sb,rb])."""
     a, b, w, r = as_dev(a), as_dev(b), as_dev(w), as_dev(r)
     Dl, d, Dr = a.shape
     Ml, Mr = w.shape[0], w.shape[3]
     _shape_check(b.shape == a.shape and w.shape == (Ml, d, d, Mr) and r.shape == (Dr, Mr, Dr),
                  "contraction_operator_step_right", a, b, w, r)
+    if block_sparse and qnums is not None:
+        return _env_sectors(_lib.PTN_ENV_RIGHT, a, b, w, r, qnums, precision, (Dl, Ml, Dl))[0]
     prec = precision_code(precision, block_sparse)
     lib = _lib.load()
     out = torch.empty((Dl, Ml, Dl), dtype=torch.complex128, device=a.device)
@@ -238,11 +260,11 @@ def _local_op(kind, precision, l, w, r, Dl, d, Dr, block_sparse=False, qnums=Non
     op.kind = kind
     op.precision = precision_code(precision, block_sparse)
     op.Dl, op.d, op.Dr = Dl, d, Dr
-    op.Ml = l.shape[1]
-    op.Mr = r.shape[1]
-    op.l = l.data_ptr()
+    op.Ml = l.shape[1] if l is not None else w.shape[0]
+    op.Mr = r.shape[1] if r is not None else w.shape[3]
+    op.l = l.data_ptr() if l is not None else None
     op.w = w.data_ptr() if w is not None else None
-    op.r = r.data_ptr()
+    op.r = r.data_ptr() if r is not None else None
     keep = None
     if qnums is not None and block_sparse and kind == _lib.PTN_OP_HEFF:
         keep = [np.ascontiguousarray(np.asarray(q, dtype=np.int64).reshape(-1)) for q in qnums]

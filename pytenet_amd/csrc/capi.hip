@@ -181,7 +181,7 @@ int ptn_local_apply(const ptn_local_op* op, const void* a, void* out, int32_t* g
   if (op->kind != PTN_OP_HEFF) return PTN_ERR_ARG;
   const SectorPlan* sp = nullptr;
   int st;
-  if ((st = sector_plan_get(op, &sp))) return st;
+  if ((st = sector_plan_get(op, SECTORS_HEFF, &sp))) return st;
   if (!sp)
     return ptn_heff_apply(op->precision, op->Dl, op->d, op->Dr, op->Ml, op->Mr, op->l, op->w,
                           op->r, a, out, ws, ws_bytes, stream);
@@ -241,6 +241,39 @@ static int env_step_b(int left, int precision, int64_t batch, int64_t Dl, int64_
   }
   if (left) return env_left(precision, D, a, b, w, e, te, out, ws, base, s, batch);
   return env_right(precision, D, a, b, w, e, te, out, ws, base, s, batch);
+}
+
+int ptn_env_apply(const ptn_local_op* op, int side, const void* a, const void* b, void* out,
+                  int32_t* grouped, void* ws, size_t ws_bytes, void* stream) {
+  if (grouped) *grouped = 0;
+  if (!op || op->kind != PTN_OP_HEFF || (side != PTN_ENV_LEFT && side != PTN_ENV_RIGHT))
+    return PTN_ERR_ARG;
+  const bool left = side == PTN_ENV_LEFT;
+  const SectorPlan* sp = nullptr;
+  int st;
+  if ((st = sector_plan_get(op, left ? SECTORS_ENV_LEFT : SECTORS_ENV_RIGHT, &sp))) return st;
+  hipStream_t s = (hipStream_t)stream;
+  if (!sp)
+    return env_step_b(left ? 1 : 0, op->precision, 1, op->Dl, op->d, op->Dr, op->Ml, op->Mr, a,
+                      b, op->w, left ? op->l : op->r, out, ws, ws_bytes, s);
+  const Dims D = {op->Dl, op->d, op->Dr, op->Ml, op->Mr};
+  if (ws_bytes < env_workspace_b(op->precision, 1, op->Dl, op->d, op->Dr, op->Ml, op->Mr))
+    return PTN_ERR_WORKSPACE;
+  const size_t base = al256(env_ws_bytes(op->precision, D));
+  const void* e = left ? op->l : op->r;
+  int te = PTN_T_C128;
+  if (lowp(op->precision)) {
+    char* e64 = (char*)ws + base;
+    const int64_t ne = left ? op->Dl * op->Ml * op->Dl : op->Dr * op->Mr * op->Dr;
+    if ((st = convert_c128_c64(ne, e, e64, s))) return st;
+    e = e64;
+    te = PTN_T_C64;
+  }
+  st = left ? env_left_sectors(op->precision, D, sp, a, b, op->w, e, te, out, ws, base, s)
+            : env_right_sectors(op->precision, D, sp, a, b, op->w, e, te, out, ws, base, s);
+  if (st) return st;
+  if (grouped) *grouped = 1;
+  return PTN_OK;
 }
 
 size_t ptn_env_workspace(int precision, int64_t Dl, int64_t d, int64_t Dr, int64_t Ml,

@@ -66,6 +66,7 @@ struct GLaunch {
   const void* B;
   void* C;
   const int* skip;
+  int conjB;                              // B read conjugated
 };
 
 // One workgroup = one BM x BN tile of one block; the MFMA tile body is the one
@@ -97,6 +98,7 @@ __global__ __launch_bounds__(256) void zgemm_grouped_kernel(GLaunch g) {
   const TA* __restrict__ A = (const TA*)g.A;
   const TB* __restrict__ B = (const TB*)g.B;
   TC* __restrict__ C = (TC*)g.C + bk->c_off;
+  const Tc sb = g.conjB ? (Tc)-1 : (Tc)1;
   int nst = 0;
   for (int sg = 0; sg < nseg; ++sg) nst += (int)((bk->K[sg] + BK - 1) / BK);
 
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(256) void zgemm_grouped_kernel(GLaunch g) {
       const int64_t n = n0 + col, k = k0 + kc;
       Tc re = 0, im = 0;
       if (n < bN && k < K) load_c<Tc>(Bp + k * g.b_k + n * g.b_n, re, im);
-      rb[i][0] = re; rb[i][1] = im;
+      rb[i][0] = re; rb[i][1] = sb * im;
     }
   };
   auto store_tiles = [&](int buf) {
@@ -247,6 +249,7 @@ static int zero_fill(int64_t n, void* p, hipStream_t s) {
 // fill and T2 no zeros outside what is read. One wave per row l of a unit,
 // the lanes over the unit's columns r' (coalesced).
 constexpr int MIX_LIST = 32;   // d * Mr and Ml * d are at most 32
+enum { PLAN_HEFF = SECTORS_HEFF, PLAN_ENV_RIGHT = SECTORS_ENV_RIGHT, PLAN_ENV_LEFT = SECTORS_ENV_LEFT };
 
 struct MixUnit {
   int64_t l0, nl, r0, nr;
@@ -260,7 +263,7 @@ __global__ __launch_bounds__(256) void k_mix_sectors(const MixUnit* __restrict__
                                                      int ni_all, int no_all, int64_t nr_all,
                                                      const double2* __restrict__ wm,
                                                      const TX* __restrict__ x, TY* __restrict__ y,
-                                                     const int* __restrict__ skip) {
+                                                     const int* __restrict__ skip, int wtrans) {
   __shared__ double2 W[MIX_LIST * MIX_LIST];
   if (skip && skip[0]) return;
   for (int t = threadIdx.x; t < ni_all * no_all; t += blockDim.x) W[t] = wm[t];
@@ -292,7 +295,8 @@ __global__ __launch_bounds__(256) void k_mix_sectors(const MixUnit* __restrict__
 #pragma unroll
       for (int k = 0; k < MIX_LIST; ++k) {
         if (k < ni) {
-          const double2 w = W[o * ni_all + u->il[k]];
+          // Wm[o, i], or stored transposed as W[i, o] (left environment)
+          const double2 w = wtrans ? W[u->il[k] * no_all + o] : W[o * ni_all + u->il[k]];
           yr += w.x * xr[k] - w.y * xi[k];
           yi += w.x * xi[k] + w.y * xr[k];
         }
@@ -329,6 +333,7 @@ struct GProduct {
 };
 
 struct SectorPlan {
+  int kind = PLAN_HEFF;
   GProduct p1, p3;
   std::vector<MixUnit> mix;
   int64_t mix_tiles = 0;
@@ -367,74 +372,160 @@ static void finish_product(GProduct& P, size_t csz_a, size_t csz_b, size_t csz_c
   P.tiles = tiles;
 }
 
-// blocks of both products; false when a bond is not sector-contiguous, the MPO
-// bond exceeds the segment limit or the launch would be too large
-static bool build_plan(const Dims& D, const int64_t* ql, const int64_t* qs, const int64_t* qr,
-                       const int64_t* qml, const int64_t* qmr, size_t csz, SectorPlan& sp) {
-  if (D.Ml > GMAXSEG || D.Dl <= 0 || D.Dr <= 0 || D.d <= 0 || D.Mr <= 0 ||
-      D.d * D.Mr > MIX_LIST || D.Ml * D.d > MIX_LIST)
-    return false;
-  std::map<int64_t, Sector> sl, sr;
-  if (!sectors_of(ql, D.Dl, sl) || !sectors_of(qr, D.Dr, sr)) return false;
-  // T1[(l, s), (m', r')] = sum_r A[(l, s), r] R[r, (m', r')]
+using SectorMap = std::map<int64_t, Sector>;
+
+static GBlock block(int64_t M, int64_t N, int64_t c_off) {
+  GBlock b{};
+  b.M = M;
+  b.N = N;
+  b.c_off = c_off;
+  return b;
+}
+static void add_segment(GBlock& b, int64_t a_off, int64_t b_off, int64_t K) {
+  b.a_off[b.nseg] = a_off;
+  b.b_off[b.nseg] = b_off;
+  b.K[b.nseg] = K;
+  ++b.nseg;
+}
+
+// T1[(l, s), (m', r')] = sum_r A[(l, s), r] R[r, (m', r')] (H_eff and the
+// right environment step): block per (s, right sector Q of r, m')
+static void plan_a_times_r(const Dims& D, const int64_t* qs, const int64_t* qmr,
+                           const SectorMap& sl, const SectorMap& sr, GProduct& P) {
   for (int64_t s = 0; s < D.d; ++s)
     for (const auto& kq : sr) {
-      const int64_t Q = kq.first;
-      const auto itl = sl.find(Q - qs[s]);
+      const auto itl = sl.find(kq.first - qs[s]);
       if (itl == sl.end()) continue;
       for (int64_t mp = 0; mp < D.Mr; ++mp) {
-        const auto itc = sr.find(Q + qmr[mp]);
+        const auto itc = sr.find(kq.first + qmr[mp]);
         if (itc == sr.end()) continue;
-        GBlock b{};
-        b.M = itl->second.len;
-        b.N = itc->second.len;
         const int64_t l0 = itl->second.start, r0 = kq.second.start, c0 = itc->second.start;
-        b.c_off = (l0 * D.d + s) * (D.Mr * D.Dr) + mp * D.Dr + c0;
-        b.nseg = 1;
-        b.a_off[0] = (l0 * D.d + s) * D.Dr + r0;
-        b.b_off[0] = r0 * (D.Mr * D.Dr) + mp * D.Dr + c0;
-        b.K[0] = kq.second.len;
-        sp.p1.blocks.push_back(b);
+        GBlock b = block(itl->second.len, itc->second.len,
+                         (l0 * D.d + s) * (D.Mr * D.Dr) + mp * D.Dr + c0);
+        add_segment(b, (l0 * D.d + s) * D.Dr + r0, r0 * (D.Mr * D.Dr) + mp * D.Dr + c0,
+                    kq.second.len);
+        P.blocks.push_back(b);
       }
     }
-  // out[lb, (s', r')] = sum_{(l, m)} L[(l, m), lb] T2[(l, m), (s', r')]
+}
+
+// H_eff: out[lb, (s', r')] = sum_{(l, m)} L[(l, m), lb] T2[(l, m), (s', r')]:
+// block per (left sector P of lb, s'), one k-segment per MPO index m
+static void plan_heff_out(const Dims& D, const int64_t* qs, const int64_t* qml,
+                          const SectorMap& sl, const SectorMap& sr, GProduct& P) {
   for (const auto& pq : sl) {
-    const int64_t P = pq.first, lb0 = pq.second.start;
-    for (int64_t sp2 = 0; sp2 < D.d; ++sp2) {
-      const auto itc = sr.find(P + qs[sp2]);
+    const int64_t lb0 = pq.second.start;
+    for (int64_t s2 = 0; s2 < D.d; ++s2) {
+      const auto itc = sr.find(pq.first + qs[s2]);
       if (itc == sr.end()) continue;
-      GBlock b{};
-      b.M = pq.second.len;
-      b.N = itc->second.len;
       const int64_t c0 = itc->second.start;
-      b.c_off = lb0 * (D.d * D.Dr) + sp2 * D.Dr + c0;
-      int ns = 0;
+      GBlock b = block(pq.second.len, itc->second.len, lb0 * (D.d * D.Dr) + s2 * D.Dr + c0);
       for (int64_t m = 0; m < D.Ml; ++m) {
-        const auto itl = sl.find(P - qml[m]);
+        const auto itl = sl.find(pq.first - qml[m]);
         if (itl == sl.end()) continue;
         const int64_t l0 = itl->second.start;
-        b.a_off[ns] = (l0 * D.Ml + m) * D.Dl + lb0;
-        b.b_off[ns] = (l0 * D.Ml + m) * (D.d * D.Dr) + sp2 * D.Dr + c0;
-        b.K[ns] = itl->second.len;
-        ++ns;
+        add_segment(b, (l0 * D.Ml + m) * D.Dl + lb0, (l0 * D.Ml + m) * (D.d * D.Dr) + s2 * D.Dr + c0,
+                    itl->second.len);
       }
-      if (ns == 0) continue;   // structurally zero output block
-      b.nseg = ns;
-      sp.p3.blocks.push_back(b);
+      if (b.nseg) P.blocks.push_back(b);   // no segment: a structurally zero block
     }
   }
-  // mix units: (left sector P, right sector Q) with outputs read by the second product
+}
+
+// right environment: out[(l, m), lb] = sum_{(s', r')} T2[(l, m), (s', r')] conj(B[lb, (s', r')]):
+// block per (left sector P of l, m), one k-segment per physical index s'
+static void plan_envr_out(const Dims& D, const int64_t* qs, const int64_t* qml,
+                          const SectorMap& sl, const SectorMap& sr, GProduct& P) {
+  for (const auto& pq : sl) {
+    const int64_t l0 = pq.second.start;
+    for (int64_t m = 0; m < D.Ml; ++m) {
+      const auto itb = sl.find(pq.first + qml[m]);
+      if (itb == sl.end()) continue;
+      const int64_t lb0 = itb->second.start;
+      GBlock b = block(pq.second.len, itb->second.len, (l0 * D.Ml + m) * D.Dl + lb0);
+      for (int64_t s2 = 0; s2 < D.d; ++s2) {
+        const auto itr = sr.find(itb->first + qs[s2]);
+        if (itr == sr.end()) continue;
+        const int64_t rb0 = itr->second.start;
+        add_segment(b, (l0 * D.Ml + m) * (D.d * D.Dr) + s2 * D.Dr + rb0,
+                    lb0 * (D.d * D.Dr) + s2 * D.Dr + rb0, itr->second.len);
+      }
+      if (b.nseg) P.blocks.push_back(b);
+    }
+  }
+}
+
+// left environment: T1[(l, m), (s', r')] = sum_lb L[(l, m), lb] conj(B[lb, (s', r')]):
+// block per (left sector P of l, m, s')
+static void plan_l_times_b(const Dims& D, const int64_t* qs, const int64_t* qml,
+                           const SectorMap& sl, const SectorMap& sr, GProduct& P) {
+  for (const auto& pq : sl) {
+    const int64_t l0 = pq.second.start;
+    for (int64_t m = 0; m < D.Ml; ++m) {
+      const auto itb = sl.find(pq.first + qml[m]);
+      if (itb == sl.end()) continue;
+      const int64_t lb0 = itb->second.start;
+      for (int64_t s2 = 0; s2 < D.d; ++s2) {
+        const auto itr = sr.find(itb->first + qs[s2]);
+        if (itr == sr.end()) continue;
+        const int64_t rb0 = itr->second.start;
+        GBlock b = block(pq.second.len, itr->second.len,
+                         (l0 * D.Ml + m) * (D.d * D.Dr) + s2 * D.Dr + rb0);
+        add_segment(b, (l0 * D.Ml + m) * D.Dl + lb0, lb0 * (D.d * D.Dr) + s2 * D.Dr + rb0,
+                    itb->second.len);
+        P.blocks.push_back(b);
+      }
+    }
+  }
+}
+
+// left environment: out[r, (m', r')] = sum_{(l, s)} A[(l, s), r] T2[(l, s), (m', r')]:
+// block per (right sector Q of r, m'), one k-segment per physical index s
+static void plan_envl_out(const Dims& D, const int64_t* qs, const int64_t* qmr,
+                          const SectorMap& sl, const SectorMap& sr, GProduct& P) {
+  for (const auto& rq : sr) {
+    const int64_t r0 = rq.second.start;
+    for (int64_t mp = 0; mp < D.Mr; ++mp) {
+      const auto itc = sr.find(rq.first + qmr[mp]);
+      if (itc == sr.end()) continue;
+      const int64_t c0 = itc->second.start;
+      GBlock b = block(rq.second.len, itc->second.len, r0 * (D.Mr * D.Dr) + mp * D.Dr + c0);
+      for (int64_t s = 0; s < D.d; ++s) {
+        const auto itl = sl.find(rq.first - qs[s]);
+        if (itl == sl.end()) continue;
+        const int64_t l0 = itl->second.start;
+        add_segment(b, (l0 * D.d + s) * D.Dr + r0, (l0 * D.d + s) * (D.Mr * D.Dr) + mp * D.Dr + c0,
+                    itl->second.len);
+      }
+      if (b.nseg) P.blocks.push_back(b);
+    }
+  }
+}
+
+// mix units for a unit (left sector P, right sector Q, dq = Q - P):
+// H_eff / right environment: inputs i = (s, m') with qs[s] + qmr[m'] = dq and
+// k-sector P + qs[s] present, outputs o = (m, s') with qml[m] + qs[s'] = dq and
+// P + qml[m] present; left environment: inputs i = (m, s') with
+// qml[m] + qs[s'] = dq and P + qml[m] present, outputs o = (s, m') with
+// qs[s] + qmr[m'] = dq and P + qs[s] present.
+static void plan_mix(const Dims& D, bool left, const int64_t* qs, const int64_t* qml,
+                     const int64_t* qmr, const SectorMap& sl, const SectorMap& sr,
+                     SectorPlan& sp) {
   for (const auto& pq : sl)
     for (const auto& rq : sr) {
       const int64_t P = pq.first, dq = rq.first - P;
       MixUnit u{};
+      int8_t* sm_list = left ? u.il : u.ol;    // (m, s') indices
+      int8_t* ms_list = left ? u.ol : u.il;    // (s, m') indices
+      int32_t& n_sm = left ? u.ni : u.no;
+      int32_t& n_ms = left ? u.no : u.ni;
       for (int64_t m = 0; m < D.Ml; ++m)
         for (int64_t s2 = 0; s2 < D.d; ++s2)
-          if (qml[m] + qs[s2] == dq && sl.count(P + qml[m])) u.ol[u.no++] = (int8_t)(m * D.d + s2);
-      if (u.no == 0) continue;
+          if (qml[m] + qs[s2] == dq && sl.count(P + qml[m])) sm_list[n_sm++] = (int8_t)(m * D.d + s2);
       for (int64_t s1 = 0; s1 < D.d; ++s1)
         for (int64_t mp = 0; mp < D.Mr; ++mp)
-          if (qs[s1] + qmr[mp] == dq && sr.count(P + qs[s1])) u.il[u.ni++] = (int8_t)(s1 * D.Mr + mp);
+          if (qs[s1] + qmr[mp] == dq && sr.count(P + qs[s1])) ms_list[n_ms++] = (int8_t)(s1 * D.Mr + mp);
+      if (u.no == 0) continue;   // nothing of this unit is read
       u.l0 = pq.second.start;
       u.nl = pq.second.len;
       u.r0 = rq.second.start;
@@ -444,8 +535,39 @@ static bool build_plan(const Dims& D, const int64_t* ql, const int64_t* qs, cons
       sp.mix_bytes += (double)u.nl * (double)u.nr * (double)(u.ni + u.no);
       sp.mix.push_back(u);
     }
-  finish_product(sp.p1, 16, csz, csz);   // A is complex128, R and T1 in the compute type
-  finish_product(sp.p3, csz, csz, 16);   // out is complex128
+}
+
+// blocks of both products and the mix units of a plan kind; false when a bond
+// is not sector-contiguous, a segment or mix list would exceed its limit, or
+// the launch would be too large
+static bool build_plan(int kind, const Dims& D, const int64_t* ql, const int64_t* qs,
+                       const int64_t* qr, const int64_t* qml, const int64_t* qmr, size_t csz,
+                       SectorPlan& sp) {
+  if (D.Dl <= 0 || D.Dr <= 0 || D.d <= 0 || D.Ml <= 0 || D.Mr <= 0 ||
+      D.d * D.Mr > MIX_LIST || D.Ml * D.d > MIX_LIST)
+    return false;
+  if ((kind == PLAN_HEFF && D.Ml > GMAXSEG) || (kind != PLAN_HEFF && D.d > GMAXSEG)) return false;
+  SectorMap sl, sr;
+  if (!sectors_of(ql, D.Dl, sl) || !sectors_of(qr, D.Dr, sr)) return false;
+  sp.kind = kind;
+  if (kind == PLAN_ENV_LEFT) {
+    plan_l_times_b(D, qs, qml, sl, sr, sp.p1);
+    plan_mix(D, true, qs, qml, qmr, sl, sr, sp);
+    plan_envl_out(D, qs, qmr, sl, sr, sp.p3);
+    finish_product(sp.p1, csz, 16, csz);   // L in the compute type, B complex128
+    finish_product(sp.p3, 16, csz, 16);    // A and out complex128
+  } else {
+    plan_a_times_r(D, qs, qmr, sl, sr, sp.p1);
+    plan_mix(D, false, qs, qml, qmr, sl, sr, sp);
+    if (kind == PLAN_HEFF) {
+      plan_heff_out(D, qs, qml, sl, sr, sp.p3);
+      finish_product(sp.p3, csz, csz, 16);
+    } else {
+      plan_envr_out(D, qs, qml, sl, sr, sp.p3);
+      finish_product(sp.p3, csz, 16, 16);
+    }
+    finish_product(sp.p1, 16, csz, csz);   // A complex128, R and T1 in the compute type
+  }
   return sp.p1.tiles > 0 && sp.p3.tiles > 0 && sp.p1.tiles < 0x7fffffff &&
          sp.p3.tiles < 0x7fffffff && !sp.mix.empty() && sp.mix_tiles < 0x7fffffff;
 }
@@ -462,11 +584,12 @@ static void append(std::string& k, const int64_t* p, int64_t n) {
   k.append((const char*)p, (size_t)n * sizeof(int64_t));
 }
 
-// The sector plan of a local operator with quantum numbers (null when the
-// operator has none, is not H_eff, or the grouped products do not apply).
-// Builds and uploads it on first use (synchronous copy: call it outside
-// stream capture); plans are cached per device and set of quantum numbers.
-int sector_plan_get(const ptn_local_op* op, const SectorPlan** out) {
+// The sector plan of kind PLAN_* for a site operator with quantum numbers
+// (null when the operator has none, is not a site operator, or the grouped
+// products do not apply). Builds and uploads it on first use (synchronous
+// copy: call it outside stream capture); plans are cached per device, kind
+// and set of quantum numbers.
+int sector_plan_get(const ptn_local_op* op, int kind, const SectorPlan** out) {
   *out = nullptr;
   if (!op || op->kind != PTN_OP_HEFF || !(op->precision & PTN_BLOCK_SPARSE) || !op->q_l ||
       !op->q_s || !op->q_r || !op->q_ml || !op->q_mr)
@@ -477,7 +600,7 @@ int sector_plan_get(const ptn_local_op* op, const SectorPlan** out) {
   int dev = 0;
   PTN_HIP_CHECK(hipGetDevice(&dev));
   std::string key;
-  const int64_t head[7] = {dev, bp, op->Dl, op->d, op->Dr, op->Ml, op->Mr};
+  const int64_t head[8] = {dev, bp, kind, op->Dl, op->d, op->Dr, op->Ml, op->Mr};
   key.append((const char*)head, sizeof(head));
   append(key, op->q_l, op->Dl);
   append(key, op->q_s, op->d);
@@ -507,8 +630,8 @@ int sector_plan_get(const ptn_local_op* op, const SectorPlan** out) {
   }
   std::unique_ptr<SectorPlan> sp(new SectorPlan());
   const Dims D = {op->Dl, op->d, op->Dr, op->Ml, op->Mr};
-  if (!build_plan(D, op->q_l, op->q_s, op->q_r, op->q_ml, op->q_mr, bp == PTN_FP64 ? 16 : 8,
-                  *sp)) {
+  if (!build_plan(kind, D, op->q_l, op->q_s, op->q_r, op->q_ml, op->q_mr,
+                  bp == PTN_FP64 ? 16 : 8, *sp)) {
     g_plans.emplace(key, nullptr);
     return PTN_OK;
   }
@@ -580,7 +703,7 @@ int heff_apply_sectors(int prec, const Dims& D, const SectorPlan* sp, const void
   const int* skip = skip_flag().p;
   int st;
   GLaunch g1{b1, (int)sp->p1.blocks.size(), D.d * D.Dr, 1, D.Mr * D.Dr, 1, D.d * D.Mr * D.Dr, 1,
-             a, r, t1, skip};
+             a, r, t1, skip, 0};
   const int ni = (int)(D.d * D.Mr), no = (int)(D.Ml * D.d);
   const dim3 mgrid((unsigned)sp->mix_tiles);
   if (bp == PTN_FP64) {
@@ -588,21 +711,117 @@ int heff_apply_sectors(int prec, const Dims& D, const SectorPlan* sp, const void
       return st;
     tlaunch("k_mix_sectors<c128, c128>", 0.0, sp->mix_bytes * 16, k_mix_sectors<double2, double2>,
             mgrid, dim3(256), s, (const MixUnit*)sp->dev_mix, (int)sp->mix.size(), ni, no, D.Dr,
-            (const double2*)w, (const double2*)t1, (double2*)t2, skip);
+            (const double2*)w, (const double2*)t1, (double2*)t2, skip, 0);
   } else {
     if ((st = grouped_launch<float, double2, float2, float2, true, true>(g1, sp->p1, s)))
       return st;
     tlaunch("k_mix_sectors<c64, c64>", 0.0, sp->mix_bytes * 8, k_mix_sectors<float2, float2>,
             mgrid, dim3(256), s, (const MixUnit*)sp->dev_mix, (int)sp->mix.size(), ni, no, D.Dr,
-            (const double2*)w, (const float2*)t1, (float2*)t2, skip);
+            (const double2*)w, (const float2*)t1, (float2*)t2, skip, 0);
   }
   PTN_LAUNCH_CHECK();
   if ((st = zero_fill<double2>(nout, out, s))) return st;
   GLaunch g3{b3, (int)sp->p3.blocks.size(), 1, D.Ml * D.Dl, D.Ml * D.d * D.Dr, 1, D.d * D.Dr, 1,
-             l, t2, out, skip};
+             l, t2, out, skip, 0};
   if (bp == PTN_FP64)
     return grouped_launch<double, double2, double2, double2, false, true>(g3, sp->p3, s);
   return grouped_launch<float, float2, float2, double2, false, true>(g3, sp->p3, s);
+}
+
+// Right environment step through the grouped products (env_right's chain:
+// T1 = A . R, the mode mix, out = T2 . B^H), only the blocks computed and out
+// zero-filled outside them. R has type tr (complex128 in FP64, complex64 in
+// FP32); A, B and out are complex128. Uses T1 and T2 of the env workspace.
+int env_right_sectors(int prec, const Dims& D, const SectorPlan* sp, const void* a, const void* b,
+                      const void* w, const void* r, int tr, void* out, void* ws, size_t ws_bytes,
+                      hipStream_t s) {
+  const int bp = prec & ~PTN_BLOCK_SPARSE;
+  if (!sp || sp->kind != PLAN_ENV_RIGHT || (bp != PTN_FP64 && bp != PTN_FP32))
+    return PTN_ERR_UNSUPPORTED;
+  const int ct = bp == PTN_FP64 ? PTN_T_C128 : PTN_T_C64;
+  if (tr != ct) return PTN_ERR_UNSUPPORTED;
+  if (ws_bytes < env_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;
+  const size_t csz = bp == PTN_FP64 ? 16 : 8;
+  const size_t big = std::max((size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr),
+                              (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr)) * csz;
+  void* t1 = ws;
+  void* t2 = (char*)ws + ((big + 255) & ~(size_t)255);
+  const GBlock* b1 = sp->dev;
+  const GBlock* b3 = sp->dev + sp->p1.blocks.size();
+  const int* skip = skip_flag().p;
+  const int ni = (int)(D.d * D.Mr), no = (int)(D.Ml * D.d);
+  const dim3 mgrid((unsigned)sp->mix_tiles);
+  int st;
+  GLaunch g1{b1, (int)sp->p1.blocks.size(), D.d * D.Dr, 1, D.Mr * D.Dr, 1, D.d * D.Mr * D.Dr, 1,
+             a, r, t1, skip, 0};
+  // out[(l, m), lb] = sum T2[(l, m), (s', r')] conj(B[lb, (s', r')])
+  GLaunch g3{b3, (int)sp->p3.blocks.size(), D.Ml * D.d * D.Dr, 1, 1, D.d * D.Dr, D.Ml * D.Dl, 1,
+             t2, b, out, skip, 1};
+  if (bp == PTN_FP64) {
+    if ((st = grouped_launch<double, double2, double2, double2, true, true>(g1, sp->p1, s)))
+      return st;
+    tlaunch("k_mix_sectors<c128, c128>", 0.0, sp->mix_bytes * 16, k_mix_sectors<double2, double2>,
+            mgrid, dim3(256), s, (const MixUnit*)sp->dev_mix, (int)sp->mix.size(), ni, no, D.Dr,
+            (const double2*)w, (const double2*)t1, (double2*)t2, skip, 0);
+    PTN_LAUNCH_CHECK();
+    if ((st = zero_fill<double2>(D.Dl * D.Ml * D.Dl, out, s))) return st;
+    return grouped_launch<double, double2, double2, double2, true, false>(g3, sp->p3, s);
+  }
+  if ((st = grouped_launch<float, double2, float2, float2, true, true>(g1, sp->p1, s))) return st;
+  tlaunch("k_mix_sectors<c64, c64>", 0.0, sp->mix_bytes * 8, k_mix_sectors<float2, float2>, mgrid,
+          dim3(256), s, (const MixUnit*)sp->dev_mix, (int)sp->mix.size(), ni, no, D.Dr,
+          (const double2*)w, (const float2*)t1, (float2*)t2, skip, 0);
+  PTN_LAUNCH_CHECK();
+  if ((st = zero_fill<double2>(D.Dl * D.Ml * D.Dl, out, s))) return st;
+  return grouped_launch<float, float2, double2, double2, true, false>(g3, sp->p3, s);
+}
+
+// Left environment step through the grouped products (env_left's chain:
+// T1 = L . B*, the mode mix with the transposed MPO tensor, out = A^T . T2).
+// L has type tl; A, B and out are complex128.
+int env_left_sectors(int prec, const Dims& D, const SectorPlan* sp, const void* a, const void* b,
+                     const void* w, const void* l, int tl, void* out, void* ws, size_t ws_bytes,
+                     hipStream_t s) {
+  const int bp = prec & ~PTN_BLOCK_SPARSE;
+  if (!sp || sp->kind != PLAN_ENV_LEFT || (bp != PTN_FP64 && bp != PTN_FP32))
+    return PTN_ERR_UNSUPPORTED;
+  const int ct = bp == PTN_FP64 ? PTN_T_C128 : PTN_T_C64;
+  if (tl != ct) return PTN_ERR_UNSUPPORTED;
+  if (ws_bytes < env_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;
+  const size_t csz = bp == PTN_FP64 ? 16 : 8;
+  const size_t big = std::max((size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr),
+                              (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr)) * csz;
+  void* t1 = ws;
+  void* t2 = (char*)ws + ((big + 255) & ~(size_t)255);
+  const GBlock* b1 = sp->dev;
+  const GBlock* b3 = sp->dev + sp->p1.blocks.size();
+  const int* skip = skip_flag().p;
+  const int ni = (int)(D.Ml * D.d), no = (int)(D.d * D.Mr);
+  const dim3 mgrid((unsigned)sp->mix_tiles);
+  int st;
+  // T1[(l, m), (s', r')] = sum_lb L[(l, m), lb] conj(B[lb, (s', r')])
+  GLaunch g1{b1, (int)sp->p1.blocks.size(), D.Ml * D.Dl, 1, D.d * D.Dr, 1, D.Ml * D.d * D.Dr, 1,
+             l, b, t1, skip, 1};
+  // out[r, (m', r')] = sum_{(l, s)} A[(l, s), r] T2[(l, s), (m', r')]
+  GLaunch g3{b3, (int)sp->p3.blocks.size(), 1, D.d * D.Dr, D.d * D.Mr * D.Dr, 1, D.Mr * D.Dr, 1,
+             a, t2, out, skip, 0};
+  if (bp == PTN_FP64) {
+    if ((st = grouped_launch<double, double2, double2, double2, true, true>(g1, sp->p1, s)))
+      return st;
+    tlaunch("k_mix_sectors<c128, c128>", 0.0, sp->mix_bytes * 16, k_mix_sectors<double2, double2>,
+            mgrid, dim3(256), s, (const MixUnit*)sp->dev_mix, (int)sp->mix.size(), ni, no, D.Dr,
+            (const double2*)w, (const double2*)t1, (double2*)t2, skip, 1);
+    PTN_LAUNCH_CHECK();
+    if ((st = zero_fill<double2>(D.Dr * D.Mr * D.Dr, out, s))) return st;
+    return grouped_launch<double, double2, double2, double2, false, true>(g3, sp->p3, s);
+  }
+  if ((st = grouped_launch<float, float2, double2, float2, true, true>(g1, sp->p1, s))) return st;
+  tlaunch("k_mix_sectors<c64, c64>", 0.0, sp->mix_bytes * 8, k_mix_sectors<float2, float2>, mgrid,
+          dim3(256), s, (const MixUnit*)sp->dev_mix, (int)sp->mix.size(), ni, no, D.Dr,
+          (const double2*)w, (const float2*)t1, (float2*)t2, skip, 1);
+  PTN_LAUNCH_CHECK();
+  if ((st = zero_fill<double2>(D.Dr * D.Mr * D.Dr, out, s))) return st;
+  return grouped_launch<float, double2, float2, double2, false, true>(g3, sp->p3, s);
 }
 
 }  // namespace ptn

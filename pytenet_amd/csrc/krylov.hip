@@ -529,7 +529,7 @@ int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, do
   // the caller's capture only a cached plan can be used
   const SectorPlan* sp = nullptr;
   if (batch == 1 && cst == hipStreamCaptureStatusNone) {
-    const int e = sector_plan_get(op, &sp);
+    const int e = sector_plan_get(op, SECTORS_HEFF, &sp);
     if (e) return e;
   }
   if (!use_graph)

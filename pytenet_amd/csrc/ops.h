@@ -29,9 +29,18 @@ int convert_c128_c64(int64_t n, const void* x, void* y, hipStream_t s);
 // quantum numbers); built and uploaded synchronously on first use, so look it
 // up outside stream capture
 struct SectorPlan;
-int sector_plan_get(const ptn_local_op* op, const SectorPlan** out);
+enum { SECTORS_HEFF = 0, SECTORS_ENV_RIGHT = 1, SECTORS_ENV_LEFT = 2 };
+int sector_plan_get(const ptn_local_op* op, int kind, const SectorPlan** out);
 int heff_apply_sectors(int prec, const Dims& D, const SectorPlan* sp, const void* l, int tl,
                        const void* w, const void* r, int tr, const void* a, void* out, void* ws,
                        size_t ws_bytes, hipStream_t s);
+// environment steps (site operator op: Dl, d, Dr, Ml, Mr, W and labels);
+// workspace of env_ws_bytes
+int env_right_sectors(int prec, const Dims& D, const SectorPlan* sp, const void* a, const void* b,
+                      const void* w, const void* r, int tr, void* out, void* ws, size_t ws_bytes,
+                      hipStream_t s);
+int env_left_sectors(int prec, const Dims& D, const SectorPlan* sp, const void* a, const void* b,
+                     const void* w, const void* l, int tl, void* out, void* ws, size_t ws_bytes,
+                     hipStream_t s);
 
 }  // namespace ptn

@@ -46,12 +46,14 @@ def dmrg_singlesite(hamiltonian: MPO, psi: MPS, numsweeps: int, numiter_lanczos:
             en, psi.a[i] = ctx.site_eigh(psi.a[i], lblocks[i], W[i], rblocks[i], ctx.q1(psi, i))
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_local_orthonormalize_left_qr(
                 psi.a[i], psi.a[i + 1], psi.qsite, psi.qbonds[i:i + 2])
-            lblocks[i + 1] = ctx.env_left(psi.a[i], W[i], lblocks[i])
+            lblocks[i + 1] = ctx.env_left(psi.a[i], W[i], lblocks[i],
+                                          ctx.q1(psi, i))
         for i in reversed(range(1, nsites)):
             en, psi.a[i] = ctx.site_eigh(psi.a[i], lblocks[i], W[i], rblocks[i], ctx.q1(psi, i))
             psi.a[i], psi.a[i - 1], psi.qbonds[i] = mps_local_orthonormalize_right_qr(
                 psi.a[i], psi.a[i - 1], psi.qsite, psi.qbonds[i:i + 2])
-            rblocks[i - 1] = ctx.env_right(psi.a[i], W[i], rblocks[i])
+            rblocks[i - 1] = ctx.env_right(psi.a[i], W[i], rblocks[i],
+                                           ctx.q1(psi, i))
         psi.a[0], _, psi.qbonds[0] = mps_local_orthonormalize_right_qr(
             psi.a[0], ctx.one(), psi.qsite, psi.qbonds[:2])
         ctx.flush()
@@ -86,7 +88,8 @@ def dmrg_twosite(hamiltonian: MPO, psi: MPS, numsweeps: int, numiter_lanczos: in
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "right", tol=tol_split,
                 max_bond=max_bond, svd_precond32=pre32)
-            lblocks[i + 1] = ctx.env_left(psi.a[i], W[i], lblocks[i])
+            lblocks[i + 1] = ctx.env_left(psi.a[i], W[i], lblocks[i],
+                                          ctx.q1(psi, i))
         for i in reversed(range(nsites - 1)):
             a2 = mps_merge_tensor_pair(psi.a[i], psi.a[i + 1])
             en, a2 = ctx.site_eigh(a2, lblocks[i], h2[i], rblocks[i + 1], ctx.q2(psi, i))
@@ -94,7 +97,8 @@ def dmrg_twosite(hamiltonian: MPO, psi: MPS, numsweeps: int, numiter_lanczos: in
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=tol_split,
                 max_bond=max_bond, svd_precond32=pre32)
-            rblocks[i] = ctx.env_right(psi.a[i + 1], W[i + 1], rblocks[i + 1])
+            rblocks[i] = ctx.env_right(psi.a[i + 1], W[i + 1], rblocks[i + 1],
+                                       ctx.q1(psi, i + 1))
         psi.a[0], _, psi.qbonds[0] = mps_local_orthonormalize_right_qr(
             psi.a[0], ctx.one(), psi.qsite, psi.qbonds[:2])
         ctx.flush()

@@ -77,11 +77,11 @@ class SweepContext:
         self._status.append(info)
         return en, out
 
-    def env_left(self, a, w, l):
-        return backend.env_left(a, a, w, l, self.precision, self.block_sparse)
+    def env_left(self, a, w, l, qnums=None):
+        return backend.env_left(a, a, w, l, self.precision, self.block_sparse, qnums)
 
-    def env_right(self, a, w, r):
-        return backend.env_right(a, a, w, r, self.precision, self.block_sparse)
+    def env_right(self, a, w, r, qnums=None):
+        return backend.env_right(a, a, w, r, self.precision, self.block_sparse, qnums)
 
     def flush(self):
         """Read the Krylov status words of the calls so far (one synchronisation)
@@ -98,7 +98,8 @@ class SweepContext:
         rblocks = [None] * n
         rblocks[n - 1] = self.one()
         for i in reversed(range(n - 1)):
-            rblocks[i] = self.env_right(psi.a[i + 1], self.W[i + 1], rblocks[i + 1])
+            rblocks[i] = self.env_right(psi.a[i + 1], self.W[i + 1], rblocks[i + 1],
+                                        self.q1(psi, i + 1))
         lblocks = [None] * n
         lblocks[0] = self.one()
         for i, rb in enumerate(rblocks):

@@ -47,7 +47,8 @@ def tdvp_singlesite(hamiltonian: MPO, psi: MPS, dt, numsteps: int, numiter_lancz
                 psi.a[i].reshape(s[0] * s[1], s[2]),
                 qnumber_flatten((psi.qbonds[i], psi.qsite)), psi.qbonds[i + 1])
             psi.a[i] = q.reshape(s[0], s[1], q.shape[1])
-            lblocks[i + 1] = ctx.env_left(psi.a[i], W[i], lblocks[i])
+            lblocks[i + 1] = ctx.env_left(psi.a[i], W[i], lblocks[i],
+                                          ctx.q1(psi, i))
             c = ctx.bond_expm(c, lblocks[i + 1], rblocks[i], 0.5 * dt)
             psi.a[i + 1] = absorb_left(c, psi.a[i + 1])
         i = nsites - 1
@@ -61,7 +62,8 @@ def tdvp_singlesite(hamiltonian: MPO, psi: MPS, dt, numsteps: int, numiter_lancz
                 qnumber_flatten((-psi.qbonds[i + 1], psi.qsite)), -psi.qbonds[i])
             psi.qbonds[i] = -qbond
             psi.a[i] = flip(q.reshape(s[0], s[1], q.shape[1]))
-            rblocks[i - 1] = ctx.env_right(psi.a[i], W[i], rblocks[i])
+            rblocks[i - 1] = ctx.env_right(psi.a[i], W[i], rblocks[i],
+                                           ctx.q1(psi, i))
             c = c.transpose(0, 1).contiguous()
             c = ctx.bond_expm(c, lblocks[i], rblocks[i - 1], 0.5 * dt)
             psi.a[i - 1] = absorb_right(psi.a[i - 1], c)
@@ -106,7 +108,8 @@ class TwoSiteTDVP:
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, (psi.qbonds[i], psi.qbonds[i + 2]), "right", tol=self.tol,
                 max_bond=self.max_bond, svd_precond32=self.pre32)
-            lblocks[i + 1] = ctx.env_left(psi.a[i], W[i], lblocks[i])
+            lblocks[i + 1] = ctx.env_left(psi.a[i], W[i], lblocks[i],
+                                          ctx.q1(psi, i))
             psi.a[i + 1] = ctx.site_expm(psi.a[i + 1], lblocks[i + 1], W[i + 1], rblocks[i + 1],
                                          0.5 * dt, ctx.q1(psi, i + 1))
         i = n - 2
@@ -116,7 +119,8 @@ class TwoSiteTDVP:
         psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
             a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=self.tol,
             max_bond=self.max_bond, svd_precond32=self.pre32)
-        rblocks[i] = ctx.env_right(psi.a[i + 1], W[i + 1], rblocks[i + 1])
+        rblocks[i] = ctx.env_right(psi.a[i + 1], W[i + 1], rblocks[i + 1],
+                                   ctx.q1(psi, i + 1))
         for i in reversed(range(n - 2)):
             psi.a[i + 1] = ctx.site_expm(psi.a[i + 1], lblocks[i + 1], W[i + 1], rblocks[i + 1],
                                          0.5 * dt, ctx.q1(psi, i + 1))
@@ -126,7 +130,8 @@ class TwoSiteTDVP:
             psi.a[i], psi.a[i + 1], psi.qbonds[i + 1] = mps_split_tensor_svd(
                 a2, qs, qs, [psi.qbonds[i], psi.qbonds[i + 2]], "left", tol=self.tol,
                 max_bond=self.max_bond, svd_precond32=self.pre32)
-            rblocks[i] = ctx.env_right(psi.a[i + 1], W[i + 1], rblocks[i + 1])
+            rblocks[i] = ctx.env_right(psi.a[i + 1], W[i + 1], rblocks[i + 1],
+                                       ctx.q1(psi, i + 1))
         ctx.flush()
 
     def finish(self):

@@ -1,10 +1,11 @@
 """
-Parity of the sector-grouped H_eff (one grouped launch per product over the
-admissible U(1) sector blocks; ptn_local_apply and the local solves with
-quantum numbers, called through the C ABI) with the oracle's
-apply_local_hamiltonian (pytenet/chain_ops.py:237-279), with the dense and
-the tile-mask products, and of the solves and a U(1) two-site DMRG run through
-them with the oracle. FP64: <= 1e-12 of the oracle's max norm and <= 1e-13 of
+Parity of the sector-grouped H_eff and environment steps (one grouped launch
+per product over the admissible U(1) sector blocks; ptn_local_apply,
+ptn_env_apply and the local solves with quantum numbers, called through the
+C ABI) with the oracle's apply_local_hamiltonian and environment steps
+(pytenet/chain_ops.py:16-99, 237-279), with the dense and the tile-mask
+products, and of the solves and a U(1) two-site DMRG run through them with the
+oracle. FP64: <= 1e-12 of the oracle's max norm and <= 1e-13 of
 the dense product's; FP32: <= 2e-5. Operators are the XXZ MPO with its quantum
 numbers (one and two sites), bonds sector-sorted as the splits leave them;
 unsorted bonds fall back to the tile-mask products.
@@ -85,6 +86,34 @@ def test_grouped_heff_matches_oracle_and_dense(Dl, Dr, two_site, precision):
     ql, qs, qr = q[0], q[1], q[2]
     off = np.add.outer(np.add.outer(ql, qs), -qr) != 0
     assert np.all(backend.to_host(out)[off] == 0)
+
+
+@pytest.mark.parametrize("Dl,Dr,two_site", [(200, 180, False), (260, 300, True), (37, 41, True),
+                                            (1, 6, True), (64, 1, False)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_grouped_environment_steps_match_oracle_and_dense(Dl, Dr, two_site, precision):
+    """ptn_env_apply (both sides) with labels: grouped products, equal to the
+    oracle's contraction_operator_step_left / _right (pytenet/chain_ops.py:16-99)
+    and to the dense products, with a bra different from the ket."""
+    rng = np.random.default_rng(Dl * 5 + Dr + 1)
+    kw = {"ql": np.zeros(1, int)} if Dl == 1 else ({"qr": np.ones(1, int)} if Dr == 1 else {})
+    a, w, l, r, q = _operator(rng, Dl, Dr, two_site, **kw)
+    b = _qsparse(rng, a.shape, [q[0], q[1], -q[2]])
+    for side, blk, ref, dense_fn in (
+            (_lib.PTN_ENV_LEFT, l, orc.contraction_operator_step_left(a, b, w, l), backend.env_left),
+            (_lib.PTN_ENV_RIGHT, r, orc.contraction_operator_step_right(a, b, w, r),
+             backend.env_right)):
+        out, grouped = backend._env_sectors(side, backend.as_dev(a), backend.as_dev(b),
+                                            backend.as_dev(w), backend.as_dev(blk), q, precision,
+                                            ref.shape)
+        assert grouped
+        assert relerr(out, ref) < TOL[precision]
+        if precision == "fp64":
+            dense = backend.to_host(dense_fn(a, b, w, blk, precision))
+            assert np.abs(backend.to_host(out) - dense).max() <= 1e-13 * np.abs(dense).max()
+        # the public wrappers route labelled calls the same way
+        again = dense_fn(a, b, w, blk, precision, block_sparse=True, qnums=q)
+        assert np.array_equal(backend.to_host(again), backend.to_host(out))
 
 
 def test_grouped_heff_large_sectors_select_the_large_tile():
