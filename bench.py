@@ -14,10 +14,11 @@ orthonormalisation and environment blocks), with everything resident in HBM.
 
 Prints one JSON line. `value` is the mixed-precision sweep time (s/sweep,
 lower is better); the FP64 sweep time, the speed-up, |1 - fidelity| and the
-relative energy error are extra fields. `roofline` is measured live for the
-dominant contraction kernel (the two-site effective-Hamiltonian product) with
-HIP events on the stream it runs on; `cpu_baseline` times the oracle (NumPy
-restatement of the reference) on a bounded sample of the same workload.
+relative energy error are extra fields. `roofline` is measured live, over the
+timed sweeps, for the kernel with the largest share of the timed kernel time
+among those with a roofline (HIP events attached to every launch on the
+stream it runs on); `cpu_baseline` times the oracle (NumPy restatement of the
+reference) on a bounded sample of the same workload.
 """
 
 import argparse
@@ -116,13 +117,15 @@ def _kernel_bound(name):
     multiply-add, the peak of their arithmetic), the Jacobi apply kernel
     streams the rotated columns (HBM); the rotation solve is latency-bound and
     has no roofline."""
-    if name.startswith("zgemm_bf16x3_kernel"):
+    if name.startswith("zgemm_bf16x3"):
         return "mfma", PEAK_TFLOPS["bf16x3"], "TFLOP/s"
-    if name.startswith("zgemm_kernel<float") or name == "k_jac_gram<float>":
+    if (name.startswith("zgemm_kernel<float") or name.startswith("zgemm_grouped_kernel<float")
+            or name == "k_jac_gram<float>"):
         return "mfma", PEAK_TFLOPS["fp32"], "TFLOP/s"
-    if name.startswith("zgemm_kernel<double") or name == "k_jac_gram<double>":
+    if (name.startswith("zgemm_kernel<double") or name.startswith("zgemm_grouped_kernel<double")
+            or name == "k_jac_gram<double>"):
         return "mfma", PEAK_TFLOPS["fp64"], "TFLOP/s"
-    if name.startswith("k_jac_apply"):
+    if name.startswith("k_jac_apply") or name.startswith("k_mix_sectors"):
         return "hbm", HBM_PEAK_GBS, "GB/s"
     return None, None, None
 
@@ -148,10 +151,22 @@ def kernel_roofline(timings):
             "source": "HIP events around every launch on its stream, over the timed sweeps"}
 
 
-def top_kernels(timings, n=8):
+def top_kernels(timings, n=8, rates=False):
+    """The n timed kernel instances with the largest summed device time; with
+    `rates`, each with its achieved rate and fraction of its roofline."""
     total = sum(t["total_ms"] for t in timings) or 1.0
-    return [{"name": t["name"], "launches": t["launches"], "total_ms": round(t["total_ms"], 1),
-             "share": round(t["total_ms"] / total, 4)} for t in timings[:n]]
+    out = []
+    for t in timings[:n]:
+        row = {"name": t["name"], "launches": t["launches"], "total_ms": round(t["total_ms"], 1),
+               "share": round(t["total_ms"] / total, 4)}
+        bound, peak, unit = _kernel_bound(t["name"])
+        if rates and bound:
+            work = t["flop"] / 1e12 if bound == "mfma" else t["bytes"] / 1e9
+            ach = work / (t["total_ms"] * 1e-3)
+            row.update({"avg_us": round(t["avg_ms"] * 1e3, 2), "achieved": round(ach, 2),
+                        "unit": unit, "frac": round(ach / peak, 4)})
+        out.append(row)
+    return out
 
 
 def heff_roofline(D, prec, reps=10):
@@ -350,8 +365,10 @@ def workload_dmrg2_u1(args):
         return r
 
     dm.mps_split_tensor_svd = split_progress
+    from pytenet_amd import backend
     try:
         torch.cuda.synchronize()
+        backend.timing_begin()
         t0[0] = time.perf_counter()
         en = ptn.dmrg_twosite(H, psi, sweeps, numiter_lanczos=args.numiter or 25, tol_split=0,
                               max_bond=D)
@@ -359,13 +376,16 @@ def workload_dmrg2_u1(args):
         dt = time.perf_counter() - t0[0]
     finally:
         dm.mps_split_tensor_svd = split
+        timings = backend.timing_end()
     return {"workload": f"two-site DMRG with U(1) sectors, XXZ L={L} Delta=1, D_max={D}, "
                         f"{sweeps} sweep(s) incl. setup, FP64, sector-grouped products",
             "value": round(dt / sweeps, 2), "unit": "s/sweep", "total_s": round(dt, 2),
             "sectors_centre": int(len(u)), "largest_sector": int(cnt.max()),
             "energies": [float(e) for e in np.asarray(en).real],
             "bond_max": int(max(psi.bond_dims)),
-            "gpu_mem_peak_gib": round(torch.cuda.max_memory_allocated() / 2**30, 1)}
+            "gpu_mem_peak_gib": round(torch.cuda.max_memory_allocated() / 2**30, 1),
+            "roofline": kernel_roofline(timings),
+            "timed_kernels": top_kernels(timings, 10, rates=True)}
 
 
 def workload_chains(args):
@@ -686,8 +706,8 @@ def main():
         "site_updates_per_s": round(nupd / tmx, 3),
         "roofline": roof,
         "fp64_roofline": kernel_roofline(tim64),
-        "timed_kernels_mixed": top_kernels(timmx),
-        "timed_kernels_fp64": top_kernels(tim64),
+        "timed_kernels_mixed": top_kernels(timmx, rates=True),
+        "timed_kernels_fp64": top_kernels(tim64, rates=True),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.D, args.L, args.numiter)
