@@ -246,15 +246,16 @@ static int zero_fill(int64_t n, void* p, hipStream_t s) {
 // the first grouped product writes), and the second product reads T2[l, o, r']
 // only for the outputs o with qml[m] + qs[s'] = Q - P whose output sector
 // P + qml[m] exists. The mix reads and writes just those: T1 needs no zero
-// fill and T2 no zeros outside what is read. One wave per row l of a unit,
-// the lanes over the unit's columns r' (coalesced).
+// fill and T2 no zeros outside what is read. One wave per row l and 64-column
+// chunk of a unit, the lanes over the columns r' (coalesced).
 constexpr int MIX_LIST = 32;   // d * Mr and Ml * d are at most 32
 enum { PLAN_HEFF = SECTORS_HEFF, PLAN_ENV_RIGHT = SECTORS_ENV_RIGHT, PLAN_ENV_LEFT = SECTORS_ENV_LEFT };
 
 struct MixUnit {
   int64_t l0, nl, r0, nr;
-  int64_t tile0;                // first workgroup of the unit (4 rows per workgroup)
+  int64_t tile0;                // first workgroup of the unit
   int32_t ni, no;
+  int32_t chunks;               // 64-column chunks per row: a wave per (row, chunk)
   int8_t il[MIX_LIST], ol[MIX_LIST];
 };
 
@@ -276,13 +277,18 @@ __global__ __launch_bounds__(256) void k_mix_sectors(const MixUnit* __restrict__
   }
   const MixUnit* __restrict__ u = units + lo;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t row = (tile - u->tile0) * 4 + wave;
+  // one wave per (row, 64-column chunk) of the unit: short independent work
+  // items keep many loads in flight
+  const int64_t wi = (tile - u->tile0) * 4 + wave;
+  const int64_t row = wi / u->chunks;
   if (row >= u->nl) return;
+  const int64_t r = u->r0 + (wi % u->chunks) * 64 + lane;
+  if (r >= u->r0 + u->nr) return;
   const int64_t l = u->l0 + row;
   const int ni = u->ni, no = u->no;
   const TX* __restrict__ xl = x + l * ni_all * nr_all;
   TY* __restrict__ yl = y + l * no_all * nr_all;
-  for (int64_t r = u->r0 + lane; r < u->r0 + u->nr; r += 64) {
+  {
     double xr[MIX_LIST], xi[MIX_LIST];
 #pragma unroll
     for (int k = 0; k < MIX_LIST; ++k) {
@@ -531,7 +537,8 @@ static void plan_mix(const Dims& D, bool left, const int64_t* qs, const int64_t*
       u.r0 = rq.second.start;
       u.nr = rq.second.len;
       u.tile0 = sp.mix_tiles;
-      sp.mix_tiles += (u.nl + 3) / 4;
+      u.chunks = (int32_t)((u.nr + 63) / 64);
+      sp.mix_tiles += (u.nl * u.chunks + 3) / 4;
       sp.mix_bytes += (double)u.nl * (double)u.nr * (double)(u.ni + u.no);
       sp.mix.push_back(u);
     }
