@@ -179,9 +179,10 @@ int ptn_local_apply(const ptn_local_op* op, const void* a, void* out, int32_t* g
                           ws_bytes, stream);
   }
   if (op->kind != PTN_OP_HEFF) return PTN_ERR_ARG;
-  const SectorPlan* sp = nullptr;
+  SectorPlanRef spr;
   int st;
-  if ((st = sector_plan_get(op, SECTORS_HEFF, &sp))) return st;
+  if ((st = sector_plan_get(op, SECTORS_HEFF, &spr))) return st;
+  const SectorPlan* sp = spr.get();
   if (!sp)
     return ptn_heff_apply(op->precision, op->Dl, op->d, op->Dr, op->Ml, op->Mr, op->l, op->w,
                           op->r, a, out, ws, ws_bytes, stream);
@@ -249,9 +250,10 @@ int ptn_env_apply(const ptn_local_op* op, int side, const void* a, const void* b
   if (!op || op->kind != PTN_OP_HEFF || (side != PTN_ENV_LEFT && side != PTN_ENV_RIGHT))
     return PTN_ERR_ARG;
   const bool left = side == PTN_ENV_LEFT;
-  const SectorPlan* sp = nullptr;
+  SectorPlanRef spr;
   int st;
-  if ((st = sector_plan_get(op, left ? SECTORS_ENV_LEFT : SECTORS_ENV_RIGHT, &sp))) return st;
+  if ((st = sector_plan_get(op, left ? SECTORS_ENV_LEFT : SECTORS_ENV_RIGHT, &spr))) return st;
+  const SectorPlan* sp = spr.get();
   hipStream_t s = (hipStream_t)stream;
   if (!sp)
     return env_step_b(left ? 1 : 0, op->precision, 1, op->Dl, op->d, op->Dr, op->Ml, op->Mr, a,

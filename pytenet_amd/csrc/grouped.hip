@@ -339,6 +339,19 @@ struct GProduct {
 };
 
 struct SectorPlan {
+  SectorPlan() = default;
+  SectorPlan(const SectorPlan&) = delete;
+  SectorPlan& operator=(const SectorPlan&) = delete;
+  // the descriptors are freed with the plan's last reference (hipFree waits
+  // for the device, so no launch still reads them)
+  ~SectorPlan() {
+    if (dev) {
+      int cur = 0;
+      if (hipGetDevice(&cur) == hipSuccess && cur != device) (void)hipSetDevice(device);
+      (void)hipFree(dev);
+      if (cur != device) (void)hipSetDevice(cur);
+    }
+  }
   int kind = PLAN_HEFF;
   GProduct p1, p3;
   std::vector<MixUnit> mix;
@@ -581,7 +594,7 @@ static bool build_plan(int kind, const Dims& D, const int64_t* ql, const int64_t
 
 namespace {
 std::mutex g_plan_mu;
-std::unordered_map<std::string, std::unique_ptr<SectorPlan>> g_plans;   // nullptr: not applicable
+std::unordered_map<std::string, std::shared_ptr<SectorPlan>> g_plans;   // nullptr: not applicable
 uint64_t g_plan_clock = 0;
 constexpr size_t PLAN_CACHE_MAX = 1024;
 }  // namespace
@@ -596,8 +609,8 @@ static void append(std::string& k, const int64_t* p, int64_t n) {
 // products do not apply). Builds and uploads it on first use (synchronous
 // copy: call it outside stream capture); plans are cached per device, kind
 // and set of quantum numbers.
-int sector_plan_get(const ptn_local_op* op, int kind, const SectorPlan** out) {
-  *out = nullptr;
+int sector_plan_get(const ptn_local_op* op, int kind, SectorPlanRef* out) {
+  out->reset();
   if (!op || op->kind != PTN_OP_HEFF || !(op->precision & PTN_BLOCK_SPARSE) || !op->q_l ||
       !op->q_s || !op->q_r || !op->q_ml || !op->q_mr)
     return PTN_OK;
@@ -618,24 +631,20 @@ int sector_plan_get(const ptn_local_op* op, int kind, const SectorPlan** out) {
   auto it = g_plans.find(key);
   if (it != g_plans.end()) {
     if (it->second) it->second->stamp = ++g_plan_clock;
-    *out = it->second.get();
+    *out = it->second;
     return PTN_OK;
   }
   if (g_plans.size() >= PLAN_CACHE_MAX) {
-    // drop the older half of this device's plans (hipFree waits for the
-    // device, so no launch still reads them)
+    // drop the older half of this device's plans from the cache (a plan still
+    // held by a caller lives on until that caller releases it)
     std::vector<std::pair<uint64_t, std::string>> age;
     for (auto& kv : g_plans)
       if (!kv.second || kv.second->device == dev)
         age.emplace_back(kv.second ? kv.second->stamp : 0, kv.first);
     std::sort(age.begin(), age.end());
-    for (size_t i = 0; i < age.size() / 2; ++i) {
-      auto e = g_plans.find(age[i].second);
-      if (e->second && e->second->dev) (void)hipFree(e->second->dev);
-      g_plans.erase(e);
-    }
+    for (size_t i = 0; i < age.size() / 2; ++i) g_plans.erase(age[i].second);
   }
-  std::unique_ptr<SectorPlan> sp(new SectorPlan());
+  std::shared_ptr<SectorPlan> sp(new SectorPlan());
   const Dims D = {op->Dl, op->d, op->Dr, op->Ml, op->Mr};
   if (!build_plan(kind, D, op->q_l, op->q_s, op->q_r, op->q_ml, op->q_mr,
                   bp == PTN_FP64 ? 16 : 8, *sp)) {
@@ -651,14 +660,13 @@ int sector_plan_get(const ptn_local_op* op, int kind, const SectorPlan** out) {
                                      hipMemcpyHostToDevice);
   sp->dev_mix = (MixUnit*)((char*)sp->dev + bb);
   if (e == hipSuccess) e = hipMemcpy(sp->dev_mix, sp->mix.data(), mb, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    if (sp->dev) (void)hipFree(sp->dev);
-    ptn_set_last_hip_error(e);
-    return PTN_ERR_HIP;
-  }
   sp->device = dev;
+  if (e != hipSuccess) {
+    ptn_set_last_hip_error(e);
+    return PTN_ERR_HIP;   // the plan (and its allocation) is released here
+  }
   sp->stamp = ++g_plan_clock;
-  *out = sp.get();
+  *out = sp;
   g_plans.emplace(key, std::move(sp));
   return PTN_OK;
 }

@@ -525,13 +525,16 @@ int krylov_run(const ptn_local_op* op, int mode, const void* v, double dt_re, do
   if (hipStreamIsCapturing(s, &cst) != hipSuccess) cst = hipStreamCaptureStatusActive;
   if (cst != hipStreamCaptureStatusNone)
     use_graph = false;   // the caller captures (or the query failed): launch into its capture
-  // the sector plan (quantum-number blocks) is built outside any capture; under
-  // the caller's capture only a cached plan can be used
-  const SectorPlan* sp = nullptr;
+  // the sector plan (quantum-number blocks) is looked up or built here, outside
+  // any capture (building copies synchronously); under the caller's own capture
+  // the tile-mask products run. The reference keeps the plan alive until the
+  // solve is enqueued.
+  SectorPlanRef spr;
   if (batch == 1 && cst == hipStreamCaptureStatusNone) {
-    const int e = sector_plan_get(op, SECTORS_HEFF, &sp);
+    const int e = sector_plan_get(op, SECTORS_HEFF, &spr);
     if (e) return e;
   }
+  const SectorPlan* sp = spr.get();
   if (!use_graph)
     return krylov_enqueue(op, mode, v, dt_re, dt_im, numiter, out, energy, info, alpha_beta, ws,
                           ws_bytes, s, batch, sp);

@@ -1,5 +1,6 @@
 // Internal interfaces between the kernel files of the sweep library.
 #pragma once
+#include <memory>
 #include "common.h"
 
 namespace ptn {
@@ -29,8 +30,9 @@ int convert_c128_c64(int64_t n, const void* x, void* y, hipStream_t s);
 // quantum numbers); built and uploaded synchronously on first use, so look it
 // up outside stream capture
 struct SectorPlan;
+using SectorPlanRef = std::shared_ptr<const SectorPlan>;   // keeps a plan alive while in use
 enum { SECTORS_HEFF = 0, SECTORS_ENV_RIGHT = 1, SECTORS_ENV_LEFT = 2 };
-int sector_plan_get(const ptn_local_op* op, int kind, const SectorPlan** out);
+int sector_plan_get(const ptn_local_op* op, int kind, SectorPlanRef* out);
 int heff_apply_sectors(int prec, const Dims& D, const SectorPlan* sp, const void* l, int tl,
                        const void* w, const void* r, int tr, const void* a, void* out, void* ws,
                        size_t ws_bytes, hipStream_t s);
