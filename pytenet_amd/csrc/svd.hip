@@ -394,12 +394,14 @@ __global__ __launch_bounds__(256) void k_jac_apply(int64_t p, int64_t qp, int nb
     if (row0 >= ld) break;
     const int64_t arow = row0 + l16;
     const bool aok = arow < ld;
+    // each lane loads only the part (real or imaginary) it feeds to the MFMA;
+    // the wave's two lanes of an element read its two adjacent halves
+    const T* __restrict__ Mt = reinterpret_cast<const T*>(M);
     T a[16];
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
       const int kc = 2 * s + (kk >> 1);
-      const C2 y = aok ? M[pair_col(bp, bq, kc) * ld + arow] : C2{0, 0};
-      a[s] = q ? y.y : y.x;
+      a[s] = aok ? Mt[2 * (pair_col(bp, bq, kc) * ld + arow) + q] : T(0);
     }
     Acc acc[4];
 #pragma unroll
