@@ -594,7 +594,10 @@ static bool build_plan(int kind, const Dims& D, const int64_t* ql, const int64_t
 
 namespace {
 std::mutex g_plan_mu;
-std::unordered_map<std::string, std::shared_ptr<SectorPlan>> g_plans;   // nullptr: not applicable
+// nullptr: not applicable. Allocated once and never destroyed, so that no
+// plan is released (hipFree) during static destruction at process exit, when
+// the HIP runtime may already be gone.
+auto& g_plans = *new std::unordered_map<std::string, std::shared_ptr<SectorPlan>>();
 uint64_t g_plan_clock = 0;
 constexpr size_t PLAN_CACHE_MAX = 1024;
 }  // namespace
