@@ -14,6 +14,7 @@ int zgemm_fp32(const ZGemm& z, int ta, int tb, int tc, hipStream_t s, void* mws,
     case 4: return launch_typed<float, float2, double2, double2>(z, s, mws, mws_bytes);
     case 5: return launch_typed<float, float2, double2, float2>(z, s, mws, mws_bytes);
     case 6: return launch_typed<float, float2, float2, double2>(z, s, mws, mws_bytes);
+    case 7: return launch_typed<float, float2, float2, float2>(z, s, mws, mws_bytes);
   }
   return PTN_ERR_UNSUPPORTED;
 }

@@ -64,14 +64,23 @@ static size_t split2(int prec, int64_t batch, int64_t m1, int64_t n1, int64_t k1
   return align256(a > b ? a : b);
 }
 
+// FP32 mode: the complex128 input vector of H_eff converted to complex64 once,
+// so that the first product streams half the A-operand bytes
+static size_t a64_bytes(int prec, const Dims& D, int64_t batch) {
+  return base_prec(prec) == PTN_FP32 ? align256((size_t)batch * (size_t)(D.Dl * D.d * D.Dr) * 8)
+                                     : 0;
+}
+
 size_t heff_ws_bytes(int prec, const Dims& D, int64_t batch) {
   const size_t sp = split2(prec, batch, D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl, D.d * D.Dr,
                            D.Dl * D.Ml);
+  const size_t a64 = a64_bytes(prec, D, batch);
   prec = base_prec(prec);
   size_t t1 = (size_t)batch * (size_t)(D.Dl * D.d) * (size_t)(D.Mr * D.Dr) * csize(prec);
   size_t t2 = (size_t)batch * (size_t)(D.Dl * D.Ml) * (size_t)(D.d * D.Dr) * csize(prec);
   return align256(t1) + align256(t2) +
-         mask_bytes2(D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl, D.d * D.Dr, D.Dl * D.Ml) + 256 + sp;
+         mask_bytes2(D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl, D.d * D.Dr, D.Dl * D.Ml) + 256 + a64 +
+         sp;
 }
 
 size_t env_ws_bytes(int prec, const Dims& D, int64_t batch) {
@@ -115,6 +124,7 @@ int heff_apply(int prec, const Dims& D, const void* l, int tl, const void* w, co
   const size_t spb = split2(prec, batch, D.Dl * D.d, D.Mr * D.Dr, D.Dr, D.Dl, D.d * D.Dr,
                             D.Dl * D.Ml);
   void* sws = spb ? (char*)ws + (wtot - spb) : nullptr;
+  const size_t a64b = a64_bytes(prec, D, batch);
   const bool sp = is_sparse(prec) && batch == 1;
   prec = base_prec(prec);
   const int ct = ctype(prec), cp = gemm_compute(prec);
@@ -128,11 +138,19 @@ int heff_apply(int prec, const Dims& D, const void* l, int tl, const void* w, co
   void* mws = sp ? p : nullptr;
   const int64_t nA = D.Dl * D.d * D.Dr, nL = D.Dl * D.Ml * D.Dl, nR = D.Dr * D.Mr * D.Dr;
   int st;
+  const void* a1 = a;
+  int ta1 = ta;
+  if (a64b && ta == PTN_T_C128) {
+    void* a64 = (char*)ws + (wtot - spb - a64b);
+    if ((st = convert_c128_c64(batch * nA, a, a64, s))) return st;
+    a1 = a64;
+    ta1 = PTN_T_C64;
+  }
   ZGemm g1 = {D.Dl * D.d, D.Mr * D.Dr, D.Dr,
-              a, D.Dr, 1, 0,
+              a1, D.Dr, 1, 0,
               r, D.Mr * D.Dr, 1, 0,
               t1, D.Mr * D.Dr, 1, 0, batch, nA, nR, n1, sws, spb};
-  if ((st = zgemm(g1, cp, ta, tr, ct, mws, mb, s))) return st;
+  if ((st = zgemm(g1, cp, ta1, tr, ct, mws, mb, s))) return st;
   if ((st = mode_mix(D.Dl, D.d * D.Mr, D.Ml * D.d, D.Dr, w, t1, ct, t2, ct, s, batch,
                      D.Ml * D.d * D.d * D.Mr)))
     return st;
