@@ -92,3 +92,37 @@ def test_padded_product_state_is_the_product_of_its_site_vectors():
     # seeded: the same draws give the same state
     again = ptn.MPS.padded_product_state([0, 0], 7, 4, np.random.default_rng(3))
     assert all(np.array_equal(x, y) for x, y in zip(psi.a, again.a))
+
+
+def test_local_op_carries_the_site_labels_only_when_they_fit():
+    """The ptn_local_op built for a labelled operator points at the host label
+    arrays (int64, contiguous) in the order q_l, q_s, q_r, q_ml, q_mr; labels of
+    the wrong length are rejected before any call; without block sparsity or for
+    the bond operator no labels are passed (NULL: the dense / tile-mask path)."""
+    import ctypes
+    import pytest
+    import torch
+    from pytenet_amd import _lib
+    Dl, d, Dr, Ml, Mr = 5, 2, 7, 3, 4
+    l = torch.zeros((Dl, Ml, Dl), dtype=torch.complex128)
+    r = torch.zeros((Dr, Mr, Dr), dtype=torch.complex128)
+    w = torch.zeros((Ml, d, d, Mr), dtype=torch.complex128)
+    q = (np.arange(Dl), np.array([1, -1]), np.arange(Dr) - 3, np.array([0, 2, -2]),
+         np.array([2, 0, -2, 0]))
+    op, keep = backend._local_op(_lib.PTN_OP_HEFF, "fp64", l, w, r, Dl, d, Dr, True, q)
+    assert op.precision == _lib.PTN_FP64 | _lib.PTN_BLOCK_SPARSE
+    for name, k in zip(("q_l", "q_s", "q_r", "q_ml", "q_mr"), range(5)):
+        ptr = getattr(op, name)
+        got = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_int64)),
+                                    shape=(len(q[k]),))
+        assert np.array_equal(got, q[k])
+    with pytest.raises(AssertionError):
+        backend._local_op(_lib.PTN_OP_HEFF, "fp64", l, w, r, Dl, d, Dr, True,
+                          (q[0][:-1],) + q[1:])
+    op, keep = backend._local_op(_lib.PTN_OP_HEFF, "fp64", l, w, r, Dl, d, Dr, False, q)
+    assert keep is None and not op.q_l
+    op, keep = backend._local_op(_lib.PTN_OP_BOND, "fp64", l, None, l, Dl, 1, Dl, True, q)
+    assert keep is None and not op.q_l
+    # an environment step describes its site with only one of the two blocks
+    op, keep = backend._local_op(_lib.PTN_OP_HEFF, "fp32", l, w, None, Dl, d, Dr, True, q)
+    assert (op.Ml, op.Mr) == (Ml, Mr) and not op.r and op.q_mr
