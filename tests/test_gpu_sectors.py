@@ -228,3 +228,58 @@ def test_u1_dmrg_runs_the_grouped_products_and_matches_the_reference_trajectory(
         if not graphs:
             names = [x["name"] for x in rows]
             assert any(n.startswith("zgemm_grouped_kernel") for n in names), names
+
+
+def test_grouped_lanczos_fp32_matches_oracle():
+    """FP32 matvecs through the grouped products (complex64 environment blocks,
+    FP64 Lanczos scalars): the expm result against the oracle's FP64 Lanczos
+    within the FP32 bound."""
+    rng = np.random.default_rng(17)
+    a, w, l, r, q = _chain_operator(rng, True)
+    hfun = lambda x: orc.apply_local_hamiltonian(x.reshape(a.shape), w, l, r).reshape(-1)
+    dt = -0.05j
+    got, _, info = backend.krylov_local(_lib.PTN_OP_HEFF, a, l, w, r, 8, "expm", dt,
+                                        precision="fp32", block_sparse=True, qnums=q)
+    ref = orc.expm_krylov(hfun, a.reshape(-1), dt, 8, hermitian=True).reshape(a.shape)
+    assert relerr(got, ref) < TOL["fp32"]
+
+
+def test_environment_step_with_unsorted_bonds_falls_back():
+    rng = np.random.default_rng(23)
+    a, w, l, r, q = _operator(rng, 70, 90, True)
+    perm = rng.permutation(90)
+    qr = q[2][perm]
+    a, r = a[:, :, perm], r[perm][:, :, perm]
+    b = a.copy()
+    out, grouped = backend._env_sectors(_lib.PTN_ENV_RIGHT, backend.as_dev(a), backend.as_dev(b),
+                                        backend.as_dev(w), backend.as_dev(r),
+                                        (q[0], q[1], qr, q[3], q[4]), "fp64", (70, w.shape[0], 70))
+    assert not grouped
+    assert relerr(out, orc.contraction_operator_step_right(a, b, w, r)) < TOL["fp64"]
+
+
+def test_plan_cache_eviction_keeps_results_exact():
+    """More distinct label sets than the plan cache holds (1024): the cache
+    drops its older half on the way, and every application still equals the
+    oracle's (released plans are rebuilt on demand)."""
+    rng = np.random.default_rng(29)
+    H = heisenberg_xxz_1d_mpo(4, 1.0, 0.8, -0.1)
+    w = np.asarray(H.a[1], dtype=complex)
+    qd, qm_l, qm_r = np.asarray(H.qsite), np.asarray(H.qbonds[1]), np.asarray(H.qbonds[2])
+    for k in range(1100):
+        Dl = 6 + k % 5
+        ql = np.sort(rng.integers(-2, 3, Dl)) * 2
+        qr = np.sort(rng.integers(-2, 3, 7)) * 2 + 1
+        if k % 100 == 0:
+            a = _qsparse(rng, (Dl, 2, 7), [ql, qd, -qr])
+            l = _qsparse(rng, (Dl, len(qm_l), Dl), [ql, qm_l, -ql])
+            r = _qsparse(rng, (7, len(qm_r), 7), [qr, qm_r, -qr])
+            out, _ = backend.heff_apply_sectors(a, w, l, r, (ql, qd, qr, qm_l, qm_r))
+            assert relerr(out, orc.apply_local_hamiltonian(a, w, l, r)) < TOL["fp64"]
+        else:
+            # a new label set per call on zero operands: only the plan matters
+            a0 = np.zeros((Dl, 2, 7), complex)
+            l0 = np.zeros((Dl, len(qm_l), Dl), complex)
+            r0 = np.zeros((7, len(qm_r), 7), complex)
+            out, _ = backend.heff_apply_sectors(a0, w, l0, r0, (ql, qd, qr, qm_l, qm_r))
+            assert np.all(backend.to_host(out) == 0)
