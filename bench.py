@@ -10,7 +10,7 @@ numiter_lanczos=8, tol_split=1e-10, bond cap D. A step is one TDVP time step
 (one left-to-right plus right-to-left sweep) after the setup (right
 orthonormalisation and environment blocks), with everything resident in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--L 100] [--D 1024] [--mixed fp32]
+    python bench.py [--steps K] [--warmup W] [--L 100] [--D 1024] [--mixed fp32|bf16x6|bf16x3]
 
 Prints one JSON line. `value` is the mixed-precision sweep time (s/sweep,
 lower is better); the FP64 sweep time, the speed-up, |1 - fidelity| and the
@@ -36,7 +36,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("2-site TDVP sweep time at D=1024: mixed vs FP64 GPU at |1−fidelity|≤1e-8")
 HBM_PEAK_GBS = 8000.0
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "bf16x6": 2500.0}
 
 
 def heff_flops(Dl, d, Dr, Ml, Mr):
@@ -60,7 +60,7 @@ def prewarm():
     """Load every kernel once on a tiny chain (not a timed or warmup step)."""
     import pytenet_amd as ptn
     H, psi = build_state(6, 8, 0)
-    for prec in ("fp64", "fp32", "bf16x3"):
+    for prec in ("fp64", "fp32", "bf16x3", "bf16x6"):
         p = copy.deepcopy(psi)
         ptn.tdvp_twosite(H, p, -0.05j, 1, numiter_lanczos=4, tol_split=1e-10, precision=prec,
                          max_bond=8)
@@ -117,7 +117,7 @@ def _kernel_bound(name):
     multiply-add, the peak of their arithmetic), the Jacobi apply kernel
     streams the rotated columns (HBM); the rotation solve is latency-bound and
     has no roofline."""
-    if name.startswith("zgemm_bf16x3"):
+    if name.startswith("zgemm_bf16x3") or name.startswith("zgemm_bf16x6"):
         return "mfma", PEAK_TFLOPS["bf16x3"], "TFLOP/s"
     if (name.startswith("zgemm_kernel<float") or name.startswith("zgemm_grouped_kernel<float")
             or name == "k_jac_gram<float>"):
@@ -558,7 +558,7 @@ def workload_growth(args):
     out = {"workload": f"two-site TDVP, XXZ L={L}, bond 16 -> cap {D}, dt=-0.05j, {steps} steps, "
                        f"numiter {nit}, tol 1e-10", "unit": "s", "E0": e0}
     finals = {}
-    for prec in ("fp64", "fp32", "bf16x3"):
+    for prec in ("fp64", "fp32", "bf16x3", "bf16x6"):
         psi = copy.deepcopy(psi0)
         run = ptn.TwoSiteTDVP(H, psi, -0.05j, nit, 1e-10, precision=prec, max_bond=D)
         torch.cuda.synchronize()
@@ -582,7 +582,7 @@ def workload_growth(args):
     ref = finals["fp64"]
     nr = ptn.mps_norm(ref)
     er = ptn.mpo_average(ref, H).real / nr**2
-    for prec in ("fp32", "bf16x3"):
+    for prec in ("fp32", "bf16x3", "bf16x6"):
         p = finals[prec]
         npr = ptn.mps_norm(p)
         out[f"{prec}_infidelity_vs_fp64"] = abs(1 - abs(ptn.mps_vdot(ref, p))**2 / (nr**2 * npr**2))
@@ -656,7 +656,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         res = {"copy_calibration_bytes": 2 * src.numel() * 8}
-        for prec in ("fp64", "fp32", "bf16x3"):
+        for prec in ("fp64", "fp32", "bf16x3", "bf16x6"):
             ms, tf = heff_roofline(args.D, prec, reps=20)
             res[prec] = {"avg_ms": round(ms, 3), "achieved_tflops": round(tf, 2),
                          "peak": PEAK_TFLOPS[prec], "frac": round(tf / PEAK_TFLOPS[prec], 4)}

@@ -16,7 +16,11 @@
  *       PTN_FP64   complex128 operands, FP64 MFMA (the parity mode);
  *       PTN_FP32   complex64 operands, FP32 MFMA, FP32 accumulation;
  *       PTN_BF16X3 each FP32 operand split into two bf16 terms, three bf16
- *                  MFMA products (hi*hi + hi*lo + lo*hi), FP32 accumulation.
+ *                  MFMA products (hi*hi + hi*lo + lo*hi), FP32 accumulation;
+ *       PTN_BF16X6 each FP32 operand split into three bf16 terms (the whole
+ *                  FP32 significand), six bf16 MFMA products, FP32
+ *                  accumulation: FP32 accuracy on the bf16 MFMA (products too
+ *                  small to fill the chip run on the FP32 MFMA).
  *     Inputs and outputs stay complex128 in every mode; Krylov vectors,
  *     inner products, norms and the small tridiagonal problems are FP64.
  *   - Every call is asynchronous on `stream` (a hipStream_t), returns a status
@@ -47,6 +51,7 @@ extern "C" {
 #define PTN_FP64    0
 #define PTN_FP32    1
 #define PTN_BF16X3  2
+#define PTN_BF16X6  3
 /* flag OR-ed into a precision code: the operands are block sparse (U(1)
  * quantum numbers, the reference's dense arrays with zero blocks;
  * pytenet/block_sparse_util.py:56-64). The contractions then build per-tile
@@ -63,8 +68,9 @@ extern "C" {
 /* Krylov status words (int32[4] in device memory):
  *   [0] number of Lanczos vectors used (numiter, or j+1 after a breakdown at step j)
  *   [1] j+1 if beta[j] < 100*n*eps stopped the iteration at step j, else 0; with
- *       PTN_FP32 / PTN_BF16X3 matvecs also if beta[j] < rel * |A v_j| (rel = 128 eps_fp32
- *       / 2^-12): a residual below the matvec's rounding noise is treated as a breakdown
+ *       PTN_FP32 / PTN_BF16X6 / PTN_BF16X3 matvecs also if beta[j] < rel * |A v_j|
+ *       (rel = 128 eps_fp32, 128 eps_fp32, 2^-12): a residual below the matvec's
+ *       rounding noise is treated as a breakdown
  *   [2] bit 0: a norm or inner product was not finite;
  *       bit 1: the tridiagonal QL iteration did not converge (the result is unusable)
  *   [3] 1 if the start vector has norm zero (the reference asserts, krylov.py:30), else 0
@@ -132,7 +138,7 @@ int ptn_zgemm(int64_t M, int64_t N, int64_t K,
  * reference's matvec closure (pytenet/tdvp.py:223-238, pytenet/dmrg.py:181-189). */
 typedef struct {
   int kind;                 /* PTN_OP_HEFF or PTN_OP_BOND */
-  int precision;            /* PTN_FP64 / PTN_FP32 / PTN_BF16X3, optionally | PTN_BLOCK_SPARSE */
+  int precision;            /* PTN_FP64 / PTN_FP32 / PTN_BF16X3 / PTN_BF16X6, optionally | PTN_BLOCK_SPARSE */
   int64_t Dl, d, Dr, Ml, Mr;/* bond operator: d = 1, Ml = Mr = M */
   const void* l;            /* L (Dl, Ml, Dl) */
   const void* w;            /* W (Ml, d, d, Mr); NULL for the bond operator */

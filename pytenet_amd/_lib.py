@@ -20,6 +20,7 @@ PTN_ERR_NONFINITE = 5
 PTN_FP64 = 0
 PTN_FP32 = 1
 PTN_BF16X3 = 2
+PTN_BF16X6 = 3
 PTN_BLOCK_SPARSE = 0x100
 
 PTN_OP_HEFF = 0

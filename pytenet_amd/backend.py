@@ -16,7 +16,8 @@ import torch
 from . import _lib
 from ._lib import c_p
 
-PRECISIONS = {"fp64": _lib.PTN_FP64, "fp32": _lib.PTN_FP32, "bf16x3": _lib.PTN_BF16X3}
+PRECISIONS = {"fp64": _lib.PTN_FP64, "fp32": _lib.PTN_FP32, "bf16x3": _lib.PTN_BF16X3,
+              "bf16x6": _lib.PTN_BF16X6}
 
 
 class PtnError(RuntimeError):

@@ -87,7 +87,7 @@ using namespace ptn;
 
 static inline bool precision_ok(int p) {
   p &= ~PTN_BLOCK_SPARSE;
-  return p == PTN_FP64 || p == PTN_FP32 || p == PTN_BF16X3;
+  return p == PTN_FP64 || p == PTN_FP32 || p == PTN_BF16X3 || p == PTN_BF16X6;
 }
 
 extern "C" {

@@ -115,11 +115,14 @@ struct ZGemm {
 };
 // scratch bytes of the pre-split split-bf16 product of an M x N x K (batched) product
 size_t bf16x3_split_bytes(int64_t M, int64_t N, int64_t K, int64_t batch);
+// the same for the three-term split (PTN_BF16X6)
+size_t bf16x6_split_bytes(int64_t M, int64_t N, int64_t K, int64_t batch);
 
 // element type codes for the GEMM / mix operands
 enum { PTN_T_C128 = 0, PTN_T_C64 = 1 };
 
-int zgemm(const ZGemm& g, int compute /*PTN_FP64 | PTN_FP32*/, int ta, int tb, int tc,
+// compute: PTN_FP64, PTN_FP32, PTN_BF16X3 or PTN_BF16X6
+int zgemm(const ZGemm& g, int compute, int ta, int tb, int tc,
           void* ws, size_t ws_bytes, hipStream_t s);
 
 // Y[l, o, r] = sum_i Wm[o, i] X[l, i, r]   (complex; Wm complex128 on device);

@@ -597,6 +597,314 @@ size_t bf16x3_split_bytes(int64_t M, int64_t N, int64_t K, int64_t batch) {
   return v2_split_bytes(M, N, K, batch);
 }
 
+// ---------------------------------------------------------------------------
+// Three-term split ("bf16x6", precision PTN_BF16X6): every FP32-rounded
+// operand value is written as
+//   x = hi + mid + lo,  hi = bf16(x), mid = bf16(x - hi), lo = bf16(x - hi - mid)
+// (each difference is exact in FP32, and the three terms hold the whole 24-bit
+// significand), and a product is formed from the six leading terms
+//   a b ~ a_lo b_hi + a_hi b_lo + a_mid b_mid + a_mid b_hi + a_hi b_mid + a_hi b_hi
+// (smallest first; the dropped a_mid b_lo, a_lo b_mid, a_lo b_lo are below
+// 2^-25 |a b|), accumulated in FP32: the accuracy of an FP32 product on the
+// bf16 MFMA, with twice the MFMA work of bf16x3 and 1.5 times its operand
+// bytes. The three terms of a value are stored interleaved per 16 real k
+// ([row][k / 16][term][k % 16]), so that one k-stage of a row is one
+// contiguous 96-byte read. Products too small for the pre-split kernel run on
+// the FP32 MFMA (the same accuracy).
+constexpr int V3_BK = 16, V3_LD = 24, V3_NT = 3;
+
+inline int64_t v3_kpad(int64_t K) { return v2_round(2 * K, V3_BK); }
+
+inline size_t v3_split_bytes(int64_t M, int64_t N, int64_t K, int64_t batch) {
+  const int64_t Mp = v2_round(M, V2_BM), N2p = v2_round(2 * N, V2_BN), K2p = v3_kpad(K);
+  const int sk = v2_splitk(M, N, K, batch);
+  const size_t part = sk > 1 ? (size_t)sk * (size_t)Mp * (size_t)N2p * sizeof(float) + 256 : 0;
+  return (size_t)batch * (size_t)(Mp + N2p) * (size_t)K2p * V3_NT * sizeof(__bf16) + 512 + part;
+}
+
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void split3(float x, __bf16& hi, __bf16& mid, __bf16& lo) {
+  hi = (__bf16)x;
+  const float r = x - (float)hi;
+  mid = (__bf16)r;
+  lo = (__bf16)(r - (float)mid);
+}
+
+// element offset of real k (even) of a row in the interleaved term planes
+__device__ __forceinline__ int64_t v3_off(int64_t row, int64_t k2, int64_t K2p) {
+  return row * (V3_NT * K2p) + (k2 / V3_BK) * (V3_NT * V3_BK) + (k2 % V3_BK);
+}
+
+// A (M x K complex, as k_split_a) -> interleaved hi / mid / lo terms of A_r
+template <typename TA>
+__global__ __launch_bounds__(256) void k_split3_a(int64_t M, int64_t K, const TA* __restrict__ A,
+                                                  int64_t a_m, int64_t a_k, int conj, int64_t bz,
+                                                  int64_t K2p, int64_t pz, __bf16* __restrict__ P) {
+  __shared__ float2 t[64][33];
+  const int64_t m0 = (int64_t)blockIdx.y * 64, k0 = (int64_t)blockIdx.x * 32;
+  A += blockIdx.z * bz;
+  P += blockIdx.z * pz;
+  const bool kc = (a_k == 1) || (a_m != 1);
+  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+    const int r = kc ? e / 32 : e % 64, c = kc ? e % 32 : e / 64;
+    const int64_t m = m0 + r, k = k0 + c;
+    float re = 0.f, im = 0.f;
+    if (m < M && k < K) load_c<float>(A + m * a_m + k * a_k, re, im);
+    t[r][c] = make_float2(re, conj ? -im : im);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+    const int r = e / 32, c = e % 32;
+    const int64_t k2 = 2 * (k0 + c);
+    if (k2 >= K2p) continue;
+    const float2 v = t[r][c];
+    __bf16 h0, d0, l0, h1, d1, l1;
+    split3(v.x, h0, d0, l0);
+    split3(v.y, h1, d1, l1);
+    __bf16* o = P + v3_off(m0 + r, k2, K2p);
+    *(bf16x2_t*)o = bf16x2_t{h0, h1};
+    *(bf16x2_t*)(o + V3_BK) = bf16x2_t{d0, d1};
+    *(bf16x2_t*)(o + 2 * V3_BK) = bf16x2_t{l0, l1};
+  }
+}
+
+// B (K x N complex, as k_split_b) -> interleaved terms of the real embedding
+// B_e: row 2n holds (Re, -Im), row 2n + 1 holds (Im, Re) per complex k
+template <typename TB>
+__global__ __launch_bounds__(256) void k_split3_b(int64_t K, int64_t N, const TB* __restrict__ B,
+                                                  int64_t b_k, int64_t b_n, int conj, int64_t bz,
+                                                  int64_t K2p, int64_t pz, __bf16* __restrict__ P) {
+  __shared__ float2 t[64][33];
+  const int64_t n0 = (int64_t)blockIdx.y * 64, k0 = (int64_t)blockIdx.x * 32;
+  B += blockIdx.z * bz;
+  P += blockIdx.z * pz;
+  const bool kc = (b_k == 1) || (b_n != 1);
+  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+    const int r = kc ? e / 32 : e % 64, c = kc ? e % 32 : e / 64;
+    const int64_t n = n0 + r, k = k0 + c;
+    float re = 0.f, im = 0.f;
+    if (n < N && k < K) load_c<float>(B + k * b_k + n * b_n, re, im);
+    t[r][c] = make_float2(re, conj ? -im : im);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+    const int r = e / 32, c = e % 32;
+    const int64_t k2 = 2 * (k0 + c);
+    if (k2 >= K2p) continue;
+    const float2 v = t[r][c];
+    __bf16 hr, dr, lr, hi_, di, li;
+    split3(v.x, hr, dr, lr);
+    split3(v.y, hi_, di, li);   // the terms of -Im are the negated terms of Im
+    __bf16* o0 = P + v3_off(2 * (n0 + r), k2, K2p);
+    __bf16* o1 = o0 + V3_NT * K2p;
+    *(bf16x2_t*)o0 = bf16x2_t{hr, -hi_};
+    *(bf16x2_t*)(o0 + V3_BK) = bf16x2_t{dr, -di};
+    *(bf16x2_t*)(o0 + 2 * V3_BK) = bf16x2_t{lr, -li};
+    *(bf16x2_t*)o1 = bf16x2_t{hi_, hr};
+    *(bf16x2_t*)(o1 + V3_BK) = bf16x2_t{di, dr};
+    *(bf16x2_t*)(o1 + 2 * V3_BK) = bf16x2_t{li, lr};
+  }
+}
+
+// 256 x 256 real tiles, 8 waves (2 x 4) of 128 x 64 as in zgemm_bf16x3_v2,
+// 16-deep k-stages double buffered in LDS (6 term planes x 256 rows x 24 bf16
+// per stage = 72 KiB; the 48-byte row pitch keeps the 16-byte fragment reads
+// of 16 consecutive rows on distinct banks). The operand pointers of V2Args
+// are used as: ahi = A terms, bhi = B terms (alo, blo unused).
+template <typename TC>
+__global__ __launch_bounds__(512) void zgemm_bf16x6_v3(V2Args g) {
+  __shared__ __align__(16) __bf16 sm[2][2 * V3_NT][V2_BM][V3_LD];
+  if (g.skip && g.skip[(g.part ? 0 : blockIdx.z) * g.skip_stride]) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 2, wn = wave & 3;
+  int tm, tn;
+  tile_coords(blockIdx.x, gridDim.x, g.tiles_m, g.tiles_n, tm, tn);
+  const int64_t m0 = (int64_t)tm * V2_BM, c0 = (int64_t)tn * V2_BN;   // c0: real column
+  const int64_t z = g.part ? 0 : blockIdx.z;
+  const int64_t kbeg = g.part ? (int64_t)blockIdx.z * g.kchunk : 0;
+  const int64_t pitch = V3_NT * g.K2p;   // bf16 per row
+  const __bf16* srcA = g.ahi + z * g.pza + m0 * pitch + V3_NT * kbeg;
+  const __bf16* srcB = g.bhi + z * g.pzb + c0 * pitch + V3_NT * kbeg;
+  // global -> register staging: per operand and stage 256 rows x 96 bytes =
+  // 1536 chunks of 16 bytes, three per thread (six consecutive threads per row)
+  u32x4_t pa[3], pb[3];
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const int c = tid + 512 * u, row = c / 6, part = c % 6;
+      const int64_t o = (int64_t)row * pitch + V3_NT * k0 + 8 * part;
+      pa[u] = *(const u32x4_t*)(srcA + o);
+      pb[u] = *(const u32x4_t*)(srcB + o);
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const int c = tid + 512 * u, row = c / 6, part = c % 6;
+      const int term = part >> 1, half = 8 * (part & 1);
+      *(u32x4_t*)&sm[buf][term][row][half] = pa[u];
+      *(u32x4_t*)&sm[buf][V3_NT + term][row][half] = pb[u];
+    }
+  };
+  f16v_t acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+  const int64_t klen = g.part ? (g.K2p - kbeg < g.kchunk ? g.K2p - kbeg : g.kchunk) : g.K2p;
+  const int nst = (int)(klen / V3_BK);
+  const int r32 = lane & 31, kg = 8 * (lane >> 5);
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const int buf = st & 1;
+    const bool more = st + 1 < nst;
+    if (more) gload((int64_t)(st + 1) * V3_BK);
+    bf16x8_t ah[4], am[4], al[4], bh[2], bm[2], bl[2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = 128 * wm + 32 * i + r32;
+      ah[i] = *(const bf16x8_t*)&sm[buf][0][row][kg];
+      am[i] = *(const bf16x8_t*)&sm[buf][1][row][kg];
+      al[i] = *(const bf16x8_t*)&sm[buf][2][row][kg];
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = 64 * wn + 32 * j + r32;
+      bh[j] = *(const bf16x8_t*)&sm[buf][V3_NT][col][kg];
+      bm[j] = *(const bf16x8_t*)&sm[buf][V3_NT + 1][col][kg];
+      bl[j] = *(const bf16x8_t*)&sm[buf][V3_NT + 2][col][kg];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am[i], bm[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bm[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      }
+    if (more) lstore(buf ^ 1);
+    __syncthreads();
+  }
+  // epilogue of zgemm_bf16x3_v2 (the same accumulator layout)
+  if (g.part) {
+    float* P = g.part + (int64_t)blockIdx.z * (int64_t)g.tiles_m * V2_BM * g.N2p;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int64_t c = c0 + 64 * wn + 32 * j + r32;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int64_t m = m0 + 128 * wm + 32 * i + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+          P[m * g.N2p + c] = acc[i][j][v];
+        }
+      }
+    return;
+  }
+  TC* C = (TC*)g.C + z * g.bz_c;
+  const int pb_ = lane & 1;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int64_t n = (c0 + 64 * wn + 32 * j + r32) >> 1;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const float mine = acc[i][j][v];
+        const float other = __shfl_xor(mine, 1);
+        const int64_t m = m0 + 128 * wm + 32 * i + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        if (!pb_ && m < g.M && n < g.N) {
+          double re = mine, im = other;
+          TC* p = C + m * g.c_m + n * g.c_n;
+          if (g.accumulate) {
+            double pr, pi;
+            load_c<double>(p, pr, pi);
+            re += pr; im += pi;
+          }
+          store_c(p, re, im);
+        }
+      }
+    }
+}
+
+template <typename TA, typename TB, typename TC>
+inline int launch_bf16_v3(const ZGemm& z, hipStream_t s) {
+  const int64_t Mp = v2_round(z.M, V2_BM), N2p = v2_round(2 * z.N, V2_BN), K2p = v3_kpad(z.K);
+  const int64_t pza = Mp * K2p * V3_NT, pzb = N2p * K2p * V3_NT;
+  char* base = (char*)(((uintptr_t)z.sws + 255) & ~(uintptr_t)255);
+  __bf16* pa = (__bf16*)base;
+  __bf16* pb = pa + z.batch * pza;
+  const unsigned bt = (unsigned)z.batch;
+  const unsigned kblk = (unsigned)((K2p + 63) / 64);   // 32 complex k per block
+  k_split3_a<TA><<<dim3(kblk, (unsigned)(Mp / 64), bt), 256, 0, s>>>(
+      z.M, z.K, (const TA*)z.A, z.a_m, z.a_k, z.conjA, z.bz_a, K2p, pza, pa);
+  k_split3_b<TB><<<dim3(kblk, (unsigned)(N2p / 128), bt), 256, 0, s>>>(
+      z.K, z.N, (const TB*)z.B, z.b_k, z.b_n, z.conjB, z.bz_b, K2p, pzb, pb);
+  PTN_LAUNCH_CHECK();
+  const int sk = v2_splitk(z.M, z.N, z.K, z.batch);
+  V2Args g;
+  g.ahi = pa; g.alo = nullptr; g.bhi = pb; g.blo = nullptr;
+  g.K2p = K2p; g.pza = pza; g.pzb = pzb;
+  g.N2p = N2p;
+  g.kchunk = v2_round((K2p + sk - 1) / sk, V3_BK);
+  g.part = nullptr;
+  if (sk > 1) g.part = (float*)(((uintptr_t)(pb + z.batch * pzb) + 255) & ~(uintptr_t)255);
+  g.M = z.M; g.N = z.N;
+  g.tiles_m = (int)(Mp / V2_BM);
+  g.tiles_n = (int)(N2p / V2_BN);
+  g.C = z.C; g.c_m = z.c_m; g.c_n = z.c_n; g.bz_c = z.bz_c;
+  g.accumulate = z.accumulate;
+  g.skip = skip_flag().p;
+  g.skip_stride = skip_flag().stride;
+  const unsigned nz = sk > 1 ? (unsigned)((K2p + g.kchunk - 1) / g.kchunk) : bt;
+  dim3 grid((unsigned)(g.tiles_m * g.tiles_n), 1, nz);
+  static const std::string v3name = std::string("zgemm_bf16x6_v3<") + type_tag<TC>::s + ">";
+  tlaunch(v3name.c_str(), gemm_flop(z), gemm_bytes<TA, TB, TC>(z), zgemm_bf16x6_v3<TC>, grid,
+          dim3(512), s, g);
+  if (sk > 1)
+    k_v2_reduce<TC><<<(unsigned)std::min<int64_t>((z.M * z.N + 255) / 256, 4096), 256, 0, s>>>(
+        z.M, z.N, Mp, N2p, (int)nz, g.part, (TC*)z.C, z.c_m, z.c_n, z.accumulate, g.skip);
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
+inline bool use_bf16_v3(const ZGemm& z, void* mws) {
+  if (mws || !z.sws) return false;
+  const int64_t tiles = ((z.M + V2_BM - 1) / V2_BM) * ((2 * z.N + V2_BN - 1) / V2_BN) * z.batch;
+  if (tiles * v2_splitk(z.M, z.N, z.K, z.batch) < 128 || z.K < 16 || z.batch > 65535) return false;
+  return z.sws_bytes >= v3_split_bytes(z.M, z.N, z.K, z.batch);
+}
+
+size_t bf16x6_split_bytes(int64_t M, int64_t N, int64_t K, int64_t batch) {
+  return v3_split_bytes(M, N, K, batch);
+}
+
+int zgemm_fp32(const ZGemm& z, int ta, int tb, int tc, hipStream_t s, void* mws, size_t mws_bytes);
+
+int zgemm_bf16x6(const ZGemm& z, int ta, int tb, int tc, hipStream_t s, void* mws,
+                 size_t mws_bytes) {
+  if (!use_bf16_v3(z, mws)) return zgemm_fp32(z, ta, tb, tc, s, mws, mws_bytes);
+  const int key = (ta << 2) | (tb << 1) | tc;
+  switch (key) {
+    case 1: return launch_bf16_v3<double2, double2, float2>(z, s);
+    case 2: return launch_bf16_v3<double2, float2, double2>(z, s);
+    case 3: return launch_bf16_v3<double2, float2, float2>(z, s);
+    case 4: return launch_bf16_v3<float2, double2, double2>(z, s);
+    case 5: return launch_bf16_v3<float2, double2, float2>(z, s);
+    case 6: return launch_bf16_v3<float2, float2, double2>(z, s);
+    case 7: return launch_bf16_v3<float2, float2, float2>(z, s);
+  }
+  return PTN_ERR_UNSUPPORTED;
+}
+
 int zgemm_bf16x3(const ZGemm& z, int ta, int tb, int tc, hipStream_t s, void* mws,
                  size_t mws_bytes) {
   const int key = (ta << 2) | (tb << 1) | tc;

@@ -7,6 +7,8 @@ namespace ptn {
 int zgemm_fp32(const ZGemm& z, int ta, int tb, int tc, hipStream_t s, void* mws, size_t mws_bytes);
 int zgemm_bf16x3(const ZGemm& z, int ta, int tb, int tc, hipStream_t s, void* mws,
                  size_t mws_bytes);
+int zgemm_bf16x6(const ZGemm& z, int ta, int tb, int tc, hipStream_t s, void* mws,
+                 size_t mws_bytes);
 
 int zgemm(const ZGemm& z, int compute, int ta, int tb, int tc, void* ws, size_t ws_bytes,
           hipStream_t s) {
@@ -19,6 +21,7 @@ int zgemm(const ZGemm& z, int compute, int ta, int tb, int tc, void* ws, size_t 
   }
   if (compute == PTN_FP32) return zgemm_fp32(z, ta, tb, tc, s, ws, ws_bytes);
   if (compute == PTN_BF16X3) return zgemm_bf16x3(z, ta, tb, tc, s, ws, ws_bytes);
+  if (compute == PTN_BF16X6) return zgemm_bf16x6(z, ta, tb, tc, s, ws, ws_bytes);
   return PTN_ERR_UNSUPPORTED;
 }
 

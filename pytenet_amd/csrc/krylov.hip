@@ -613,7 +613,8 @@ static int krylov_enqueue(const ptn_local_op* op, int mode, const void* v, doubl
   if (!op || numiter < 1 || numiter > LZ_MAX || batch < 1 || batch > 65535) return PTN_ERR_ARG;
   if (op->kind != PTN_OP_HEFF && op->kind != PTN_OP_BOND) return PTN_ERR_ARG;
   const int bprec = op->precision & ~PTN_BLOCK_SPARSE;
-  if (bprec != PTN_FP64 && bprec != PTN_FP32 && bprec != PTN_BF16X3) return PTN_ERR_UNSUPPORTED;
+  if (bprec != PTN_FP64 && bprec != PTN_FP32 && bprec != PTN_BF16X3 && bprec != PTN_BF16X6)
+    return PTN_ERR_UNSUPPORTED;
   if (op->Dl <= 0 || op->Dr <= 0 || op->Ml <= 0 || op->Mr <= 0) return PTN_ERR_ARG;
   if (op->kind == PTN_OP_HEFF && (op->d <= 0 || !op->w)) return PTN_ERR_ARG;
   if (op->kind == PTN_OP_BOND && op->Ml != op->Mr) return PTN_ERR_ARG;
@@ -631,8 +632,11 @@ static int krylov_enqueue(const ptn_local_op* op, int mode, const void* v, doubl
   const int rb = red_width(n);
   const int64_t js = batch * n;   // stride between V_j and V_{j+1}
   const double thresh = 100.0 * (double)n * DBL_EPSILON;
-  // relative breakdown level of reduced-precision matvecs (0: FP64, the reference's rule)
-  const double rel = bprec == PTN_FP32 ? 128.0 * FLT_EPSILON : (bprec == PTN_BF16X3 ? 1.0 / 4096 : 0.0);
+  // relative breakdown level of reduced-precision matvecs (0: FP64, the reference's rule;
+  // the three-term split has the accuracy of an FP32 product)
+  const double rel = (bprec == PTN_FP32 || bprec == PTN_BF16X6)
+                         ? 128.0 * FLT_EPSILON
+                         : (bprec == PTN_BF16X3 ? 1.0 / 4096 : 0.0);
 
   // environment blocks in the matvec's operand precision, converted once per solve
   const void* L = op->l;

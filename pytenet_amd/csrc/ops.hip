@@ -59,8 +59,16 @@ int convert_c128_c64(int64_t n, const void* x, void* y, hipStream_t s) {
 // of its two products; 0 in the other modes)
 static size_t split2(int prec, int64_t batch, int64_t m1, int64_t n1, int64_t k1, int64_t m3,
                      int64_t n3, int64_t k3) {
-  if (base_prec(prec) != PTN_BF16X3) return 0;
-  const size_t a = bf16x3_split_bytes(m1, n1, k1, batch), b = bf16x3_split_bytes(m3, n3, k3, batch);
+  size_t a, b;
+  if (base_prec(prec) == PTN_BF16X3) {
+    a = bf16x3_split_bytes(m1, n1, k1, batch);
+    b = bf16x3_split_bytes(m3, n3, k3, batch);
+  } else if (base_prec(prec) == PTN_BF16X6) {
+    a = bf16x6_split_bytes(m1, n1, k1, batch);
+    b = bf16x6_split_bytes(m3, n3, k3, batch);
+  } else {
+    return 0;
+  }
   return align256(a > b ? a : b);
 }
 

@@ -4,8 +4,9 @@ environment blocks and all local Krylov solves resident on the GPU.
 
 Same signatures as the reference plus keyword extensions with
 reference-preserving defaults:
-  precision: "fp64" (default; the parity mode), "fp32" or "bf16x3" for the
-             dense contractions (Krylov scalars, QR and SVD stay FP64);
+  precision: "fp64" (default; the parity mode), "fp32", "bf16x6" (FP32
+             accuracy on the bf16 MFMA) or "bf16x3" for the dense
+             contractions (Krylov scalars, QR and SVD stay FP64);
   max_bond:  bond-dimension cap for the two-site split (None = no cap, as
              in the reference);
   svd_precond32: two-site split SVD with FP32 Jacobi preconditioning sweeps

@@ -91,7 +91,7 @@ def test_dmrg_twosite_reference_trajectory_quantum_numbers():
     np.testing.assert_allclose(_phase_aligned(psi.to_vector(), ref), ref, atol=1e-8)
 
 
-@pytest.mark.parametrize("precision,rtol", [("fp32", 1e-6), ("bf16x3", 1e-4)])
+@pytest.mark.parametrize("precision,rtol", [("fp32", 1e-6), ("bf16x3", 1e-4), ("bf16x6", 1e-6)])
 def test_dmrg_twosite_quantum_numbers_low_precision(precision, rtol):
     """U(1) sectors with FP32 / split-bf16 tile-skipping products: the energies
     of the reference trajectory to the precision of the contractions (split
@@ -258,3 +258,27 @@ def test_bf16x3_contractions_keep_fidelity(L, D, steps):
     assert abs(1 - f) < 1e-6
     e64, e16 = ptn.mpo_average(p64, H).real, ptn.mpo_average(p16, H).real
     assert abs(e16 - e64) < 1e-4 * abs(e64)
+
+
+def test_bf16x6_contractions_keep_fp32_fidelity():
+    """Three-term split-bf16 contractions (six bf16 MFMA products per operand
+    pair, FP32 accuracy) in the two-site TDVP at D = 512, where the two-site
+    products take the pre-split three-term path: fidelity and energy against
+    the all-FP64 run within the FP32 mode's bounds."""
+    rng = np.random.default_rng(43)
+    L, D = 20, 512
+    H = ptn.heisenberg_xxz_1d_mpo(L, 1.0, 0.8, -0.1).zero_qnumbers()
+    bonds = [min(D, 2**min(i, L - i)) for i in range(L + 1)]
+    psi = ptn.MPS(H.qsite, [np.zeros(b, dtype=int) for b in bonds], fill="random", rng=rng)
+    psi.orthonormalize("left")
+    p64, p6 = copy.deepcopy(psi), copy.deepcopy(psi)
+    ptn.tdvp_twosite(H, p64, -0.05j, 1, numiter_lanczos=8, tol_split=1e-10, max_bond=D)
+    backend.timing_begin()
+    ptn.tdvp_twosite(H, p6, -0.05j, 1, numiter_lanczos=8, tol_split=1e-10, max_bond=D,
+                     precision="bf16x6")
+    names = [t["name"] for t in backend.timing_end()]
+    assert any(n.startswith("zgemm_bf16x6_v3<") for n in names), names
+    f = abs(ptn.mps_vdot(p64, p6))**2 / (ptn.mps_norm(p64)**2 * ptn.mps_norm(p6)**2)
+    assert abs(1 - f) < 1e-8
+    e64, e6 = ptn.mpo_average(p64, H).real, ptn.mpo_average(p6, H).real
+    assert abs(e6 - e64) < 1e-6 * abs(e64)

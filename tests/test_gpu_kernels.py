@@ -92,7 +92,7 @@ SHAPES = [(1, 2, 1, 1, 4), (3, 2, 5, 4, 5), (17, 2, 23, 5, 5), (40, 4, 36, 5, 5)
 
 @pytest.mark.parametrize("Dl,d,Dr,Ml,Mr", SHAPES)
 @pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL),
-                                           ("bf16x3", BF16X3_TOL)])
+                                           ("bf16x3", BF16X3_TOL), ("bf16x6", FP32_TOL)])
 def test_local_kernels_random(Dl, d, Dr, Ml, Mr, precision, tol):
     rng = np.random.default_rng(Dl * 7 + Dr)
     a = orc.crandn((Dl, d, Dr), rng)
@@ -441,7 +441,7 @@ def _qsparse(rng, shape, qnums):
 
 @pytest.mark.parametrize("Dl,Dr,two_site", [(200, 180, False), (260, 300, True), (37, 41, True)])
 @pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL),
-                                           ("bf16x3", BF16X3_TOL)])
+                                           ("bf16x3", BF16X3_TOL), ("bf16x6", FP32_TOL)])
 def test_block_sparse_contractions_match_dense(Dl, Dr, two_site, precision, tol):
     """U(1) block-sparse operands (XXZ MPO with quantum numbers, sector-sorted
     bonds): the tile-skipping contractions equal the dense ones and the oracle."""
@@ -642,7 +642,7 @@ def test_graph_captured_solves_equal_direct_launches():
         backend.set_graphs(prev)
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x6"])
 def test_low_precision_lanczos_breaks_down_on_invariant_subspace(precision):
     """A start vector in a 2-dimensional invariant subspace of a block-diagonal
     H_eff: in FP64 the residual after two steps is at rounding level and the
@@ -664,7 +664,7 @@ def test_low_precision_lanczos_breaks_down_on_invariant_subspace(precision):
     i64, ilp = backend.to_host(i64), backend.to_host(ilp)
     assert i64[0] == 2 and i64[1] == 2
     assert ilp[0] == 2 and ilp[1] == 2
-    tol = 1e-6 if precision == "fp32" else 1e-3
+    tol = 1e-3 if precision == "bf16x3" else 1e-6
     assert relerr(olp, backend.to_host(o64)) < tol
 
 
@@ -691,6 +691,38 @@ def test_presplit_bf16x3_products_match_oracle(Dl, Dr):
     c = orc.crandn((Dl, Dr), rng)
     assert relerr(backend.bond_apply(c, l, r, "bf16x3"),
                   orc.apply_local_bond_contraction(c, l, r)) < BF16X3_TOL
+
+
+@pytest.mark.parametrize("Dl,Dr", [(515, 509), (512, 512)])
+def test_presplit_bf16x6_products_have_fp32_accuracy(Dl, Dr):
+    """The three-term split-bf16 products (hi / mid / lo terms interleaved per
+    16 real k, six MFMA products, 256 x 256 tiles, k-chunked where the tiles
+    do not fill the chip) at ragged and aligned shapes: H_eff, both environment
+    steps and the bond operator against the oracle at the FP32 bound. The
+    error is within twice the FP32 product's on the same operands (a dropped or
+    misplaced third term would leave the two-term split's error, several times
+    larger, which the comparison with bf16x3 rules out); the launched kernels
+    include the three-term product."""
+    a, w, l, r = _heff_problem(Dl + Dr, Dl=Dl, d=4, Dr=Dr, M=5)
+    ref = orc.apply_local_hamiltonian(a, w, l, r)
+    backend.timing_begin()
+    got = backend.heff_apply(a, w, l, r, "bf16x6")
+    names = [t["name"] for t in backend.timing_end()]
+    assert any(n.startswith("zgemm_bf16x6_v3<") for n in names), names
+    e6 = relerr(got, ref)
+    e32 = relerr(backend.heff_apply(a, w, l, r, "fp32"), ref)
+    e3 = relerr(backend.heff_apply(a, w, l, r, "bf16x3"), ref)
+    assert e6 < FP32_TOL and e6 < 2 * e32 and e6 < e3 / 3, (e6, e32, e3)
+    a2 = a[:, :2, :] + 0.0
+    w2 = w[:, :2, :2, :] + 0.0
+    assert relerr(backend.env_left(a2, a2, w2, l, "bf16x6"),
+                  orc.contraction_operator_step_left(a2, a2, w2, l)) < FP32_TOL
+    assert relerr(backend.env_right(a2, a2, w2, r, "bf16x6"),
+                  orc.contraction_operator_step_right(a2, a2, w2, r)) < FP32_TOL
+    rng = np.random.default_rng(4)
+    c = orc.crandn((Dl, Dr), rng)
+    assert relerr(backend.bond_apply(c, l, r, "bf16x6"),
+                  orc.apply_local_bond_contraction(c, l, r)) < FP32_TOL
 
 
 def test_presplit_bf16x3_inside_lanczos_matches_fp64():

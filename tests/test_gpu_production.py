@@ -20,11 +20,12 @@ from pytenet_amd.hamiltonian import heisenberg_xxz_1d_mpo
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp64": 1e-12, "fp32": 2e-5, "bf16x3": 2e-4}
+TOL = {"fp64": 1e-12, "fp32": 2e-5, "bf16x3": 2e-4, "bf16x6": 2e-5}
 # product-kernel name prefixes of the large tiles the launch rule selects at D = 1024
 BIG_TILE = {"fp64": lambda n: n.startswith("zgemm_kernel<double,") and ", 4, 4, " in n,
             "fp32": lambda n: n.startswith("zgemm_kernel<float,") and ", 4, 8, " in n,
-            "bf16x3": lambda n: n.startswith("zgemm_bf16x3_v2<")}
+            "bf16x3": lambda n: n.startswith("zgemm_bf16x3_v2<"),
+            "bf16x6": lambda n: n.startswith("zgemm_bf16x6_v3<")}
 
 
 def relerr(got, ref):
@@ -55,7 +56,7 @@ def problem_d1024():
     return dev, ref
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3", "bf16x6"])
 def test_contractions_at_d1024_match_oracle_in_production_tiles(problem_d1024, precision):
     dv, ref = problem_d1024
     prev = backend.set_autotune(False)   # the fixed launch rule of the sweeps
