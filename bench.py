@@ -13,7 +13,8 @@ orthonormalisation and environment blocks), with everything resident in HBM.
     python bench.py [--steps K] [--warmup W] [--L 100] [--D 1024] [--mixed fp32|bf16x6|bf16x3]
 
 Prints one JSON line. `value` is the mixed-precision sweep time (s/sweep,
-lower is better); the FP64 sweep time, the speed-up, |1 - fidelity| and the
+lower is better; by default the contractions run as three-term split-bf16
+products with FP32 accuracy); the FP64 sweep time, the speed-up, |1 - fidelity| and the
 relative energy error are extra fields. `roofline` is measured live, over the
 timed sweeps, for the kernel with the largest share of the timed kernel time
 among those with a roofline (HIP events attached to every launch on the
@@ -37,6 +38,9 @@ sys.path.insert(0, ROOT)
 METRIC = ("2-site TDVP sweep time at D=1024: mixed vs FP64 GPU at |1−fidelity|≤1e-8")
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "bf16x6": 2500.0}
+MIXED_DTYPE = {"bf16x6": "bf16x6 (bf16 MFMA on three-term splits of f32 operands, f32 accumulation)",
+               "bf16x3": "bf16x3 (bf16 MFMA on two-term splits of f32 operands, f32 accumulation)",
+               "fp32": "f32"}
 
 
 def heff_flops(Dl, d, Dr, Ml, Mr):
@@ -614,7 +618,9 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="switch on the tile autotuner of the complex products")
     ap.add_argument("--steps-chains", type=int, default=20)
-    ap.add_argument("--mixed", default="fp32")
+    ap.add_argument("--mixed", default="bf16x6",
+                    help="contraction precision of the mixed leg: bf16x6 (three bf16 terms per "
+                         "FP32 operand, FP32 accuracy), fp32 or bf16x3")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--strict-svd", action="store_true",
@@ -704,8 +710,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": f"{args.mixed} contractions, f64 Krylov/QR, SVD: f32 Jacobi preconditioning "
-                 "sweeps finished by f64 sweeps (f64 accuracy)",
+        "dtype": f"{MIXED_DTYPE.get(args.mixed, args.mixed)} contractions, f64 Krylov/QR, "
+                 "SVD: f32 Jacobi preconditioning sweeps finished by f64 sweeps (f64 accuracy)",
         "data": "synthetic (seeded random MPS, XXZ MPO)",
         "config": {"workload": "two-site TDVP sweep, XXZ L=%d J=1 Delta=0.8 h=-0.1, fixed D=%d "
                                "(tapered), dt=-0.05j, numiter=%d, tol_split=1e-10"
