@@ -622,8 +622,6 @@ inline size_t v3_split_bytes(int64_t M, int64_t N, int64_t K, int64_t batch) {
   return (size_t)batch * (size_t)(Mp + N2p) * (size_t)K2p * V3_NT * sizeof(__bf16) + 512 + part;
 }
 
-typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
-
 __device__ __forceinline__ void split3(float x, __bf16& hi, __bf16& mid, __bf16& lo) {
   hi = (__bf16)x;
   const float r = x - (float)hi;
@@ -631,9 +629,28 @@ __device__ __forceinline__ void split3(float x, __bf16& hi, __bf16& mid, __bf16&
   lo = (__bf16)(r - (float)mid);
 }
 
-// element offset of real k (even) of a row in the interleaved term planes
-__device__ __forceinline__ int64_t v3_off(int64_t row, int64_t k2, int64_t K2p) {
-  return row * (V3_NT * K2p) + (k2 / V3_BK) * (V3_NT * V3_BK) + (k2 % V3_BK);
+// term `term` (0 hi, 1 mid, 2 lo) of eight values
+__device__ __forceinline__ bf16x8_t term8(const float (&x)[8], int term) {
+  bf16x8_t o;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    __bf16 h, d, l;
+    split3(x[e], h, d, l);
+    o[e] = term == 0 ? h : (term == 1 ? d : l);
+  }
+  return o;
+}
+
+// The split kernels stage a 64 x 32 complex block in LDS and write it as
+// 16-byte chunks (eight real k of one term): a row's four k-blocks of 16 real
+// k are 4 x 3 x 2 = 24 consecutive chunks (384 contiguous bytes), written by
+// consecutive threads, so every store is a whole chunk of a contiguous run.
+constexpr int V3_CHUNKS = 4 * V3_NT * 2;
+
+// element offset of chunk (term, half) of k-block kb of a row
+__device__ __forceinline__ int64_t v3_chunk(int64_t row, int64_t kb, int term, int half,
+                                            int64_t K2p) {
+  return row * (V3_NT * K2p) + kb * (V3_NT * V3_BK) + term * V3_BK + 8 * half;
 }
 
 // A (M x K complex, as k_split_a) -> interleaved hi / mid / lo terms of A_r
@@ -654,18 +671,20 @@ __global__ __launch_bounds__(256) void k_split3_a(int64_t M, int64_t K, const TA
     t[r][c] = make_float2(re, conj ? -im : im);
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
-    const int r = e / 32, c = e % 32;
-    const int64_t k2 = 2 * (k0 + c);
-    if (k2 >= K2p) continue;
-    const float2 v = t[r][c];
-    __bf16 h0, d0, l0, h1, d1, l1;
-    split3(v.x, h0, d0, l0);
-    split3(v.y, h1, d1, l1);
-    __bf16* o = P + v3_off(m0 + r, k2, K2p);
-    *(bf16x2_t*)o = bf16x2_t{h0, h1};
-    *(bf16x2_t*)(o + V3_BK) = bf16x2_t{d0, d1};
-    *(bf16x2_t*)(o + 2 * V3_BK) = bf16x2_t{l0, l1};
+  // A_r row m: real k 2c + 0 / 1 = Re / Im of complex k c
+  for (int q = threadIdx.x; q < 64 * V3_CHUNKS; q += 256) {
+    const int r = q / V3_CHUNKS, w = q % V3_CHUNKS;
+    const int kb = w / (2 * V3_NT), term = (w % (2 * V3_NT)) >> 1, half = w & 1;
+    const int64_t kbg = 2 * k0 / V3_BK + kb;   // k-block in the row
+    if (kbg * V3_BK >= K2p) continue;
+    float x[8];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float2 v = t[r][8 * kb + 4 * half + j];
+      x[2 * j] = v.x;
+      x[2 * j + 1] = v.y;
+    }
+    *(bf16x8_t*)(P + v3_chunk(m0 + r, kbg, term, half, K2p)) = term8(x, term);
   }
 }
 
@@ -688,22 +707,22 @@ __global__ __launch_bounds__(256) void k_split3_b(int64_t K, int64_t N, const TB
     t[r][c] = make_float2(re, conj ? -im : im);
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
-    const int r = e / 32, c = e % 32;
-    const int64_t k2 = 2 * (k0 + c);
-    if (k2 >= K2p) continue;
-    const float2 v = t[r][c];
-    __bf16 hr, dr, lr, hi_, di, li;
-    split3(v.x, hr, dr, lr);
-    split3(v.y, hi_, di, li);   // the terms of -Im are the negated terms of Im
-    __bf16* o0 = P + v3_off(2 * (n0 + r), k2, K2p);
-    __bf16* o1 = o0 + V3_NT * K2p;
-    *(bf16x2_t*)o0 = bf16x2_t{hr, -hi_};
-    *(bf16x2_t*)(o0 + V3_BK) = bf16x2_t{dr, -di};
-    *(bf16x2_t*)(o0 + 2 * V3_BK) = bf16x2_t{lr, -li};
-    *(bf16x2_t*)o1 = bf16x2_t{hi_, hr};
-    *(bf16x2_t*)(o1 + V3_BK) = bf16x2_t{di, dr};
-    *(bf16x2_t*)(o1 + 2 * V3_BK) = bf16x2_t{li, lr};
+  // real rows 2 n (Re, -Im per complex k) and 2 n + 1 (Im, Re)
+  for (int q = threadIdx.x; q < 128 * V3_CHUNKS; q += 256) {
+    const int rr = q / V3_CHUNKS, w = q % V3_CHUNKS;
+    const int kb = w / (2 * V3_NT), term = (w % (2 * V3_NT)) >> 1, half = w & 1;
+    const int64_t kbg = 2 * k0 / V3_BK + kb;
+    if (kbg * V3_BK >= K2p) continue;
+    const int r = rr >> 1;
+    const bool odd = rr & 1;
+    float x[8];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float2 v = t[r][8 * kb + 4 * half + j];
+      x[2 * j] = odd ? v.y : v.x;
+      x[2 * j + 1] = odd ? v.x : -v.y;
+    }
+    *(bf16x8_t*)(P + v3_chunk(2 * n0 + rr, kbg, term, half, K2p)) = term8(x, term);
   }
 }
 
