@@ -149,13 +149,15 @@ typedef struct {
    * pytenet/block_sparse_util.py:40-44) and MPO virtual bonds (Ml / Mr), with
    * the reference's sign conventions (A: q_l + q_s = q_r, pytenet/mps.py:71;
    * W[m, s', s, m']: q_ml + q_s' = q_s + q_mr, pytenet/mpo.py:71,154; L, R:
-   * q + q_m = q', pytenet/tdvp.py:62). With PTN_BLOCK_SPARSE, PTN_FP64 or PTN_FP32, an
-   * H_eff operator, batch 1 and sector-contiguous virtual bonds (every label
-   * in one run of indices, as the reference's QR and SVD splits leave them),
-   * the two products of H_eff (and of the environment steps, ptn_env_apply)
-   * run as one grouped launch each over the admissible sector blocks only (the block descriptors are built on the
-   * first call with a given set of labels and cached; that first call copies
-   * them to the device synchronously). Otherwise the tile-mask products run.
+   * q + q_m = q', pytenet/tdvp.py:62). With PTN_BLOCK_SPARSE, PTN_FP64,
+   * PTN_FP32 or PTN_BF16X6 (which takes the FP32 products here), an H_eff
+   * operator, batch 1 and sector-contiguous virtual bonds (every label in one
+   * run of indices, as the reference's QR and SVD splits leave them), the two
+   * products of H_eff (and of the environment steps, ptn_env_apply) run as
+   * one grouped launch each over the admissible sector blocks only (the block
+   * descriptors are built on the first call with a given set of labels and
+   * cached; that first call copies them to the device synchronously).
+   * Otherwise the tile-mask products run.
    * Results are the same either way: the grouped sums run over exactly the
    * nonzero terms. The labels are read during the call only. */
   const int64_t* q_l;

@@ -128,8 +128,8 @@ def heff_apply(a, w, l, r, precision="fp64", block_sparse=False):
 def heff_apply_sectors(a, w, l, r, qnums, precision="fp64"):
     """H_eff applied once through ptn_local_apply with U(1) quantum numbers
     qnums = (q_l, q_s, q_r, q_ml, q_mr): the sector-grouped products when
-    they apply (sector-contiguous virtual bonds, FP64 / FP32), else the
-    tile-mask products. Returns (out, grouped)."""
+    they apply (sector-contiguous virtual bonds; FP64, or FP32 for "fp32" and
+    "bf16x6"), else the tile-mask products. Returns (out, grouped)."""
     a, w, l, r = as_dev(a), as_dev(w), as_dev(l), as_dev(r)
     Dl, d, Dr = a.shape
     Ml, Mr = w.shape[0], w.shape[3]

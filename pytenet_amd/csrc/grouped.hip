@@ -592,6 +592,17 @@ static bool build_plan(int kind, const Dims& D, const int64_t* ql, const int64_t
          sp.p3.tiles < 0x7fffffff && !sp.mix.empty() && sp.mix_tiles < 0x7fffffff;
 }
 
+// arithmetic of the grouped products for a precision code: FP64 or FP32
+// (the three-term split-bf16 mode has FP32 accuracy and takes the FP32
+// products); -1 when they do not apply (two-term split-bf16: the tile-mask
+// products)
+static int grouped_prec(int prec) {
+  const int bp = prec & ~PTN_BLOCK_SPARSE;
+  if (bp == PTN_FP64) return PTN_FP64;
+  if (bp == PTN_FP32 || bp == PTN_BF16X6) return PTN_FP32;
+  return -1;
+}
+
 namespace {
 std::mutex g_plan_mu;
 // nullptr: not applicable. Allocated once and never destroyed, so that no
@@ -617,8 +628,8 @@ int sector_plan_get(const ptn_local_op* op, int kind, SectorPlanRef* out) {
   if (!op || op->kind != PTN_OP_HEFF || !(op->precision & PTN_BLOCK_SPARSE) || !op->q_l ||
       !op->q_s || !op->q_r || !op->q_ml || !op->q_mr)
     return PTN_OK;
-  const int bp = op->precision & ~PTN_BLOCK_SPARSE;
-  if (bp != PTN_FP64 && bp != PTN_FP32) return PTN_OK;   // split-bf16: the tile-mask products
+  const int bp = grouped_prec(op->precision);
+  if (bp < 0) return PTN_OK;
   if (noalloc_mode()) return PTN_OK;
   int dev = 0;
   PTN_HIP_CHECK(hipGetDevice(&dev));
@@ -707,8 +718,8 @@ static int grouped_launch(const GLaunch& g, const GProduct& P, hipStream_t s) {
 int heff_apply_sectors(int prec, const Dims& D, const SectorPlan* sp, const void* l, int tl,
                        const void* w, const void* r, int tr, const void* a, void* out, void* ws,
                        size_t ws_bytes, hipStream_t s) {
-  const int bp = prec & ~PTN_BLOCK_SPARSE;
-  if (!sp || (bp != PTN_FP64 && bp != PTN_FP32)) return PTN_ERR_UNSUPPORTED;
+  const int bp = grouped_prec(prec);
+  if (!sp || bp < 0) return PTN_ERR_UNSUPPORTED;
   const int ct = bp == PTN_FP64 ? PTN_T_C128 : PTN_T_C64;
   if (tl != ct || tr != ct) return PTN_ERR_UNSUPPORTED;
   if (ws_bytes < heff_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;
@@ -753,9 +764,8 @@ int heff_apply_sectors(int prec, const Dims& D, const SectorPlan* sp, const void
 int env_right_sectors(int prec, const Dims& D, const SectorPlan* sp, const void* a, const void* b,
                       const void* w, const void* r, int tr, void* out, void* ws, size_t ws_bytes,
                       hipStream_t s) {
-  const int bp = prec & ~PTN_BLOCK_SPARSE;
-  if (!sp || sp->kind != PLAN_ENV_RIGHT || (bp != PTN_FP64 && bp != PTN_FP32))
-    return PTN_ERR_UNSUPPORTED;
+  const int bp = grouped_prec(prec);
+  if (!sp || sp->kind != PLAN_ENV_RIGHT || bp < 0) return PTN_ERR_UNSUPPORTED;
   const int ct = bp == PTN_FP64 ? PTN_T_C128 : PTN_T_C64;
   if (tr != ct) return PTN_ERR_UNSUPPORTED;
   if (ws_bytes < env_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;
@@ -800,9 +810,8 @@ int env_right_sectors(int prec, const Dims& D, const SectorPlan* sp, const void*
 int env_left_sectors(int prec, const Dims& D, const SectorPlan* sp, const void* a, const void* b,
                      const void* w, const void* l, int tl, void* out, void* ws, size_t ws_bytes,
                      hipStream_t s) {
-  const int bp = prec & ~PTN_BLOCK_SPARSE;
-  if (!sp || sp->kind != PLAN_ENV_LEFT || (bp != PTN_FP64 && bp != PTN_FP32))
-    return PTN_ERR_UNSUPPORTED;
+  const int bp = grouped_prec(prec);
+  if (!sp || sp->kind != PLAN_ENV_LEFT || bp < 0) return PTN_ERR_UNSUPPORTED;
   const int ct = bp == PTN_FP64 ? PTN_T_C128 : PTN_T_C64;
   if (tl != ct) return PTN_ERR_UNSUPPORTED;
   if (ws_bytes < env_ws_bytes(prec, D)) return PTN_ERR_WORKSPACE;

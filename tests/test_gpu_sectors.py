@@ -24,7 +24,7 @@ from pytenet_amd.hamiltonian import heisenberg_xxz_1d_mpo
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp64": 1e-12, "fp32": 2e-5}
+TOL = {"fp64": 1e-12, "fp32": 2e-5, "bf16x6": 2e-5}
 
 
 def relerr(got, ref):
@@ -66,7 +66,7 @@ def _operator(rng, Dl, Dr, two_site, ql=None, qr=None, spread=4):
 
 @pytest.mark.parametrize("Dl,Dr,two_site", [(200, 180, False), (260, 300, True), (37, 41, True),
                                             (1, 6, True), (64, 1, False)])
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x6"])
 def test_grouped_heff_matches_oracle_and_dense(Dl, Dr, two_site, precision):
     rng = np.random.default_rng(Dl * 7 + Dr)
     if Dl == 1:
@@ -90,7 +90,7 @@ def test_grouped_heff_matches_oracle_and_dense(Dl, Dr, two_site, precision):
 
 @pytest.mark.parametrize("Dl,Dr,two_site", [(200, 180, False), (260, 300, True), (37, 41, True),
                                             (1, 6, True), (64, 1, False)])
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x6"])
 def test_grouped_environment_steps_match_oracle_and_dense(Dl, Dr, two_site, precision):
     """ptn_env_apply (both sides) with labels: grouped products, equal to the
     oracle's contraction_operator_step_left / _right (pytenet/chain_ops.py:16-99)
@@ -230,16 +230,18 @@ def test_u1_dmrg_runs_the_grouped_products_and_matches_the_reference_trajectory(
             assert any(n.startswith("zgemm_grouped_kernel") for n in names), names
 
 
-def test_grouped_lanczos_fp32_matches_oracle():
+@pytest.mark.parametrize("precision", ["fp32", "bf16x6"])
+def test_grouped_lanczos_fp32_matches_oracle(precision):
     """FP32 matvecs through the grouped products (complex64 environment blocks,
-    FP64 Lanczos scalars): the expm result against the oracle's FP64 Lanczos
+    FP64 Lanczos scalars; the three-term split-bf16 mode takes the same FP32
+    grouped products): the expm result against the oracle's FP64 Lanczos
     within the FP32 bound."""
     rng = np.random.default_rng(17)
     a, w, l, r, q = _chain_operator(rng, True)
     hfun = lambda x: orc.apply_local_hamiltonian(x.reshape(a.shape), w, l, r).reshape(-1)
     dt = -0.05j
     got, _, info = backend.krylov_local(_lib.PTN_OP_HEFF, a, l, w, r, 8, "expm", dt,
-                                        precision="fp32", block_sparse=True, qnums=q)
+                                        precision=precision, block_sparse=True, qnums=q)
     ref = orc.expm_krylov(hfun, a.reshape(-1), dt, 8, hermitian=True).reshape(a.shape)
     assert relerr(got, ref) < TOL["fp32"]
 
