@@ -285,10 +285,25 @@ __device__ __forceinline__ void split1(float x, __bf16& hi, __bf16& lo) {
   lo = (__bf16)(x - (float)hi);
 }
 
+// plane `plane` (0 hi, 1 lo) of eight values
+__device__ __forceinline__ bf16x8_t plane8(const float (&x)[8], int plane) {
+  bf16x8_t o;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    __bf16 h, l;
+    split1(x[e], h, l);
+    o[e] = plane ? l : h;
+  }
+  return o;
+}
+
 // A (M x K complex, element (m, k) at A[m a_m + k a_k], conjugated if conj)
 // -> A_r hi / lo planes. One workgroup per 64 x 32 complex block, staged
 // through LDS so that both the reads (along the operand's contiguous index)
-// and the writes (along k) are coalesced; problem blockIdx.z at A + z bz.
+// and the writes (along k) are coalesced: each store is a 16-byte chunk
+// (eight real k of one plane), a row's 64 real k of a plane being eight
+// consecutive chunks written by consecutive threads; problem blockIdx.z at
+// A + z bz.
 template <typename TA>
 __global__ __launch_bounds__(256) void k_split_a(int64_t M, int64_t K, const TA* __restrict__ A,
                                                  int64_t a_m, int64_t a_k, int conj, int64_t bz,
@@ -308,17 +323,18 @@ __global__ __launch_bounds__(256) void k_split_a(int64_t M, int64_t K, const TA*
     t[r][c] = make_float2(re, conj ? -im : im);
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
-    const int r = e / 32, c = e % 32;
-    const int64_t k2 = 2 * (k0 + c);
+  for (int q = threadIdx.x; q < 64 * 16; q += 256) {
+    const int r = q / 16, plane = (q % 16) >> 3, c8 = q % 8;
+    const int64_t k2 = 2 * k0 + 8 * c8;   // first real k of the chunk
     if (k2 >= K2p) continue;
-    const float2 v = t[r][c];
-    __bf16 h0, l0, h1, l1;
-    split1(v.x, h0, l0);
-    split1(v.y, h1, l1);
-    const int64_t o = (m0 + r) * K2p + k2;
-    hi[o] = h0; hi[o + 1] = h1;
-    lo[o] = l0; lo[o + 1] = l1;
+    float x[8];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float2 v = t[r][4 * c8 + j];
+      x[2 * j] = v.x;
+      x[2 * j + 1] = v.y;
+    }
+    *(bf16x8_t*)((plane ? lo : hi) + (m0 + r) * K2p + k2) = plane8(x, plane);
   }
 }
 
@@ -342,18 +358,22 @@ __global__ __launch_bounds__(256) void k_split_b(int64_t K, int64_t N, const TB*
     t[r][c] = make_float2(re, conj ? -im : im);
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
-    const int r = e / 32, c = e % 32;
-    const int64_t k2 = 2 * (k0 + c);
+  // real rows 2 n (Re, -Im per complex k) and 2 n + 1 (Im, Re), 16-byte chunks
+  // as in k_split_a
+  for (int q = threadIdx.x; q < 128 * 16; q += 256) {
+    const int rr = q / 16, plane = (q % 16) >> 3, c8 = q % 8;
+    const int64_t k2 = 2 * k0 + 8 * c8;
     if (k2 >= K2p) continue;
-    const float2 v = t[r][c];
-    __bf16 hr, lr, hi_, li_, hmi, lmi;
-    split1(v.x, hr, lr);
-    split1(v.y, hi_, li_);
-    split1(-v.y, hmi, lmi);
-    const int64_t o0 = (2 * (n0 + r)) * K2p + k2, o1 = o0 + K2p;
-    hi[o0] = hr; hi[o0 + 1] = hmi; hi[o1] = hi_; hi[o1 + 1] = hr;
-    lo[o0] = lr; lo[o0 + 1] = lmi; lo[o1] = li_; lo[o1 + 1] = lr;
+    const int r = rr >> 1;
+    const bool odd = rr & 1;
+    float x[8];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float2 v = t[r][4 * c8 + j];
+      x[2 * j] = odd ? v.y : v.x;
+      x[2 * j + 1] = odd ? v.x : -v.y;
+    }
+    *(bf16x8_t*)((plane ? lo : hi) + (2 * n0 + rr) * K2p + k2) = plane8(x, plane);
   }
 }
 
