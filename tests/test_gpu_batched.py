@@ -38,7 +38,7 @@ def test_qr_batched_matches_single(nb, m, n):
         assert np.abs(r[b] - r0).max() < 1e-12 * max(1.0, np.abs(r0).max())
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("fp32", 2e-5)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("fp32", 2e-5), ("bf16x6", 2e-5)])
 def test_krylov_batched_matches_single(precision, tol):
     rng = np.random.default_rng(11)
     nb, Dl, d, Dr, M = 6, 10, 2, 12, 3
@@ -86,6 +86,29 @@ def test_env_batched_matches_single():
     for b in range(nb):
         assert np.abs(el[b] - orc.contraction_operator_step_left(a[b], a[b], w[b], l[b])).max() < 1e-12 * np.abs(el[b]).max()
         assert np.abs(er[b] - orc.contraction_operator_step_right(a[b], a[b], w[b], r[b])).max() < 1e-12 * np.abs(er[b]).max()
+
+
+def test_env_batched_three_term_split_bf16_matches_oracle():
+    """Batched environment steps large enough for the pre-split three-term
+    split-bf16 product (two problems of D = 512: the problem index on the grid's
+    third axis, the term planes of each problem at their own stride), per
+    problem against the oracle at the FP32 bound."""
+    rng = np.random.default_rng(8)
+    nb, D, d, M = 2, 512, 2, 5
+    a = np.stack([orc.crandn((D, d, D), rng) / D for _ in range(nb)])
+    w = np.stack([orc.crandn((M, d, d, M), rng) for _ in range(nb)])
+    l = np.stack([orc.crandn((D, M, D), rng) / D for _ in range(nb)])
+    r = np.stack([orc.crandn((D, M, D), rng) / D for _ in range(nb)])
+    backend.timing_begin()
+    el = backend.to_host(backend.env_left_batched(a, w, l, "bf16x6"))
+    er = backend.to_host(backend.env_right_batched(a, w, r, "bf16x6"))
+    names = [t["name"] for t in backend.timing_end()]
+    assert any(n.startswith("zgemm_bf16x6_v3<") for n in names), names
+    for b in range(nb):
+        refl = orc.contraction_operator_step_left(a[b], a[b], w[b], l[b])
+        refr = orc.contraction_operator_step_right(a[b], a[b], w[b], r[b])
+        assert np.abs(el[b] - refl).max() < 2e-5 * np.abs(refl).max()
+        assert np.abs(er[b] - refr).max() < 2e-5 * np.abs(refr).max()
 
 
 def _ising_chains(nchains, L, Ds, seed):
