@@ -32,6 +32,7 @@ __device__ __forceinline__ void store_c(double2* p, double re, double im) { *p =
 __device__ __forceinline__ void store_c(float2* p, double re, double im) {
   *p = make_float2((float)re, (float)im);
 }
+__device__ __forceinline__ void store_c(float2* p, float re, float im) { *p = make_float2(re, im); }
 
 #define PTN_HIP_CHECK(expr)                                  \
   do {                                                       \

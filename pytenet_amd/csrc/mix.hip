@@ -7,11 +7,15 @@ namespace ptn {
 // Mode mix: Y[l, o, r] = sum_i Wm[o, i] X[l, i, r] for the small MPO index
 // pairs (ni, no <= 32). One thread per (l, r); the complex mixing matrix lives
 // in LDS and its zero entries (most of an XXZ/Ising MPO) are skipped with a
-// workgroup-uniform branch. HBM-bound: reads ni and writes no complex values per (l, r).
+// workgroup-uniform branch. HBM-bound: reads ni and writes no complex values
+// per (l, r), so the bytes in flight per CU set its speed: complex64 operands
+// (the reduced-precision modes, whose products round T1 to FP32 anyway) are
+// mixed in FP32 arithmetic (A = float), which halves the registers of the
+// thread's ni inputs and doubles the waves per SIMD.
 constexpr int MIX_MAX = 32;
 
 // blockIdx.z selects the problem of a batch (strides in complex elements).
-template <typename TX, typename TY>
+template <typename TX, typename TY, typename A>
 __global__ __launch_bounds__(256) void mode_mix_kernel(int64_t nl, int ni, int no, int64_t nr,
                                                        const double2* __restrict__ wm,
                                                        const TX* __restrict__ x,
@@ -29,21 +33,22 @@ __global__ __launch_bounds__(256) void mode_mix_kernel(int64_t nl, int ni, int n
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t l = blockIdx.y;
   if (r >= nr) return;
-  double xr[MIX_MAX], xi[MIX_MAX];
+  A xr[MIX_MAX], xi[MIX_MAX];
 #pragma unroll
   for (int i = 0; i < MIX_MAX; ++i) {
     xr[i] = 0; xi[i] = 0;
-    if (i < ni) load_c<double>(x + (l * ni + i) * nr + r, xr[i], xi[i]);
+    if (i < ni) load_c<A>(x + (l * ni + i) * nr + r, xr[i], xi[i]);
   }
   for (int o = 0; o < no; ++o) {
-    double yr = 0, yi = 0;
+    A yr = 0, yi = 0;
 #pragma unroll
     for (int i = 0; i < MIX_MAX; ++i) {
       if (i < ni) {
-        double2 w = W[o * ni + i];
+        const double2 w = W[o * ni + i];
         if (w.x != 0.0 || w.y != 0.0) {
-          yr += w.x * xr[i] - w.y * xi[i];
-          yi += w.x * xi[i] + w.y * xr[i];
+          const A wr = (A)w.x, wi = (A)w.y;
+          yr += wr * xr[i] - wi * xi[i];
+          yi += wr * xi[i] + wi * xr[i];
         }
       }
     }
@@ -60,21 +65,21 @@ int mode_mix(int64_t nl, int64_t ni, int64_t no, int64_t nr, const void* wm, con
   const int64_t bx = nl * ni * nr, by = nl * no * nr;
   const SkipFlag sk = skip_flag();
   if (tx == PTN_T_C128 && ty == PTN_T_C128)
-    mode_mix_kernel<double2, double2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
-                                                           (const double2*)wm, (const double2*)x,
-                                                           (double2*)y, bw, bx, by, sk.p, sk.stride);
+    mode_mix_kernel<double2, double2, double><<<grid, 256, 0, s>>>(
+        nl, (int)ni, (int)no, nr, (const double2*)wm, (const double2*)x, (double2*)y, bw, bx, by,
+        sk.p, sk.stride);
   else if (tx == PTN_T_C64 && ty == PTN_T_C64)
-    mode_mix_kernel<float2, float2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
-                                                         (const double2*)wm, (const float2*)x,
-                                                         (float2*)y, bw, bx, by, sk.p, sk.stride);
+    mode_mix_kernel<float2, float2, float><<<grid, 256, 0, s>>>(
+        nl, (int)ni, (int)no, nr, (const double2*)wm, (const float2*)x, (float2*)y, bw, bx, by,
+        sk.p, sk.stride);
   else if (tx == PTN_T_C128 && ty == PTN_T_C64)
-    mode_mix_kernel<double2, float2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
-                                                          (const double2*)wm, (const double2*)x,
-                                                          (float2*)y, bw, bx, by, sk.p, sk.stride);
+    mode_mix_kernel<double2, float2, double><<<grid, 256, 0, s>>>(
+        nl, (int)ni, (int)no, nr, (const double2*)wm, (const double2*)x, (float2*)y, bw, bx, by,
+        sk.p, sk.stride);
   else
-    mode_mix_kernel<float2, double2><<<grid, 256, 0, s>>>(nl, (int)ni, (int)no, nr,
-                                                          (const double2*)wm, (const float2*)x,
-                                                          (double2*)y, bw, bx, by, sk.p, sk.stride);
+    mode_mix_kernel<float2, double2, double><<<grid, 256, 0, s>>>(
+        nl, (int)ni, (int)no, nr, (const double2*)wm, (const float2*)x, (double2*)y, bw, bx, by,
+        sk.p, sk.stride);
   PTN_LAUNCH_CHECK();
   return PTN_OK;
 }
