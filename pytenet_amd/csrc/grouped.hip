@@ -247,7 +247,10 @@ static int zero_fill(int64_t n, void* p, hipStream_t s) {
 // only for the outputs o with qml[m] + qs[s'] = Q - P whose output sector
 // P + qml[m] exists. The mix reads and writes just those: T1 needs no zero
 // fill and T2 no zeros outside what is read. One wave per row l and 64-column
-// chunk of a unit, the lanes over the columns r' (coalesced).
+// chunk of a unit, the lanes over the columns r' (coalesced). The kernel is
+// bound by the bytes in flight per CU: a lane's inputs are held in registers
+// sized to the plan's longest input list (CAP = 8, 16 or 32), and complex64
+// operands are mixed in FP32 arithmetic (A = float).
 constexpr int MIX_LIST = 32;   // d * Mr and Ml * d are at most 32
 enum { PLAN_HEFF = SECTORS_HEFF, PLAN_ENV_RIGHT = SECTORS_ENV_RIGHT, PLAN_ENV_LEFT = SECTORS_ENV_LEFT };
 
@@ -259,7 +262,7 @@ struct MixUnit {
   int8_t il[MIX_LIST], ol[MIX_LIST];
 };
 
-template <typename TX, typename TY>
+template <typename TX, typename TY, typename A, int CAP>
 __global__ __launch_bounds__(256) void k_mix_sectors(const MixUnit* __restrict__ units, int nunits,
                                                      int ni_all, int no_all, int64_t nr_all,
                                                      const double2* __restrict__ wm,
@@ -289,22 +292,23 @@ __global__ __launch_bounds__(256) void k_mix_sectors(const MixUnit* __restrict__
   const TX* __restrict__ xl = x + l * ni_all * nr_all;
   TY* __restrict__ yl = y + l * no_all * nr_all;
   {
-    double xr[MIX_LIST], xi[MIX_LIST];
+    A xr[CAP], xi[CAP];
 #pragma unroll
-    for (int k = 0; k < MIX_LIST; ++k) {
+    for (int k = 0; k < CAP; ++k) {
       xr[k] = 0; xi[k] = 0;
-      if (k < ni) load_c<double>(xl + (int64_t)u->il[k] * nr_all + r, xr[k], xi[k]);
+      if (k < ni) load_c<A>(xl + (int64_t)u->il[k] * nr_all + r, xr[k], xi[k]);
     }
     for (int j = 0; j < no; ++j) {
       const int o = u->ol[j];
-      double yr = 0, yi = 0;
+      A yr = 0, yi = 0;
 #pragma unroll
-      for (int k = 0; k < MIX_LIST; ++k) {
+      for (int k = 0; k < CAP; ++k) {
         if (k < ni) {
           // Wm[o, i], or stored transposed as W[i, o] (left environment)
           const double2 w = wtrans ? W[u->il[k] * no_all + o] : W[o * ni_all + u->il[k]];
-          yr += w.x * xr[k] - w.y * xi[k];
-          yi += w.x * xi[k] + w.y * xr[k];
+          const A wr = (A)w.x, wi = (A)w.y;
+          yr += wr * xr[k] - wi * xi[k];
+          yi += wr * xi[k] + wi * xr[k];
         }
       }
       store_c(yl + (int64_t)o * nr_all + r, yr, yi);
@@ -357,6 +361,7 @@ struct SectorPlan {
   std::vector<MixUnit> mix;
   int64_t mix_tiles = 0;
   double mix_bytes = 0.0;   // T1 entries read and T2 entries written by the mix (elements)
+  int mix_cap = MIX_LIST;   // register capacity for a unit's inputs (8, 16 or 32)
   GBlock* dev = nullptr;    // p1's blocks, then p3's, then the mix units
   MixUnit* dev_mix = nullptr;
   int device = 0;
@@ -588,6 +593,9 @@ static bool build_plan(int kind, const Dims& D, const int64_t* ql, const int64_t
     }
     finish_product(sp.p1, 16, csz, csz);   // A complex128, R and T1 in the compute type
   }
+  int maxni = 0;
+  for (const MixUnit& u : sp.mix) maxni = u.ni > maxni ? u.ni : maxni;
+  sp.mix_cap = maxni <= 8 ? 8 : (maxni <= 16 ? 16 : MIX_LIST);
   return sp.p1.tiles > 0 && sp.p3.tiles > 0 && sp.p1.tiles < 0x7fffffff &&
          sp.p3.tiles < 0x7fffffff && !sp.mix.empty() && sp.mix_tiles < 0x7fffffff;
 }
@@ -711,6 +719,38 @@ static int grouped_launch(const GLaunch& g, const GProduct& P, hipStream_t s) {
   return PTN_OK;
 }
 
+// the sector mix of a plan: T2 (y) from T1 (x) with the MPO matrix w (complex128),
+// complex128 operands in FP64 or complex64 in FP32 arithmetic; wtrans: w stored
+// transposed (left environment)
+template <int CAP>
+static void mix_launch_cap(const SectorPlan* sp, bool fp64, int ni, int no, int64_t nr,
+                           const void* w, const void* x, void* y, const int* skip, int wtrans,
+                           hipStream_t s) {
+  const dim3 grid((unsigned)sp->mix_tiles);
+  const MixUnit* units = sp->dev_mix;
+  const int nu = (int)sp->mix.size();
+  if (fp64)
+    tlaunch("k_mix_sectors<c128, c128>", 0.0, sp->mix_bytes * 16,
+            k_mix_sectors<double2, double2, double, CAP>, grid, dim3(256), s, units, nu, ni, no,
+            nr, (const double2*)w, (const double2*)x, (double2*)y, skip, wtrans);
+  else
+    tlaunch("k_mix_sectors<c64, c64>", 0.0, sp->mix_bytes * 8,
+            k_mix_sectors<float2, float2, float, CAP>, grid, dim3(256), s, units, nu, ni, no, nr,
+            (const double2*)w, (const float2*)x, (float2*)y, skip, wtrans);
+}
+
+static int mix_launch(const SectorPlan* sp, bool fp64, int ni, int no, int64_t nr, const void* w,
+                      const void* x, void* y, const int* skip, int wtrans, hipStream_t s) {
+  if (sp->mix_cap <= 8)
+    mix_launch_cap<8>(sp, fp64, ni, no, nr, w, x, y, skip, wtrans, s);
+  else if (sp->mix_cap <= 16)
+    mix_launch_cap<16>(sp, fp64, ni, no, nr, w, x, y, skip, wtrans, s);
+  else
+    mix_launch_cap<MIX_LIST>(sp, fp64, ni, no, nr, w, x, y, skip, wtrans, s);
+  PTN_LAUNCH_CHECK();
+  return PTN_OK;
+}
+
 // H_eff through the grouped products: out = L^T . (W (x) (A . R)) as
 // heff_apply computes it (same operand types: A and out complex128, L and R
 // complex128 in FP64 and complex64 in FP32), only the blocks computed and out
@@ -734,19 +774,14 @@ int heff_apply_sectors(int prec, const Dims& D, const SectorPlan* sp, const void
   GLaunch g1{b1, (int)sp->p1.blocks.size(), D.d * D.Dr, 1, D.Mr * D.Dr, 1, D.d * D.Mr * D.Dr, 1,
              a, r, t1, skip, 0};
   const int ni = (int)(D.d * D.Mr), no = (int)(D.Ml * D.d);
-  const dim3 mgrid((unsigned)sp->mix_tiles);
   if (bp == PTN_FP64) {
     if ((st = grouped_launch<double, double2, double2, double2, true, true>(g1, sp->p1, s)))
       return st;
-    tlaunch("k_mix_sectors<c128, c128>", 0.0, sp->mix_bytes * 16, k_mix_sectors<double2, double2>,
-            mgrid, dim3(256), s, (const MixUnit*)sp->dev_mix, (int)sp->mix.size(), ni, no, D.Dr,
-            (const double2*)w, (const double2*)t1, (double2*)t2, skip, 0);
+    if ((st = mix_launch(sp, true, ni, no, D.Dr, w, t1, t2, skip, 0, s))) return st;
   } else {
     if ((st = grouped_launch<float, double2, float2, float2, true, true>(g1, sp->p1, s)))
       return st;
-    tlaunch("k_mix_sectors<c64, c64>", 0.0, sp->mix_bytes * 8, k_mix_sectors<float2, float2>,
-            mgrid, dim3(256), s, (const MixUnit*)sp->dev_mix, (int)sp->mix.size(), ni, no, D.Dr,
-            (const double2*)w, (const float2*)t1, (float2*)t2, skip, 0);
+    if ((st = mix_launch(sp, false, ni, no, D.Dr, w, t1, t2, skip, 0, s))) return st;
   }
   PTN_LAUNCH_CHECK();
   if ((st = zero_fill<double2>(nout, out, s))) return st;
@@ -778,7 +813,6 @@ int env_right_sectors(int prec, const Dims& D, const SectorPlan* sp, const void*
   const GBlock* b3 = sp->dev + sp->p1.blocks.size();
   const int* skip = skip_flag().p;
   const int ni = (int)(D.d * D.Mr), no = (int)(D.Ml * D.d);
-  const dim3 mgrid((unsigned)sp->mix_tiles);
   int st;
   GLaunch g1{b1, (int)sp->p1.blocks.size(), D.d * D.Dr, 1, D.Mr * D.Dr, 1, D.d * D.Mr * D.Dr, 1,
              a, r, t1, skip, 0};
@@ -788,18 +822,12 @@ int env_right_sectors(int prec, const Dims& D, const SectorPlan* sp, const void*
   if (bp == PTN_FP64) {
     if ((st = grouped_launch<double, double2, double2, double2, true, true>(g1, sp->p1, s)))
       return st;
-    tlaunch("k_mix_sectors<c128, c128>", 0.0, sp->mix_bytes * 16, k_mix_sectors<double2, double2>,
-            mgrid, dim3(256), s, (const MixUnit*)sp->dev_mix, (int)sp->mix.size(), ni, no, D.Dr,
-            (const double2*)w, (const double2*)t1, (double2*)t2, skip, 0);
-    PTN_LAUNCH_CHECK();
+    if ((st = mix_launch(sp, true, ni, no, D.Dr, w, t1, t2, skip, 0, s))) return st;
     if ((st = zero_fill<double2>(D.Dl * D.Ml * D.Dl, out, s))) return st;
     return grouped_launch<double, double2, double2, double2, true, false>(g3, sp->p3, s);
   }
   if ((st = grouped_launch<float, double2, float2, float2, true, true>(g1, sp->p1, s))) return st;
-  tlaunch("k_mix_sectors<c64, c64>", 0.0, sp->mix_bytes * 8, k_mix_sectors<float2, float2>, mgrid,
-          dim3(256), s, (const MixUnit*)sp->dev_mix, (int)sp->mix.size(), ni, no, D.Dr,
-          (const double2*)w, (const float2*)t1, (float2*)t2, skip, 0);
-  PTN_LAUNCH_CHECK();
+  if ((st = mix_launch(sp, false, ni, no, D.Dr, w, t1, t2, skip, 0, s))) return st;
   if ((st = zero_fill<double2>(D.Dl * D.Ml * D.Dl, out, s))) return st;
   return grouped_launch<float, float2, double2, double2, true, false>(g3, sp->p3, s);
 }
@@ -824,7 +852,6 @@ int env_left_sectors(int prec, const Dims& D, const SectorPlan* sp, const void* 
   const GBlock* b3 = sp->dev + sp->p1.blocks.size();
   const int* skip = skip_flag().p;
   const int ni = (int)(D.Ml * D.d), no = (int)(D.d * D.Mr);
-  const dim3 mgrid((unsigned)sp->mix_tiles);
   int st;
   // T1[(l, m), (s', r')] = sum_lb L[(l, m), lb] conj(B[lb, (s', r')])
   GLaunch g1{b1, (int)sp->p1.blocks.size(), D.Ml * D.Dl, 1, D.d * D.Dr, 1, D.Ml * D.d * D.Dr, 1,
@@ -835,18 +862,12 @@ int env_left_sectors(int prec, const Dims& D, const SectorPlan* sp, const void* 
   if (bp == PTN_FP64) {
     if ((st = grouped_launch<double, double2, double2, double2, true, true>(g1, sp->p1, s)))
       return st;
-    tlaunch("k_mix_sectors<c128, c128>", 0.0, sp->mix_bytes * 16, k_mix_sectors<double2, double2>,
-            mgrid, dim3(256), s, (const MixUnit*)sp->dev_mix, (int)sp->mix.size(), ni, no, D.Dr,
-            (const double2*)w, (const double2*)t1, (double2*)t2, skip, 1);
-    PTN_LAUNCH_CHECK();
+    if ((st = mix_launch(sp, true, ni, no, D.Dr, w, t1, t2, skip, 1, s))) return st;
     if ((st = zero_fill<double2>(D.Dr * D.Mr * D.Dr, out, s))) return st;
     return grouped_launch<double, double2, double2, double2, false, true>(g3, sp->p3, s);
   }
   if ((st = grouped_launch<float, float2, double2, float2, true, true>(g1, sp->p1, s))) return st;
-  tlaunch("k_mix_sectors<c64, c64>", 0.0, sp->mix_bytes * 8, k_mix_sectors<float2, float2>, mgrid,
-          dim3(256), s, (const MixUnit*)sp->dev_mix, (int)sp->mix.size(), ni, no, D.Dr,
-          (const double2*)w, (const float2*)t1, (float2*)t2, skip, 1);
-  PTN_LAUNCH_CHECK();
+  if ((st = mix_launch(sp, false, ni, no, D.Dr, w, t1, t2, skip, 1, s))) return st;
   if ((st = zero_fill<double2>(D.Dr * D.Mr * D.Dr, out, s))) return st;
   return grouped_launch<float, double2, float2, double2, false, true>(g3, sp->p3, s);
 }
