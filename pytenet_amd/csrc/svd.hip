@@ -280,7 +280,7 @@ __global__ __launch_bounds__(256) void k_jac_solve(int nblk, const typename JacT
     T sr[NE], si[NE];
 #pragma unroll
     for (int u = 0; u < NE; ++u) { sr[u] = 0; si[u] = 0; }
-#pragma unroll 4
+#pragma unroll 8
     for (int s = 0; s < nseg; ++s) {
 #pragma unroll
       for (int u = 0; u < NE; ++u) {
@@ -372,17 +372,27 @@ __global__ __launch_bounds__(256) void k_jac_apply(int64_t p, int64_t qp, int nb
   __shared__ C2 J[JP][JP + 1];
   const int64_t prob = nmat == 2 ? (blockIdx.z >> 1) : blockIdx.z;
   const int64_t pidx = prob * gridDim.y + blockIdx.y;
-  if (!rotflag[pidx]) return;
   const int pass = nmat == 2 ? (blockIdx.z & 1) : 0;
   C2* M = pass ? V + prob * qp * qp : X + prob * p * qp;
   const int64_t ld = pass ? qp : p;
   const int64_t rowblk = (int64_t)blockIdx.x * AROWS;
   if (rowblk >= ld) return;
+  const int t = threadIdx.x;
+  // the rotation and the pair's flag are loaded together (J is in bounds
+  // whether or not the pair rotated; it is used only if it did)
+  const C2* jsrc = Jg + pidx * JP * JP;
+  constexpr int NJ = JP * JP / 256;
+  C2 jv[NJ];
+#pragma unroll
+  for (int u = 0; u < NJ; ++u) jv[u] = jsrc[t + 256 * u];
+  if (!rotflag[pidx]) return;
   int bp, bq;
   round_pair(nblk, r, blockIdx.y, bp, bq);
-  const int t = threadIdx.x;
-  const C2* jsrc = Jg + pidx * JP * JP;
-  for (int e = t; e < JP * JP; e += 256) J[e / JP][e % JP] = jsrc[e];
+#pragma unroll
+  for (int u = 0; u < NJ; ++u) {
+    const int e = t + 256 * u;
+    J[e / JP][e % JP] = jv[u];
+  }
   __syncthreads();
   const int lane = t & 63, wave = t >> 6;
   const int l16 = lane & 15, kk = lane >> 4;
