@@ -624,10 +624,10 @@ size_t bf16x3_split_bytes(int64_t M, int64_t N, int64_t K, int64_t batch) {
 // (each difference is exact in FP32, and the three terms hold the whole 24-bit
 // significand), and a product is formed from the six leading terms
 //   a b ~ a_lo b_hi + a_hi b_lo + a_mid b_mid + a_mid b_hi + a_hi b_mid + a_hi b_hi
-// (smallest first; the dropped a_mid b_lo, a_lo b_mid, a_lo b_lo are below
-// 2^-25 |a b|), accumulated in FP32: the accuracy of an FP32 product on the
-// bf16 MFMA, with twice the MFMA work of bf16x3 and 1.5 times its operand
-// bytes. The three terms of a value are stored interleaved per 16 real k
+// (smallest first; the dropped a_mid b_lo, a_lo b_mid, a_lo b_lo are at most
+// 2^-23 |a b|: |mid| <= 2^-8 |x|, |lo| <= 2^-16 |x|), accumulated in FP32: the
+// accuracy of an FP32 product on the bf16 MFMA, with twice the MFMA work of
+// bf16x3 and 1.5 times its operand bytes. The three terms of a value are stored interleaved per 16 real k
 // ([row][k / 16][term][k % 16]), so that one k-stage of a row is one
 // contiguous 96-byte read. Products too small for the pre-split kernel run on
 // the FP32 MFMA (the same accuracy).

@@ -126,3 +126,44 @@ def test_local_op_carries_the_site_labels_only_when_they_fit():
     # an environment step describes its site with only one of the two blocks
     op, keep = backend._local_op(_lib.PTN_OP_HEFF, "fp32", l, w, None, Dl, d, Dr, True, q)
     assert (op.Ml, op.Mr) == (Ml, Mr) and not op.r and op.q_mr
+
+
+def _bf16(x):
+    """Round float32 values to bfloat16 (nearest, ties to even), returned as float32."""
+    b = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+    return b.astype(np.uint32).view(np.float32)
+
+
+def test_split_bf16_terms_are_exact_and_bound_the_dropped_products():
+    """The operand splits of the split-bf16 products (gemm_bf16.hip), emulated
+    with exact bf16 rounding: hi + mid + lo reproduces every FP32 value
+    exactly (each difference is exact in FP32), the six-term product drops at
+    most 2^-23 of a b (|mid| <= 2^-8 |x|, |lo| <= 2^-16 |x|), and the two-term
+    split, which drops a_lo b_lo and the representation residues, is about a
+    thousand times less accurate at the median."""
+    rng = np.random.default_rng(5)
+    n = 200_000
+    x = (rng.standard_normal(n) * np.exp(rng.uniform(-20, 20, n))).astype(np.float32)
+    y = (rng.standard_normal(n) * np.exp(rng.uniform(-20, 20, n))).astype(np.float32)
+
+    def split3(v):
+        h = _bf16(v)
+        r = (v - h).astype(np.float32)
+        m = _bf16(r)
+        return h, m, _bf16((r - m).astype(np.float32))
+
+    hx, mx, lx = [t.astype(np.float64) for t in split3(x)]
+    hy, my, ly = [t.astype(np.float64) for t in split3(y)]
+    assert np.array_equal(hx + mx + lx, x.astype(np.float64))
+    assert np.all(np.abs(mx) <= 2.0**-8 * np.abs(x)) and np.all(np.abs(lx) <= 2.0**-16 * np.abs(x))
+    ab = x.astype(np.float64) * y
+    six = lx * hy + hx * ly + mx * my + mx * hy + hx * my + hx * hy
+    rel6 = np.abs(ab - six) / np.abs(ab)
+    assert rel6.max() <= 2.0**-23
+    h2x = _bf16(x).astype(np.float64)
+    l2x = _bf16((x - _bf16(x)).astype(np.float32)).astype(np.float64)
+    h2y = _bf16(y).astype(np.float64)
+    l2y = _bf16((y - _bf16(y)).astype(np.float32)).astype(np.float64)
+    rel3 = np.abs(ab - (h2x * h2y + h2x * l2y + l2x * h2y)) / np.abs(ab)
+    assert np.median(rel3) > 500 * np.median(rel6)
